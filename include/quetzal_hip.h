@@ -1,0 +1,137 @@
+/*
+ * quetzal_hip.h -- C-ABI of the MI355X-native quasistatic slice loop.
+ *
+ * Plain C, plain pointers and sizes.  Every entry point returns an int status
+ * (QP_OK = 0, QP_STOPPED = 1 for the reference's soft stop, < 0 errors) and
+ * leaves a message for qp_last_error() (thread-local).  Host buffers passed in
+ * are borrowed for the duration of the call; device memory is owned by the
+ * handle.  One handle per host thread.
+ *
+ * What each entry point replaces in QuetzalPIC (file:line of the reference):
+ *   qp_fourier2d_*   class Fourier2d (src/Fourier2d.h:4-19, src/Fourier2d.cpp:3-43)
+ *   qp_solve2d*      System_3d::solve_poisson_equation (src/System_3d.cpp:686-706)
+ *   qp_wake_*        System_3d::solve_wakefield (src/System_3d.cpp:238-627) with
+ *                    its particle state (src/containers.h:31-43,
+ *                    src/System_3d.cpp:724-748) and output_step
+ *                    (src/System_3d.cpp:629-666)
+ */
+#ifndef QUETZAL_HIP_H
+#define QUETZAL_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QP_OK 0
+#define QP_STOPPED 1          /* stop rule fired (src/System_3d.cpp:541-591) */
+#define QP_ERR_ARG (-1)
+#define QP_ERR_HIP (-2)
+#define QP_ERR_STATE (-3)
+#define QP_ERR_ALLOC (-4)
+
+/* Arithmetic of the slice loop. */
+#define QP_PREC_FP64 0   /* plain path: FP64 SoA particles, one kernel per reference loop */
+#define QP_PREC_MIXED 1  /* fused path: int cell + 32-bit offsets, FP32 momenta/weights,
+                            FP64 grids, solves, 1/(1+psi) and gamma, hipGraph slice step */
+
+typedef struct qp_ctx_s *qp_ctx;
+typedef struct qp_fourier2d_s *qp_fourier2d;
+typedef struct qp_wake_s *qp_wake;
+
+/* Thread-local message of the last failing call ("" if none). */
+const char *qp_last_error(void);
+/* Library version string. */
+const char *qp_version(void);
+
+/* Device context: HIP device index and one stream (default priority). */
+int qp_ctx_create(int device, qp_ctx *out);
+int qp_ctx_destroy(qp_ctx ctx);
+int qp_ctx_synchronize(qp_ctx ctx);
+
+/* ---- Fourier2d (src/Fourier2d.h:4-19) --------------------------------------
+ * n1 x n2 real <-> n1 x (n2/2+1) complex (interleaved re,im), unnormalised,
+ * FFTW r2c/c2r conventions (src/Fourier2d.cpp:33-34). */
+int qp_fourier2d_create(qp_ctx ctx, int n1, int n2, qp_fourier2d *out);
+int qp_fourier2d_destroy(qp_fourier2d f);
+/* Fourier2d::forward_transform (src/Fourier2d.cpp:37-39): host in -> host out. */
+int qp_fourier2d_forward(qp_fourier2d f, const double *in_real, double *out_complex);
+/* Fourier2d::backward_transform (src/Fourier2d.cpp:41-43): host in -> host out. */
+int qp_fourier2d_backward(qp_fourier2d f, const double *in_complex, double *out_real);
+
+/* ---- solve_poisson_equation (src/System_3d.cpp:686-706) --------------------
+ * dst = c2r( r2c(src) with (0,0) := 0 and each other mode divided by
+ *            -D + (2cos(2 pi j/n1)-2)/d1^2 + (2cos(2 pi k/n2)-2)/d2^2 ),
+ * unnormalised like the reference (the caller divides by n1*n2).
+ * src/dst may alias.  Host pointers. */
+int qp_solve2d(qp_fourier2d f, double d1, double d2, double D, const double *src, double *dst);
+/* Same with device pointers, enqueued on the context stream (no sync). */
+int qp_solve2d_device(qp_fourier2d f, double d1, double d2, double D, const double *src_dev,
+                      double *dst_dev);
+
+/* ---- One wake: System_3d::solve_wakefield (src/System_3d.cpp:238-627) ---- */
+typedef struct {
+    int nx, ny, nz;          /* slices and transverse nodes, n = (int)(l/d) (src/System_3d.cpp:87-89) */
+    double dx, dy, dz;       /* steps d = l/n (src/System_3d.cpp:93-95) */
+    double ly, lz;           /* periodic transverse box */
+    int K;                   /* magnetic_field_iterations (>= 0) */
+    double D;                /* magnetic_field_D, reference 10 (src/System_3d.h:59) */
+    double psi_threshold;    /* reference 1e-4 (src/System_3d.h:57) */
+    double b_threshold;      /* THRESHOLD_B, reference 100 (src/System_3d.cpp:326) */
+    int precision;           /* QP_PREC_FP64 | QP_PREC_MIXED */
+    int use_graph;           /* mixed path: capture the slice step in a hipGraph (1) or not (0) */
+    int resort_every;        /* mixed path: re-bin particles by cell tile every N slices (0 = never) */
+} qp_wake_params;
+
+/* Fields of one slice handed to the per-slice callback (FP64, host memory,
+ * valid during the callback only; ny*nz row-major [j][k]).
+ * kind QP_SLICE_NORMAL: output_step(i) inside the loop (src/System_3d.cpp:610);
+ * kind QP_SLICE_STOPPED: the slice where the stop rule fired -- only by, bz, chi
+ *   are meaningful (the reference breaks before output_step, :582-591);
+ * kind QP_SLICE_FINAL: output_step(0) after the loop (:622). */
+#define QP_SLICE_NORMAL 0
+#define QP_SLICE_STOPPED 1
+#define QP_SLICE_FINAL 2
+typedef struct {
+    int slice;
+    int kind;
+    const double *psi, *rho, *jx, *jy, *jz;   /* 2D arrays written by write_slice */
+    const double *ex, *ey, *ez, *by, *bz, *chi;  /* slice i of the 3D arrays */
+} qp_slice_view;
+typedef int (*qp_slice_cb)(const qp_slice_view *view, void *user);
+
+int qp_wake_create(qp_ctx ctx, const qp_wake_params *params, qp_wake *out);
+int qp_wake_destroy(qp_wake w);
+/* Plasma macro-particles, FP64 SoA (wake_particle_2d y, z, py, pz, n;
+ * src/containers.h:31-43).  py/pz may be NULL (cold plasma).  The library
+ * converts them to its own device layout.  Called before every pass (the
+ * reference re-initialises its particles each pass, src/System_3d.cpp:262). */
+int qp_wake_set_particles(qp_wake w, size_t np, const double *y, const double *z,
+                          const double *py, const double *pz, const double *weight);
+/* Laser <a^2> on the nodes, nx*ny*nz host array (a_sqr, src/System_3d.cpp:805-816). */
+int qp_wake_set_a_sqr(qp_wake w, const double *a_sqr);
+/* Gaussian laser evaluated on the device per slice: 0.5*a0^2 exp(-sum (x-x0)^2/w^2)
+ * with w = full 1/e^2 width / sqrt(8) (src/Config_reader.cpp:114-128). */
+int qp_wake_set_laser_gaussian(qp_wake w, double a0, double xsigma, double ysigma, double zsigma,
+                               double x0, double y0, double z0);
+/* Driver-bunch sources rho_bunch, jx_bunch (nx*ny*nz host arrays, either may be
+ * NULL = 0) (src/System_3d.cpp:345,384-385,449). */
+int qp_wake_set_bunch_sources(qp_wake w, const double *rho_b, const double *jx_b);
+/* Run one pass of the slice loop.  cb may be NULL (no per-slice host copies).
+ * Returns QP_OK or QP_STOPPED (*stop_slice = slice index, else -1). */
+int qp_wake_run(qp_wake w, qp_slice_cb cb, void *user, int *stop_slice);
+/* Device time of the last qp_wake_run in milliseconds (hipEvents around the pass). */
+int qp_wake_last_time_ms(qp_wake w, double *ms);
+/* Instrumented pass: same work as qp_wake_run with cb = NULL, with hipEvents
+ * around every launch of the named kernel class; returns launches and the mean
+ * device duration in microseconds.  Classes: "push" (particle phases),
+ * "solve" (all transform launches), "deposit_push_B" etc. (see docs). */
+int qp_wake_profile_kernel(qp_wake w, const char *kernel_class, int *launches, double *mean_us);
+/* Copy the current particle state back (FP64 SoA; any pointer may be NULL). */
+int qp_wake_get_particles(qp_wake w, size_t np, double *y, double *z, double *py, double *pz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,253 @@
+"""ctypes binding of the C-ABI in include/quetzal_hip.h (libquetzal_hip.so).
+
+This is the Python-side mirror of the drop-in boundary used by the tests and
+bench.py.  It has no fallback: if the HIP library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libquetzal_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "quetzal_hip.h")
+
+QP_OK, QP_STOPPED = 0, 1
+QP_PREC_FP64, QP_PREC_MIXED = 0, 1
+QP_SLICE_NORMAL, QP_SLICE_STOPPED, QP_SLICE_FINAL = 0, 1, 2
+FIELD_NAMES = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class QpError(RuntimeError):
+    pass
+
+
+class WakeParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("K", ctypes.c_int), ("D", ctypes.c_double),
+        ("psi_threshold", ctypes.c_double), ("b_threshold", ctypes.c_double),
+        ("precision", ctypes.c_int), ("use_graph", ctypes.c_int), ("resort_every", ctypes.c_int),
+    ]
+
+
+class SliceView(ctypes.Structure):
+    _fields_ = [("slice", ctypes.c_int), ("kind", ctypes.c_int)] + [(n, _dp) for n in FIELD_NAMES]
+
+
+SLICE_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(SliceView), ctypes.c_void_p)
+
+_lib = None
+
+
+def load():
+    """Load libquetzal_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QpError(f"{LIB_PATH} not built: run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    sig = {
+        "qp_last_error": ([], ctypes.c_char_p),
+        "qp_version": ([], ctypes.c_char_p),
+        "qp_ctx_create": ([ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
+        "qp_ctx_destroy": ([vp], ctypes.c_int),
+        "qp_ctx_synchronize": ([vp], ctypes.c_int),
+        "qp_fourier2d_create": ([vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
+        "qp_fourier2d_destroy": ([vp], ctypes.c_int),
+        "qp_fourier2d_forward": ([vp, _dp, _dp], ctypes.c_int),
+        "qp_fourier2d_backward": ([vp, _dp, _dp], ctypes.c_int),
+        "qp_solve2d": ([vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp], ctypes.c_int),
+        "qp_solve2d_device": ([vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp], ctypes.c_int),
+        "qp_wake_create": ([vp, ctypes.POINTER(WakeParams), ctypes.POINTER(vp)], ctypes.c_int),
+        "qp_wake_destroy": ([vp], ctypes.c_int),
+        "qp_wake_set_particles": ([vp, ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "qp_wake_set_a_sqr": ([vp, _dp], ctypes.c_int),
+        "qp_wake_set_laser_gaussian": ([vp] + [ctypes.c_double] * 7, ctypes.c_int),
+        "qp_wake_set_bunch_sources": ([vp, _dp, _dp], ctypes.c_int),
+        "qp_wake_run": ([vp, SLICE_CB, vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "qp_wake_last_time_ms": ([vp, _dp], ctypes.c_int),
+        "qp_wake_profile_kernel": ([vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
+        "qp_wake_get_particles": ([vp, ctypes.c_size_t, _dp, _dp, _dp, _dp], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(st, what):
+    if st < 0:
+        raise QpError(f"{what} failed ({st}): {load().qp_last_error().decode()}")
+    return st
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """qp_ctx: one HIP device and one stream."""
+
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        _check(load().qp_ctx_create(device, ctypes.byref(self.h)), "qp_ctx_create")
+
+    def synchronize(self):
+        _check(load().qp_ctx_synchronize(self.h), "qp_ctx_synchronize")
+
+    def close(self):
+        if self.h:
+            load().qp_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Fourier2d:
+    """Mirror of the reference's Fourier2d (src/Fourier2d.h:4-19): r2c / c2r,
+    unnormalised, n1 x n2 real <-> n1 x (n2/2+1) complex."""
+
+    def __init__(self, ctx, n1, n2):
+        self.ctx, self.n1, self.n2 = ctx, n1, n2
+        self.h = ctypes.c_void_p()
+        _check(load().qp_fourier2d_create(ctx.h, n1, n2, ctypes.byref(self.h)), "qp_fourier2d_create")
+
+    def forward_transform(self, real):
+        real = np.ascontiguousarray(real, dtype=np.float64)
+        out = np.zeros((self.n1, self.n2 // 2 + 1), dtype=np.complex128)
+        _check(load().qp_fourier2d_forward(self.h, _ptr(real), _ptr(out.view(np.float64))), "forward")
+        return out
+
+    def backward_transform(self, spec):
+        spec = np.ascontiguousarray(spec, dtype=np.complex128)
+        out = np.zeros((self.n1, self.n2))
+        _check(load().qp_fourier2d_backward(self.h, _ptr(spec.view(np.float64)), _ptr(out)), "backward")
+        return out
+
+    def solve_poisson_equation(self, src, d1, d2, D=0.0):
+        """System_3d::solve_poisson_equation (src/System_3d.cpp:686-706); unnormalised."""
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        out = np.zeros_like(src)
+        _check(load().qp_solve2d(self.h, d1, d2, D, _ptr(src), _ptr(out)), "qp_solve2d")
+        return out
+
+    def close(self):
+        if self.h:
+            load().qp_fourier2d_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Wake:
+    """One wake on the device: System_3d::solve_wakefield behind the C-ABI."""
+
+    def __init__(self, ctx, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
+                 b_threshold=100.0, precision=QP_PREC_FP64, use_graph=1, resort_every=0):
+        self.ctx = ctx
+        self.shape = (nx, ny, nz)
+        self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
+                                 precision, use_graph, resort_every)
+        self.h = ctypes.c_void_p()
+        _check(load().qp_wake_create(ctx.h, ctypes.byref(self.params), ctypes.byref(self.h)), "qp_wake_create")
+        self.np = 0
+
+    def set_particles(self, y, z, w, py=None, pz=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (y, z, py, pz, w)]
+        y, z, py, pz, w = arrs
+        self.np = len(y)
+        _check(load().qp_wake_set_particles(self.h, len(y), _ptr(y), _ptr(z), _ptr(py), _ptr(pz), _ptr(w)),
+               "qp_wake_set_particles")
+
+    def set_a_sqr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape
+        _check(load().qp_wake_set_a_sqr(self.h, _ptr(a)), "qp_wake_set_a_sqr")
+
+    def set_laser_gaussian(self, a0, xsigma, ysigma, zsigma, x0, y0, z0):
+        _check(load().qp_wake_set_laser_gaussian(self.h, a0, xsigma, ysigma, zsigma, x0, y0, z0),
+               "qp_wake_set_laser_gaussian")
+
+    def set_bunch_sources(self, rho_b=None, jx_b=None):
+        rho_b = None if rho_b is None else np.ascontiguousarray(rho_b, dtype=np.float64)
+        jx_b = None if jx_b is None else np.ascontiguousarray(jx_b, dtype=np.float64)
+        _check(load().qp_wake_set_bunch_sources(self.h, _ptr(rho_b), _ptr(jx_b)), "qp_wake_set_bunch_sources")
+
+    def run(self, fields=None, callback=None):
+        """Run one pass.  fields: iterable of names to collect into [nx,ny,nz]
+        arrays with the reference's semantics (2D per-slice writes for psi..jz,
+        3D arrays for ex..chi).  Returns (dict, status, stop_slice)."""
+        nx, ny, nz = self.shape
+        out = {}
+        if fields:
+            out = {k: np.zeros(self.shape) for k in fields}
+        stop = ctypes.c_int(-1)
+        if fields or callback:
+            nn = ny * nz
+
+            def _cb(vp, _user):
+                v = vp.contents
+                i = v.slice
+                for k in out:
+                    arr = np.ctypeslib.as_array(getattr(v, k), shape=(nn,)).reshape(ny, nz)
+                    if v.kind == QP_SLICE_STOPPED and k not in ("by", "bz", "chi"):
+                        continue
+                    out[k][i] = arr
+                if callback:
+                    return int(callback(v) or 0)
+                return 0
+
+            cb = SLICE_CB(_cb)
+        else:
+            cb = SLICE_CB()
+        st = _check(load().qp_wake_run(self.h, cb, None, ctypes.byref(stop)), "qp_wake_run")
+        return out, st, stop.value
+
+    def last_time_ms(self):
+        ms = ctypes.c_double()
+        _check(load().qp_wake_last_time_ms(self.h, ctypes.byref(ms)), "qp_wake_last_time_ms")
+        return ms.value
+
+    def profile_kernel(self, cls):
+        n = ctypes.c_int()
+        us = ctypes.c_double()
+        _check(load().qp_wake_profile_kernel(self.h, cls.encode(), ctypes.byref(n), ctypes.byref(us)),
+               "qp_wake_profile_kernel")
+        return n.value, us.value
+
+    def get_particles(self):
+        y, z, py, pz = (np.zeros(self.np) for _ in range(4))
+        _check(load().qp_wake_get_particles(self.h, self.np, _ptr(y), _ptr(z), _ptr(py), _ptr(pz)),
+               "qp_wake_get_particles")
+        return y, z, py, pz
+
+    def close(self):
+        if self.h:
+            load().qp_wake_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,112 @@
+// qp_wake.h -- device state of one wake (host side).
+#pragma once
+
+#include <vector>
+
+#include "qp_internal.h"
+
+namespace qp {
+
+// Gaussian laser <a^2> evaluated per slice (src/Config_reader.cpp:114-128,
+// src/profiles.cpp:4-11): amp * exp(-(x-x0)^2/wx^2 - (y-y0)^2/wy^2 - (z-z0)^2/wz^2).
+struct LaserGauss {
+    double amp, wx, wy, wz, x0, y0, z0;
+};
+
+// Plain FP64 particle state: the fields of wake_particle_2d
+// (src/containers.h:31-43) as separate device arrays.
+struct PartFP64 {
+    double *y = nullptr, *z = nullptr, *py = nullptr, *pz = nullptr, *n = nullptr, *gamma = nullptr,
+           *px = nullptr, *ym = nullptr, *zm = nullptr, *pyn = nullptr, *pzn = nullptr;
+};
+
+// Mixed particle state.  Persistent (24 B): packed cell (j << 16 | k),
+// in-cell offsets as unsigned 32-bit fractions of a cell, FP32 momenta and
+// weight.  Scratch (8 B): FP32 predicted momenta p*.
+struct PartMixed {
+    uint32_t *cell = nullptr, *oy = nullptr, *oz = nullptr;
+    float *py = nullptr, *pz = nullptr, *w = nullptr;
+    float *pys = nullptr, *pzs = nullptr;
+    // re-binning scratch
+    uint32_t *cell2 = nullptr, *oy2 = nullptr, *oz2 = nullptr;
+    float *py2 = nullptr, *pz2 = nullptr, *w2 = nullptr;
+    unsigned *tile_count = nullptr, *tile_cursor = nullptr;
+};
+
+struct Wake {
+    qp_ctx_s *ctx = nullptr;
+    qp_wake_params p{};
+    Solver2d S;
+    size_t np = 0;
+    size_t cap = 0;  // allocated particle capacity
+    bool particles_set = false;
+    // host copy of the initial particles (FP64), re-uploaded every pass
+    std::vector<double> h_y, h_z, h_py, h_pz, h_w;
+
+    PartFP64 pf;
+    PartMixed pm;
+
+    // grids, ny*nz FP64 each
+    double *g_base = nullptr;
+    double *src = nullptr, *psi = nullptr, *psi_prev = nullptr, *psim = nullptr, *rho = nullptr,
+           *jx = nullptr, *jy = nullptr, *jz = nullptr, *jyn = nullptr, *jzn = nullptr, *djy = nullptr,
+           *djz = nullptr, *rho_ion = nullptr, *by = nullptr, *bz = nullptr, *chi = nullptr, *ex = nullptr,
+           *ey = nullptr, *ez = nullptr, *src2 = nullptr, *asq2 = nullptr, *psi_b[2] = {nullptr, nullptr},
+           *zero2 = nullptr;
+    // driver inputs
+    double *a_sqr3 = nullptr;            // nx*ny*nz or null
+    bool laser_set = false;
+    LaserGauss laser{};
+    double *rho_b3 = nullptr, *jx_b3 = nullptr;  // nx*ny*nz or null
+
+    Ctl *ctl = nullptr;
+    double *h_out = nullptr;             // pinned host staging, 11 planes
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+
+    // mixed-path state
+    double2 *bpart = nullptr;            // border partial sums from the column pass
+    unsigned *win_stats = nullptr;       // [0] windows that overflowed LDS
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_slices = 0;
+
+    // profiling hooks: when non-null, launches of `prof_class` are bracketed by events
+    const char *prof_class = nullptr;
+    std::vector<hipEvent_t> prof_ev;
+    int prof_count = 0;
+
+    size_t nn() const { return (size_t)p.ny * p.nz; }
+    hipStream_t stream() const { return ctx->stream; }
+};
+
+int wake_alloc_common(Wake *w);
+int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice);
+int wake_run_mixed(Wake *w, qp_slice_cb cb, void *user, int *stop_slice);
+int wake_upload_fp64(Wake *w);
+int wake_upload_mixed(Wake *w);
+int wake_download_fp64(Wake *w, double *y, double *z, double *py, double *pz);
+int wake_download_mixed(Wake *w, double *y, double *z, double *py, double *pz);
+void wake_free_particles(Wake *w);
+void wake_destroy_mixed(Wake *w);
+
+// Fill the 2D <a^2> plane of slice i (from the 3D host upload or the laser).
+int wake_asq_slice(Wake *w, int i, const double **asq);
+// Pointers to slice i of the bunch sources (null when absent).
+const double *wake_rho_b(Wake *w, int i);
+const double *wake_jx_b(Wake *w, int i);
+
+// Profiling bracket helpers (no-ops unless a profile pass is active).
+void prof_begin(Wake *w, const char *cls);
+void prof_end(Wake *w, const char *cls);
+
+// Stage the 11 output planes of slice i to host and call cb.
+int deliver_slice(Wake *w, qp_slice_cb cb, void *user, int slice, int kind, const double *psi,
+                  const double *rho, const double *jx, const double *jy, const double *jz, const double *ex,
+                  const double *ey, const double *ez, const double *by, const double *bz, const double *chi);
+
+}  // namespace qp
+
+struct qp_wake_s {
+    qp::Wake w;
+};
