@@ -5,6 +5,7 @@ bench.py.  It has no fallback: if the HIP library is missing, loading raises.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -102,12 +103,19 @@ class Context:
 
     def __init__(self, device=0):
         self.h = ctypes.c_void_p()
+        self._children = weakref.WeakSet()
         _check(load().qp_ctx_create(device, ctypes.byref(self.h)), "qp_ctx_create")
+
+    def _adopt(self, child):
+        self._children.add(child)
 
     def synchronize(self):
         _check(load().qp_ctx_synchronize(self.h), "qp_ctx_synchronize")
 
     def close(self):
+        """Destroy the context; objects created on it are closed first."""
+        for c in list(self._children):
+            c.close()
         if self.h:
             load().qp_ctx_destroy(self.h)
             self.h = ctypes.c_void_p()
@@ -127,6 +135,7 @@ class Fourier2d:
         self.ctx, self.n1, self.n2 = ctx, n1, n2
         self.h = ctypes.c_void_p()
         _check(load().qp_fourier2d_create(ctx.h, n1, n2, ctypes.byref(self.h)), "qp_fourier2d_create")
+        ctx._adopt(self)
 
     def forward_transform(self, real):
         real = np.ascontiguousarray(real, dtype=np.float64)
@@ -171,6 +180,7 @@ class Wake:
         self.h = ctypes.c_void_p()
         _check(load().qp_wake_create(ctx.h, ctypes.byref(self.params), ctypes.byref(self.h)), "qp_wake_create")
         self.np = 0
+        ctx._adopt(self)
 
     def set_particles(self, y, z, w, py=None, pz=None):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (y, z, py, pz, w)]

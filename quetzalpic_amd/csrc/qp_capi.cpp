@@ -73,7 +73,7 @@ int qp_ctx_synchronize(qp_ctx ctx) {
 // ------------------------------------------------------------ Fourier2d
 int qp_fourier2d_create(qp_ctx ctx, int n1, int n2, qp_fourier2d *out) {
     ARG_CHECK(ctx && out, "qp_fourier2d_create: null argument");
-    ARG_CHECK(n1 >= 1 && n2 >= 1 && n1 <= kMaxN && n2 <= kMaxN, "qp_fourier2d_create: size out of range [1, 2048]");
+    ARG_CHECK(n1 >= 1 && n2 >= 1 && n1 <= kMaxN && n2 <= kMaxN, "qp_fourier2d_create: size out of range [1, 1024]");
     QP_HIP_CHECK(hipSetDevice(ctx->device));
     auto *f = new (std::nothrow) qp_fourier2d_s();
     if (!f) return QP_ERR_ALLOC;
@@ -165,7 +165,7 @@ int qp_wake_create(qp_ctx ctx, const qp_wake_params *params, qp_wake *out) {
     ARG_CHECK(ctx && params && out, "qp_wake_create: null argument");
     const qp_wake_params &p = *params;
     ARG_CHECK(p.nx >= 1 && p.ny >= 2 && p.nz >= 2, "qp_wake_create: need nx >= 1, ny >= 2, nz >= 2");
-    ARG_CHECK(p.ny <= kMaxN && p.nz <= kMaxN, "qp_wake_create: ny, nz must be <= 2048");
+    ARG_CHECK(p.ny <= kMaxN && p.nz <= kMaxN, "qp_wake_create: ny, nz must be <= 1024");
     ARG_CHECK(p.ny < 65536 && p.nz < 65536, "qp_wake_create: grid too large");
     ARG_CHECK(p.dx > 0 && p.dy > 0 && p.dz > 0 && p.ly > 0 && p.lz > 0, "qp_wake_create: steps must be positive");
     // Magnetic field iterations should be non-negative (src/System_3d.cpp:77-79)

@@ -97,9 +97,9 @@ void Solver2d::destroy() {
 }
 
 // ------------------------------------------------------------ launches
-static inline size_t rows_lds(const Solver2d &S) { return 2 * sizeof(double2) * (size_t)S.rows_per_block * S.n2; }
-static inline size_t cols_lds(const Solver2d &S) { return 2 * sizeof(double2) * (size_t)S.cols_per_block * S.n1; }
-static inline int rows_blocks(const Solver2d &S) {
+size_t rows_lds(const Solver2d &S) { return 2 * sizeof(double2) * (size_t)S.rows_per_block * S.n2; }
+size_t cols_lds(const Solver2d &S) { return 2 * sizeof(double2) * (size_t)S.cols_per_block * S.n1; }
+int rows_blocks(const Solver2d &S) {
     const int npairs = (S.n1 + 1) / 2;
     return (npairs + S.rows_per_block - 1) / S.rows_per_block;
 }
@@ -108,7 +108,7 @@ int solve_rows_fwd_plain_ctl(const Solver2d &S, const double *src, double2 *spec
                              const Ctl *ctl, int skip_mode) {
     SrcPlain f{src, S.n2};
     hipLaunchKernelGGL(k_rows_fwd<SrcPlain>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, f,
-                       spec, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, skip_mode);
+                       spec, spec, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, skip_mode);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
@@ -137,8 +137,8 @@ int solve_cols_inverse_only(const Solver2d &S, hipStream_t s) { return cols_laun
 
 int solve_rows_inv_unscaled(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s) {
     SinkRaw f{dst, S.n2};
-    hipLaunchKernelGGL(k_rows_inv<SinkRaw>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec, f,
-                       S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)nullptr, 0);
+    hipLaunchKernelGGL(k_rows_inv<SinkRaw>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
+                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)nullptr, 0);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
@@ -147,7 +147,7 @@ int solve_rows_inv_plain_ctl(const Solver2d &S, const double2 *spec, double *dst
                              const Ctl *ctl) {
     SinkScaled f{dst, S.n1, S.n2};
     hipLaunchKernelGGL(k_rows_inv<SinkScaled>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
-                       f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, ctl ? 1 : 0);
+                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, ctl ? 1 : 0);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
@@ -160,7 +160,7 @@ int solve_rows_inv_bcheck(const Solver2d &S, const double2 *spec, double *dst, d
                           hipStream_t s) {
     SinkBCheck f{dst, S.n1, S.n2, thr, ctl, slice};
     hipLaunchKernelGGL(k_rows_inv<SinkBCheck>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
-                       f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)ctl, 2);
+                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)ctl, 2);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }

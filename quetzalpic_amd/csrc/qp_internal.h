@@ -29,8 +29,8 @@ void set_error(const std::string &msg);
 
 // ------------------------------------------------------------ FFT plan
 constexpr int kMaxStages = 24;
-constexpr int kMaxN = 2048;          // longest transform the LDS kernels take
-constexpr int kLdsComplex = 2048;    // complex doubles per LDS buffer (32 KiB; two buffers)
+constexpr int kMaxN = 1024;          // longest transform the LDS kernels take
+constexpr int kLdsComplex = 1536;    // complex doubles per LDS buffer (24 KiB; two buffers)
 
 // One transform axis as the kernels see it (passed by value).
 struct FftAxisDev {
@@ -73,8 +73,7 @@ struct Ctl {
     int stop_acc;
     int stop_now;
     int stop_slice;
-    int slice;         // current slice index (graph-captured kernels read it)
-    int psi_cur;       // mixed path: which psi buffer holds the current slice
+    int slice_of[2];   // mixed path: slice index for the even / odd slice step
     int pad[3];
     double border[4];  // plain path: border mean of the last psi solve
 };
@@ -200,6 +199,9 @@ int solve_rows_inv_plain(const Solver2d &S, const double2 *spec, double *dst, hi
 // Plain-path B solve epilogue: store v/n1/n2 and flag |B| > thr or NaN.
 int solve_rows_inv_bcheck(const Solver2d &S, const double2 *spec, double *dst, double thr, Ctl *ctl,
                           int slice, hipStream_t s);
+size_t rows_lds(const Solver2d &S);
+size_t cols_lds(const Solver2d &S);
+int rows_blocks(const Solver2d &S);
 int solve_cols_forward_only(const Solver2d &S, hipStream_t s);
 int solve_cols_inverse_only(const Solver2d &S, hipStream_t s);
 int solve_rows_inv_unscaled(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s);

@@ -1,0 +1,306 @@
+// qp_mixed_kernels.h -- device code of the fused mixed-precision slice step.
+//
+// Particle layout (structure of arrays, 24 B persistent + 8 B scratch):
+//   cell  uint32  (j << 16) | k          node-origin cell of the particle
+//   oy,oz uint32  in-cell offset in units of 2^-32 cell (Y = j + oy 2^-32)
+//   py,pz float   transverse momenta
+//   w     float   macro-particle weight (wake_particle_2d::n)
+//   pys,pzs float predicted momenta p* of the current iteration (scratch)
+// Arithmetic is FP64 throughout (FP64 is cheap next to the memory traffic);
+// only the stored state is narrowed.  Positions are advanced in FP64 cell
+// units and re-split with round-to-nearest into cell + offset, so the
+// stored position keeps ~2^-32 cell resolution over thousands of slices.
+//
+// The 35 particle sweeps of a reference slice (src/System_3d.cpp:328-619)
+// become 2K+1 launches: A (commit + advance + psi source deposit), then per
+// iteration B (gamma, predictor, half step, psi_middle source deposit) and
+// C (transverse current at the half step, gamma/px, rho/jx/chi deposits).
+#pragma once
+
+#include "qp_solver_kernels.h"
+#include "qp_wake.h"
+
+namespace qp {
+
+constexpr int kPT = 256;          // threads (= particles) per block
+constexpr int kWinCap = 640;      // LDS window nodes per deposited grid
+constexpr double kTwo32 = 4294967296.0;
+constexpr double kInvTwo32 = 1.0 / 4294967296.0;
+
+struct MxGeom {
+    int n1, n2;
+    double d1, d2, ly, lz, dx;
+    double dx_d1, dx_d2;  // dx/dy, dx/dz: position update in cell units
+};
+
+struct MxPart {
+    uint32_t *cell, *oy, *oz;
+    float *py, *pz, *w, *pys, *pzs;
+    int np;
+};
+
+// Slice-dependent inputs resolved on the device from the slice counter.
+struct MxSliceIn {
+    const double *a_sqr3;   // nx*n1*n2 or null
+    const double *asq2;     // laser plane buffer (used when a_sqr3 is null)
+    const double *rho_b3, *jx_b3;
+    int slot;               // which ctl->slice_of[] entry holds this step's slice
+};
+
+__device__ __forceinline__ int mx_slice(const Ctl *c, const MxSliceIn &si) { return c->slice_of[si.slot]; }
+__device__ __forceinline__ const double *mx_asq(const Ctl *c, const MxSliceIn &si, size_t nn) {
+    return si.a_sqr3 ? si.a_sqr3 + (size_t)mx_slice(c, si) * nn : si.asq2;
+}
+
+// ----------------------------------------------------------- positions
+struct Pos {
+    int j, k;
+    double fy, fz;
+};
+
+__device__ __forceinline__ Pos load_pos(const MxPart &P, int p) {
+    const uint32_t c = P.cell[p];
+    Pos r;
+    r.j = (int)(c >> 16);
+    r.k = (int)(c & 0xffffu);
+    r.fy = (double)P.oy[p] * kInvTwo32;
+    r.fz = (double)P.oz[p] * kInvTwo32;
+    return r;
+}
+
+// Split a cell-unit coordinate Y in [0, n) (after wrap) into cell and offset,
+// offset rounded to the nearest 2^-32.
+__device__ __forceinline__ void split_coord(double Y, int n, int &j, uint32_t &o) {
+    double fl = floor(Y);
+    int jj = (int)fl;
+    double u = rint((Y - fl) * kTwo32);
+    if (u >= kTwo32) {
+        u -= kTwo32;
+        jj += 1;
+    }
+    if (jj >= n) jj -= n;
+    if ((unsigned)jj >= (unsigned)n) jj = 0;
+    j = jj;
+    o = (uint32_t)u;
+}
+
+// Periodic wrap in cell units (normalize_coordinates, src/System_3d.cpp:668-684).
+__device__ __forceinline__ double wrap_cells(double Y, int n) {
+    Y = fmod(Y, (double)n);
+    if (Y < 0) Y += n;
+    if (Y >= n) Y = 0.0;
+    return Y;
+}
+
+__device__ __forceinline__ Pos pos_from_cells(double Y, double Z, int n1, int n2) {
+    Pos r;
+    double fl = floor(Y);
+    r.j = (int)fl;
+    r.fy = Y - fl;
+    if ((unsigned)r.j >= (unsigned)n1) r.j = 0;
+    fl = floor(Z);
+    r.k = (int)fl;
+    r.fz = Z - fl;
+    if ((unsigned)r.k >= (unsigned)n2) r.k = 0;
+    return r;
+}
+
+// CIC axis indices: node origin (j1 = cell) and half-cell origin (j1 = floor(Y - 1/2)).
+__device__ __forceinline__ void ax_node(int j, double f, int n, int &j1, int &j2, double &fr) {
+    j1 = j;
+    j2 = j + 1 == n ? 0 : j + 1;
+    fr = f;
+}
+__device__ __forceinline__ void ax_half(int j, double f, int n, int &j1, int &j2, double &fr) {
+    if (f >= 0.5) {
+        j1 = j;
+        fr = f - 0.5;
+    } else {
+        j1 = j == 0 ? n - 1 : j - 1;
+        fr = f + 0.5;
+    }
+    j2 = j1 + 1 == n ? 0 : j1 + 1;
+}
+
+// Node-origin gather (array_to_particle, src/array_utils.cpp:167-188).
+__device__ __forceinline__ double mx_gather(const double *a, const Pos &q, int n1, int n2) {
+    int j1, j2, k1, k2;
+    double yf, zf;
+    ax_node(q.j, q.fy, n1, j1, j2, yf);
+    ax_node(q.k, q.fz, n2, k1, k2, zf);
+    return gather4(a, n2, j1, j2, k1, k2, yf, zf);
+}
+// Gathers on the staggered B grids: B_y has origin (0, dz/2), B_z (dy/2, 0).
+__device__ __forceinline__ double mx_gather_hz(const double *a, const Pos &q, int n1, int n2) {
+    int j1, j2, k1, k2;
+    double yf, zf;
+    ax_node(q.j, q.fy, n1, j1, j2, yf);
+    ax_half(q.k, q.fz, n2, k1, k2, zf);
+    return gather4(a, n2, j1, j2, k1, k2, yf, zf);
+}
+__device__ __forceinline__ double mx_gather_hy(const double *a, const Pos &q, int n1, int n2) {
+    int j1, j2, k1, k2;
+    double yf, zf;
+    ax_half(q.j, q.fy, n1, j1, j2, yf);
+    ax_node(q.k, q.fz, n2, k1, k2, zf);
+    return gather4(a, n2, j1, j2, k1, k2, yf, zf);
+}
+// Staggered-derivative gathers (src/array_utils.cpp:240-288).
+__device__ __forceinline__ double mx_yder(const double *a, const Pos &q, int n1, int n2, double d1) {
+    int j1, j2, k1, k2;
+    double yf, zf;
+    ax_half(q.j, q.fy, n1, j1, j2, yf);
+    const int j3 = j2 + 1 == n1 ? 0 : j2 + 1;
+    ax_node(q.k, q.fz, n2, k1, k2, zf);
+    const double g11 = (a[(size_t)n2 * j2 + k1] - a[(size_t)n2 * j1 + k1]) / d1;
+    const double g21 = (a[(size_t)n2 * j3 + k1] - a[(size_t)n2 * j2 + k1]) / d1;
+    const double g12 = (a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j1 + k2]) / d1;
+    const double g22 = (a[(size_t)n2 * j3 + k2] - a[(size_t)n2 * j2 + k2]) / d1;
+    return g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf;
+}
+__device__ __forceinline__ double mx_zder(const double *a, const Pos &q, int n1, int n2, double d2) {
+    int j1, j2, k1, k2;
+    double yf, zf;
+    ax_node(q.j, q.fy, n1, j1, j2, yf);
+    ax_half(q.k, q.fz, n2, k1, k2, zf);
+    const int k3 = k2 + 1 == n2 ? 0 : k2 + 1;
+    const double g11 = (a[(size_t)n2 * j1 + k2] - a[(size_t)n2 * j1 + k1]) / d2;
+    const double g21 = (a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j2 + k1]) / d2;
+    const double g12 = (a[(size_t)n2 * j1 + k3] - a[(size_t)n2 * j1 + k2]) / d2;
+    const double g22 = (a[(size_t)n2 * j2 + k3] - a[(size_t)n2 * j2 + k2]) / d2;
+    return g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf;
+}
+
+// ------------------------------------------------------ LDS deposit window
+// Each block deposits into an LDS window covering the nodes its particles
+// touch, then flushes the window with one global atomic per non-zero node.
+// The window is found from the block's own particles (periodic-aware, relative
+// to the first particle's cell).  If it exceeds kWinCap nodes the block falls
+// back to direct global atomics.
+struct Win {
+    int j0, k0, wj, wk;
+    bool lds;
+};
+
+__device__ __forceinline__ int rel_idx(int j, int ref, int n) {
+    int d = j - ref;
+    if (d < -(n / 2)) d += n;
+    else if (d >= n - n / 2) d -= n;
+    return d;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+// rmin/rmax/cmin/cmax: this thread's relative node range (rmin > rmax when
+// the thread deposits nothing).  `sh` is 4*(blockDim/64)+2 ints of LDS.
+__device__ inline Win block_window(int rmin, int rmax, int cmin, int cmax, int jref, int kref, int n1, int n2,
+                                   int *sh) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    rmin = wave_min(rmin);
+    rmax = wave_max(rmax);
+    cmin = wave_min(cmin);
+    cmax = wave_max(cmax);
+    if (lane == 0) {
+        sh[4 * wv + 0] = rmin;
+        sh[4 * wv + 1] = rmax;
+        sh[4 * wv + 2] = cmin;
+        sh[4 * wv + 3] = cmax;
+    }
+    __syncthreads();
+    int a = sh[0], b = sh[1], c = sh[2], d = sh[3];
+    for (int i = 1; i < nw; ++i) {
+        a = min(a, sh[4 * i + 0]);
+        b = max(b, sh[4 * i + 1]);
+        c = min(c, sh[4 * i + 2]);
+        d = max(d, sh[4 * i + 3]);
+    }
+    __syncthreads();
+    Win w;
+    if (a > b) {  // no deposits in this block
+        w.j0 = w.k0 = 0;
+        w.wj = w.wk = 0;
+        w.lds = true;
+        return w;
+    }
+    w.wj = b - a + 1;
+    w.wk = d - c + 1;
+    w.j0 = jref + a;
+    w.k0 = kref + c;
+    w.j0 = ((w.j0 % n1) + n1) % n1;
+    w.k0 = ((w.k0 % n2) + n2) % n2;
+    w.lds = (w.wj <= n1 && w.wk <= n2 && w.wj * w.wk <= kWinCap);
+    return w;
+}
+
+__device__ __forceinline__ int win_slot(const Win &w, int j, int k, int n1, int n2) {
+    int dj = j - w.j0;
+    if (dj < 0) dj += n1;
+    int dk = k - w.k0;
+    if (dk < 0) dk += n2;
+    if (dj >= w.wj || dk >= w.wk) return -1;
+    return dj * w.wk + dk;
+}
+
+// One CIC deposit of value v at (j1..j2, k1..k2) into grid g (LDS window or global).
+__device__ __forceinline__ void dep4(double *lds, const Win &w, double *glob, int n1, int n2, int j1, int j2, int k1,
+                                     int k2, double yf, double zf, double v) {
+    const double a11 = v * (1 - yf) * (1 - zf), a21 = v * yf * (1 - zf), a12 = v * (1 - yf) * zf,
+                 a22 = v * yf * zf;
+    if (w.lds) {
+        const int s11 = win_slot(w, j1, k1, n1, n2), s21 = win_slot(w, j2, k1, n1, n2);
+        const int s12 = win_slot(w, j1, k2, n1, n2), s22 = win_slot(w, j2, k2, n1, n2);
+        if (s11 >= 0 && s21 >= 0 && s12 >= 0 && s22 >= 0) {
+            atomicAdd(&lds[s11], a11);
+            atomicAdd(&lds[s21], a21);
+            atomicAdd(&lds[s12], a12);
+            atomicAdd(&lds[s22], a22);
+            return;
+        }
+    }
+    unsafeAtomicAdd(&glob[(size_t)n2 * j1 + k1], a11);
+    unsafeAtomicAdd(&glob[(size_t)n2 * j2 + k1], a21);
+    unsafeAtomicAdd(&glob[(size_t)n2 * j1 + k2], a12);
+    unsafeAtomicAdd(&glob[(size_t)n2 * j2 + k2], a22);
+}
+
+__device__ inline void win_zero(double *lds, int ngrid, const Win &w) {
+    if (!w.lds) return;
+    const int m = w.wj * w.wk;
+    for (int g = 0; g < ngrid; ++g)
+        for (int i = threadIdx.x; i < m; i += blockDim.x) lds[g * kWinCap + i] = 0.0;
+}
+
+__device__ inline void win_flush(const double *lds, double *const *glob, int ngrid, const Win &w, int n1, int n2) {
+    if (!w.lds) return;
+    const int m = w.wj * w.wk;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const int dj = i / w.wk, dk = i - dj * w.wk;
+        int j = w.j0 + dj;
+        if (j >= n1) j -= n1;
+        int k = w.k0 + dk;
+        if (k >= n2) k -= n2;
+        for (int g = 0; g < ngrid; ++g) {
+            const double v = lds[g * kWinCap + i];
+            if (v != 0.0) unsafeAtomicAdd(&glob[g][(size_t)n2 * j + k], v);
+        }
+    }
+}
+
+// Relative node range [r, r+1] x [c, c+1] of a deposit at (j1, k1).
+__device__ __forceinline__ void range_add(int j1, int k1, int jref, int kref, int n1, int n2, int &rmin, int &rmax,
+                                          int &cmin, int &cmax) {
+    const int r = rel_idx(j1, jref, n1), c = rel_idx(k1, kref, n2);
+    rmin = min(rmin, r);
+    rmax = max(rmax, r + 1);
+    cmin = min(cmin, c);
+    cmax = max(cmax, c + 1);
+}
+
+}  // namespace qp

@@ -34,6 +34,7 @@ struct SrcPlain {
 struct SinkScaled {
     double *dst;
     int n1, n2;
+    __device__ void begin() {}
     __device__ void operator()(int j, int k, double v) const { dst[(size_t)j * n2 + k] = v / n1 / n2; }
 };
 
@@ -41,6 +42,7 @@ struct SinkScaled {
 struct SinkRaw {
     double *dst;
     int n2;
+    __device__ void begin() {}
     __device__ void operator()(int j, int k, double v) const { dst[(size_t)j * n2 + k] = v; }
 };
 
@@ -51,6 +53,7 @@ struct SinkBCheck {
     double thr;
     Ctl *ctl;
     int slice;
+    __device__ void begin() {}
     __device__ void operator()(int j, int k, double v) const {
         const double b = v / n1 / n2;
         dst[(size_t)j * n2 + k] = b;
@@ -64,10 +67,11 @@ struct SinkBCheck {
 // Rows forward: two real rows (2p, 2p+1) packed as z = x_a + i x_b, complex
 // FFT along n2, then X_a[k] = (Z[k] + conj Z[-k])/2, X_b[k] = (Z[k] - conj Z[-k])/(2i).
 template <class Src>
-__global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *spec, int n1, int n2, int nh,
-                                                             FftAxisDev ax, int B, const Ctl *ctl,
+__global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *spec0, double2 *spec1, int n1,
+                                                             int n2, int nh, FftAxisDev ax, int B, const Ctl *ctl,
                                                              int skip_mode) {
     if (solver_skip(ctl, skip_mode)) return;
+    double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
     extern __shared__ double2 lds[];
     double2 *a = lds, *b = lds + (size_t)B * n2;
     const int pair0 = blockIdx.x * B;
@@ -155,10 +159,12 @@ static __global__ __launch_bounds__(kSolverThreads) void k_cols_solve(double2 *s
 // X[-k] = conj X[k]; imaginary parts of the k = 0 and k = n2/2 terms ignored
 // (FFTW c2r semantics).  Re z -> row 2p, Im z -> row 2p+1.
 template <class Sink>
-__global__ __launch_bounds__(kSolverThreads) void k_rows_inv(const double2 *spec, Sink sink, int n1, int n2,
-                                                             int nh, FftAxisDev ax, int B, const Ctl *ctl,
-                                                             int skip_mode) {
+__global__ __launch_bounds__(kSolverThreads) void k_rows_inv(const double2 *spec0, const double2 *spec1, Sink sink,
+                                                             int n1, int n2, int nh, FftAxisDev ax, int B,
+                                                             const Ctl *ctl, int skip_mode) {
     if (solver_skip(ctl, skip_mode)) return;
+    const double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+    sink.begin();
     extern __shared__ double2 lds[];
     double2 *a = lds, *b = lds + (size_t)B * n2;
     const int pair0 = blockIdx.x * B;

@@ -20,19 +20,6 @@ struct PartFP64 {
            *px = nullptr, *ym = nullptr, *zm = nullptr, *pyn = nullptr, *pzn = nullptr;
 };
 
-// Mixed particle state.  Persistent (24 B): packed cell (j << 16 | k),
-// in-cell offsets as unsigned 32-bit fractions of a cell, FP32 momenta and
-// weight.  Scratch (8 B): FP32 predicted momenta p*.
-struct PartMixed {
-    uint32_t *cell = nullptr, *oy = nullptr, *oz = nullptr;
-    float *py = nullptr, *pz = nullptr, *w = nullptr;
-    float *pys = nullptr, *pzs = nullptr;
-    // re-binning scratch
-    uint32_t *cell2 = nullptr, *oy2 = nullptr, *oz2 = nullptr;
-    float *py2 = nullptr, *pz2 = nullptr, *w2 = nullptr;
-    unsigned *tile_count = nullptr, *tile_cursor = nullptr;
-};
-
 struct Wake {
     qp_ctx_s *ctx = nullptr;
     qp_wake_params p{};
@@ -44,7 +31,6 @@ struct Wake {
     std::vector<double> h_y, h_z, h_py, h_pz, h_w;
 
     PartFP64 pf;
-    PartMixed pm;
 
     // grids, ny*nz FP64 each
     double *g_base = nullptr;
@@ -64,12 +50,8 @@ struct Wake {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
 
-    // mixed-path state
-    double2 *bpart = nullptr;            // border partial sums from the column pass
-    unsigned *win_stats = nullptr;       // [0] windows that overflowed LDS
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_slices = 0;
+    // mixed-path state (owned by qp_wake_mixed.hip)
+    void *mixed = nullptr;
 
     // profiling hooks: when non-null, launches of `prof_class` are bracketed by events
     const char *prof_class = nullptr;

@@ -1,0 +1,109 @@
+"""GPU parity of the fused mixed-precision slice loop against the FP64 oracle.
+
+Bar (from the target statement): max and RMS relative error <= 1e-5 on psi and
+on e_x (the longitudinal field; the reference's `ex`).  The other fields are
+reported and held to looser bounds set by the FP32 particle state.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
+TOL_PSI_EX = 1e-5
+
+
+def case(oracle, L=20.0, n=32, lx=4.0, dx=0.1, a0=1.0, K=5, ppc=2):
+    g = oracle.grid_from_box(lx, L, L, dx, L / n, L / n, K=K)
+    y, z, w = oracle.init_lattice(g, ppc, ppc)
+    a = oracle.gaussian_a_sqr(g, a0, 4.0, 6.0, 6.0, lx - 2.0, L / 2, L / 2)
+    return g, y, z, w, a
+
+
+def run_mixed(qp_ctx, g, y, z, w, a=None, laser=None, use_graph=1, resort_every=0, fields=FIELDS, **kw):
+    from quetzalpic_amd import capi
+    wk = capi.Wake(qp_ctx, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ly, g.lz, K=g.K, D=g.D,
+                   psi_threshold=g.psi_threshold, b_threshold=g.b_threshold, precision=capi.QP_PREC_MIXED,
+                   use_graph=use_graph, resort_every=resort_every)
+    wk.set_particles(y, z, w, kw.get("py"), kw.get("pz"))
+    if a is not None:
+        wk.set_a_sqr(a)
+    if laser is not None:
+        wk.set_laser_gaussian(*laser)
+    out, st, ss = wk.run(fields=fields)
+    return wk, out, st, ss
+
+
+def check(got, want, tol_main=TOL_PSI_EX, tol_other=1e-4):
+    errs = {}
+    for f in FIELDS:
+        errs[f] = rel_err(got[f], want[f])
+    for f in ("psi", "ex"):
+        assert errs[f][0] < tol_main and errs[f][1] < tol_main, (f, errs[f])
+    for f in FIELDS:
+        assert errs[f][0] < tol_other, (f, errs[f])
+    return errs
+
+
+@pytest.mark.parametrize("K", [5, 1, 0])
+def test_mixed_matches_oracle(qp_ctx, oracle, K):
+    g, y, z, w, a = case(oracle, K=K)
+    want, st0, _ = oracle.wake_run(g, y, z, w, a)
+    wk, got, st1, _ = run_mixed(qp_ctx, g, y, z, w, a)
+    wk.close()
+    assert st0 == st1 == 0
+    check(got, want)
+
+
+def test_mixed_graph_equals_stream(qp_ctx, oracle):
+    """hipGraph replay of the slice step gives the same result as direct launches."""
+    g, y, z, w, a = case(oracle, lx=3.0)
+    wk1, o1, _, _ = run_mixed(qp_ctx, g, y, z, w, a, use_graph=0, fields=("psi", "ex", "by"))
+    wk2, o2, _, _ = run_mixed(qp_ctx, g, y, z, w, a, use_graph=1, fields=("psi", "ex", "by"))
+    for f in ("psi", "ex", "by"):
+        # only the order of the FP64 atomic deposits differs between the runs
+        assert rel_err(o2[f], o1[f])[0] < 1e-9, f
+    # graph path without a callback must also agree (state read back afterwards)
+    _, st, _ = wk2.run()
+    y2, z2, py2, pz2 = wk2.get_particles()
+    _, st, _ = wk1.run()
+    y1, z1, py1, pz1 = wk1.get_particles()
+    assert np.abs(y2 - y1).max() < 1e-9 and np.abs(py2 - py1).max() < 1e-6
+    wk1.close()
+    wk2.close()
+
+
+def test_mixed_device_laser_matches_host_a_sqr(qp_ctx, oracle):
+    g, y, z, w, a = case(oracle)
+    lx = g.nx * g.dx
+    laser = (1.0, 4.0, 6.0, 6.0, lx - 2.0, g.ly / 2, g.lz / 2)
+    wk1, o1, _, _ = run_mixed(qp_ctx, g, y, z, w, a=a, fields=("psi", "ex"))
+    wk2, o2, _, _ = run_mixed(qp_ctx, g, y, z, w, laser=laser, fields=("psi", "ex"))
+    wk1.close()
+    wk2.close()
+    for f in ("psi", "ex"):
+        assert rel_err(o2[f], o1[f])[0] < 1e-9, f
+
+
+def test_mixed_resort_matches(qp_ctx, oracle):
+    """Re-binning particles by tile during the pass changes only summation order."""
+    g, y, z, w, a = case(oracle, a0=2.0)
+    want, _, _ = oracle.wake_run(g, y, z, w, a)
+    wk, got, st, _ = run_mixed(qp_ctx, g, y, z, w, a, resort_every=3)
+    wk.close()
+    check(got, want)
+
+
+def test_mixed_larger_nonlinear(qp_ctx, oracle):
+    """64^2, 100 slices, a0 = 2: stronger wake, longer accumulation."""
+    g = oracle.grid_from_box(5.0, 20.0, 20.0, 0.05, 20 / 64, 20 / 64, K=5)
+    y, z, w = oracle.init_lattice(g, 2, 2)
+    a = oracle.gaussian_a_sqr(g, 2.0, 4.243, 8.0, 8.0, 5.0 - 3.0, 10.0, 10.0)
+    want, _, _ = oracle.wake_run(g, y, z, w, a, fields=("psi", "ex", "ey", "by"))
+    wk, got, st, _ = run_mixed(qp_ctx, g, y, z, w, a, fields=("psi", "ex", "ey", "by"))
+    wk.close()
+    for f in ("psi", "ex"):
+        mx, rms = rel_err(got[f], want[f])
+        assert mx < TOL_PSI_EX and rms < TOL_PSI_EX, (f, mx, rms)
