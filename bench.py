@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark of the quasistatic slice loop on one MI355X.
+
+A step is one pass of the slice loop (System_3d::solve_wakefield) over the
+workload's n_x slices: value = particle-slice pushes/s = N_p (n_x - 1) / t.
+Default workload: C2 (256^2, 262144 particles, 1000 slices).  Prints one JSON
+line on stdout (progress on stderr).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bytes_per_particle(cls, K):
+    """Algorithmic HBM bytes per particle and launch of a particle phase
+    (24 B persistent state S, 8 B scratch T = p*):
+      A: read S + T, write position + momenta (20 B)          = 52
+      B: read S (+ T after the first iteration), write T      = 32 | 40
+      C: read S + T                                           = 32
+      S: read S                                               = 24"""
+    if cls == "push_A":
+        return 52.0
+    if cls == "push_B":
+        return (32.0 + 40.0 * (K - 1)) / K if K > 0 else 32.0
+    if cls == "push_C":
+        return 32.0
+    if cls == "push_S":
+        return 24.0
+    raise KeyError(cls)
+
+
+def solve_bytes(n1, n2):
+    """Algorithmic bytes of one three-launch solve: real source read + half
+    spectrum write, spectrum read + write, spectrum read + real write."""
+    nh = n2 // 2 + 1
+    real = 8.0 * n1 * n2
+    spec = 16.0 * n1 * nh
+    return (real + spec) + 2 * spec + (spec + real)
+
+
+def read_traffic(kernel_cls):
+    """Per-launch HBM KiB of a kernel from the committed rocprofv3 --pmc summary
+    (profiles/traffic_<class>_C2.txt: lines 'FETCH_KIB <v>', 'WRITE_KIB <v>',
+    'COPY_FACTOR <f>'); None when absent."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel_cls}_C2.txt")
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    with open(p) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 2:
+                try:
+                    vals[parts[0]] = float(parts[1])
+                except ValueError:
+                    pass
+    if "FETCH_KIB" not in vals or "WRITE_KIB" not in vals:
+        return None
+    fac = vals.get("COPY_FACTOR", 1.0)
+    return (vals["FETCH_KIB"] + vals["WRITE_KIB"]) * 1024.0 * fac
+
+
+def make_wake(capi, ctx, wl, precision, use_graph=1):
+    w = capi.Wake(ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.L, wl.L, K=wl.K, precision=precision,
+                  use_graph=use_graph, resort_every=0)
+    y, z, wt, py, pz = wl.particles()
+    w.set_particles(y, z, wt, py, pz)
+    w.set_laser_gaussian(*wl.laser)
+    return w
+
+
+def cpu_baseline(wl, budget_s, gpu_fields):
+    """The oracle's OpenMP -Ofast build (the reference's loop structure and
+    Release flags) on the first S slices of the same workload."""
+    from oracle import oracle_ctypes as oc
+    lib = oc.load("omp")
+    g = oc.grid_from_box(wl.lx, wl.L, wl.L, wl.dx0, wl.L / wl.n0, wl.L / wl.n0, K=wl.K)
+    y, z, w, py, pz = wl.particles()
+    a = oc.gaussian_a_sqr(g, *wl.laser)
+    # calibrate on 3 slices, then size the sample to the time budget
+    t0 = time.perf_counter()
+    oc.wake_run(g, y, z, w, a, py=py, pz=pz, n_slices=3, fields=(), variant="omp")
+    per = (time.perf_counter() - t0) / 3
+    s = int(max(3, min(g.nx - 1, budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    out, _, _ = oc.wake_run(g, y, z, w, a, py=py, pz=pz, n_slices=s, fields=("psi", "ex"), variant="omp")
+    t = time.perf_counter() - t0
+    np_ = len(y)
+    res = {"value": np_ * s / t, "unit": "particle-slice pushes/s", "cores": int(lib.qpo_num_threads()),
+           "kind": "port",
+           "sample": f"first {s} of {g.nx} slices of the same workload ({np_} particles), FP64, "
+                     f"OpenMP atomics, own mixed-radix FFT, -Ofast; {t:.1f} s"}
+    errs = {}
+    if gpu_fields is not None:
+        lo = g.nx - s
+        for f in ("psi", "ex"):
+            ref = out[f][lo:g.nx]
+            got = gpu_fields[f][lo:g.nx]
+            if f == "ex":  # the last slice of ex is zero in both
+                ref, got = ref[:-1], got[:-1]
+            d = got - ref
+            errs[f] = [float(np.abs(d).max() / max(np.abs(ref).max(), 1e-300)),
+                       float(np.linalg.norm(d) / max(np.linalg.norm(ref), 1e-300))]
+    return res, errs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="C2")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the slice step directly instead of replaying hipGraphs (for tracers)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the slice loop has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    from quetzalpic_amd import capi, workloads
+    wl = workloads.get(args.workload)
+    log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles}")
+    ctx = capi.Context(0)
+
+    # ---- timed region: the mixed fused path, hipGraph slice step
+    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=0 if args.no_graph else 1)
+    for _ in range(args.warmup):
+        wk.run()
+    ctx.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        wk.run()
+        dev_ms.append(wk.last_time_ms())
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = 1e3 * (t1 - t0) / args.steps
+    pushes = wl.n_particles * (wl.nx - 1)
+    solves = (1 + 3 * wl.K) * (wl.nx - 1) + 1
+    value = pushes / (ms_per_step * 1e-3)
+    log(f"[bench] mixed: {ms_per_step:.2f} ms/pass wall, device {np.median(dev_ms):.2f} ms -> {value:.3e} pushes/s")
+    tuning = json.loads(wk.describe())
+    log(f"[bench] launch shapes: {tuning}")
+
+    # ---- per-kernel-class device times (separate instrumented pass, same work)
+    classes = {}
+    for cls in ("push_A", "push_B", "push_C", "solve"):
+        n, us = wk.profile_kernel(cls)
+        classes[cls] = {"launches": n, "mean_us": us, "total_ms": n * us * 1e-3}
+        log(f"[bench]   {cls}: {n} launches x {us:.2f} us = {n * us * 1e-3:.2f} ms")
+    dom = max(("push_A", "push_B", "push_C"), key=lambda c: classes[c]["total_ms"])
+    dom_bytes = bytes_per_particle(dom, wl.K) * wl.n_particles
+    achieved = dom_bytes / (classes[dom]["mean_us"] * 1e-6) / 1e9
+    traffic = read_traffic(dom)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_mx_{dom[-1]}",
+                "algorithmic_bytes_per_launch": dom_bytes}
+
+    # ---- plain FP64 GPU path (one kernel per reference loop) on the same workload
+    fp64 = None
+    fields64 = None
+    if not args.no_fp64:
+        wf = make_wake(capi, ctx, wl, capi.QP_PREC_FP64)
+        wf.run()
+        t0 = time.perf_counter()
+        wf.run()
+        t64 = time.perf_counter() - t0
+        fp64 = pushes / t64
+        log(f"[bench] fp64 plain: {t64 * 1e3:.1f} ms/pass -> {fp64:.3e} pushes/s")
+        if not args.no_check:
+            fields64, _, _ = wf.run(fields=("psi", "ex"))
+        wf.close()
+
+    # ---- error of the mixed path against the FP64 path over the whole pass
+    err64 = None
+    fieldsmx = None
+    if not args.no_check:
+        fieldsmx, _, _ = wk.run(fields=("psi", "ex"))
+        if fields64 is not None:
+            err64 = {}
+            for f in ("psi", "ex"):
+                d = fieldsmx[f] - fields64[f]
+                err64[f] = [float(np.abs(d).max() / np.abs(fields64[f]).max()),
+                            float(np.linalg.norm(d) / np.linalg.norm(fields64[f]))]
+            log(f"[bench] mixed vs fp64 GPU (max, rms rel): {err64}")
+    wk.close()
+
+    cpu = None
+    err_oracle = None
+    if not args.no_cpu:
+        cpu, err_oracle = cpu_baseline(wl, args.cpu_budget, fieldsmx)
+        log(f"[bench] cpu baseline: {cpu['value']:.3e} pushes/s on {cpu['cores']} threads; err vs oracle "
+            f"{err_oracle}")
+    ctx.close()
+
+    line = {
+        "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, <=1e-5 rel err in psi and Ez vs FP64)",
+        "value": value,
+        "unit": "particle-slice pushes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "storage": "particles: u32 cell + u32 offsets, f32 momenta/weight; grids and solves f64",
+        "data": "synthetic",
+        "config": wl.describe(),
+        "solves_per_s": solves / (ms_per_step * 1e-3),
+        "device_ms_per_step": float(np.median(dev_ms)),
+        "kernel_classes": classes,
+        "launch_shapes": tuning,
+        "roofline": roofline,
+        "fp64_gpu_pushes_per_s": fp64,
+        "speedup_vs_fp64_gpu": (value / fp64) if fp64 else None,
+        "err_vs_fp64_gpu": err64,
+        "err_vs_oracle_sample": err_oracle,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,9 @@ int qp_wake_last_time_ms(qp_wake w, double *ms);
  * device duration in microseconds.  Classes: "push" (particle phases),
  * "solve" (all transform launches), "deposit_push_B" etc. (see docs). */
 int qp_wake_profile_kernel(qp_wake w, const char *kernel_class, int *launches, double *mean_us);
+/* JSON description of the launch shapes the library picked for this wake
+ * (FFT rows/columns per block from the autotuner). */
+int qp_wake_describe(qp_wake w, char *buf, size_t len);
 /* Copy the current particle state back (FP64 SoA; any pointer may be NULL). */
 int qp_wake_get_particles(qp_wake w, size_t np, double *y, double *z, double *py, double *pz);
 

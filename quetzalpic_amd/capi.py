@@ -76,6 +76,7 @@ def load():
         "qp_wake_last_time_ms": ([vp, _dp], ctypes.c_int),
         "qp_wake_profile_kernel": ([vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
         "qp_wake_get_particles": ([vp, ctypes.c_size_t, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "qp_wake_describe": ([vp, ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -244,6 +245,11 @@ class Wake:
         _check(load().qp_wake_profile_kernel(self.h, cls.encode(), ctypes.byref(n), ctypes.byref(us)),
                "qp_wake_profile_kernel")
         return n.value, us.value
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        _check(load().qp_wake_describe(self.h, buf, 512), "qp_wake_describe")
+        return buf.value.decode()
 
     def get_particles(self):
         y, z, py, pz = (np.zeros(self.np) for _ in range(4))

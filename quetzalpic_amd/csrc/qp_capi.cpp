@@ -226,6 +226,7 @@ int qp_wake_set_particles(qp_wake h, size_t np, const double *y, const double *z
     if (py) w->h_py.assign(py, py + np);
     if (pz) w->h_pz.assign(pz, pz + np);
     w->particles_set = true;
+    w->particles_dirty = true;
     return QP_OK;
 }
 
@@ -311,6 +312,17 @@ int qp_wake_profile_kernel(qp_wake h, const char *kernel_class, int *launches, d
     return QP_OK;
 }
 
+int qp_wake_describe(qp_wake h, char *buf, size_t len) {
+    ARG_CHECK(h && buf && len > 0, "null argument");
+    const Wake *w = &h->w;
+    std::string s = "{\"rows_per_block\": " + std::to_string(w->S.rows_per_block) +
+                    ", \"cols_per_block\": " + std::to_string(w->S.cols_per_block) +
+                    ", \"precision\": " + std::to_string(w->p.precision) + "}";
+    std::strncpy(buf, s.c_str(), len - 1);
+    buf[len - 1] = 0;
+    return QP_OK;
+}
+
 int qp_wake_get_particles(qp_wake h, size_t np, double *y, double *z, double *py, double *pz) {
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;
@@ -341,6 +353,10 @@ int wake_alloc_common(Wake *w) {
     QP_HIP_CHECK(hipHostMalloc(&w->h_out, sizeof(double) * nn * 11, hipHostMallocDefault));
     QP_HIP_CHECK(hipEventCreate(&w->ev0));
     QP_HIP_CHECK(hipEventCreate(&w->ev1));
+    if (p.precision == QP_PREC_MIXED) {
+        QP_TRY(w->S.autotune(w->stream(), w->src));
+        QP_HIP_CHECK(hipStreamSynchronize(w->stream()));
+    }
     return QP_OK;
 }
 
@@ -350,6 +366,10 @@ void wake_free_particles(Wake *w) {
     for (double *a : arrs)
         if (a) (void)hipFree(a);
     w->pf = PartFP64{};
+    for (double *&a : w->init64) {
+        if (a) (void)hipFree(a);
+        a = nullptr;
+    }
 }
 
 const double *wake_rho_b(Wake *w, int i) { return w->rho_b3 ? w->rho_b3 + (size_t)i * w->nn() : nullptr; }

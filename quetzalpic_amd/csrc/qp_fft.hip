@@ -8,6 +8,8 @@
 // columns forward + divide + inverse (one block owns whole columns), rows
 // inverse (Hermitian rows packed two per complex transform).
 #include <cmath>
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "qp_fft_lds.h"
@@ -85,6 +87,66 @@ int Solver2d::init(int n1_, int n2_, double d1_, double d2_) {
     return QP_OK;
 }
 
+// Pick rows_per_block / cols_per_block for this grid by timing the row and
+// column passes (20 launches each) on scratch data; cached per (n1, n2).
+int Solver2d::autotune(hipStream_t s, double *scratch_real) {
+    static std::map<std::pair<int, int>, std::pair<int, int>> cache;
+    auto key = std::make_pair(n1, n2);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        rows_per_block = it->second.first;
+        cols_per_block = it->second.second;
+        return QP_OK;
+    }
+    hipEvent_t e0, e1;
+    QP_HIP_CHECK(hipEventCreate(&e0));
+    QP_HIP_CHECK(hipEventCreate(&e1));
+    QP_HIP_CHECK(hipMemsetAsync(scratch_real, 0, sizeof(double) * (size_t)n1 * n2, s));
+    float best = 1e30f;
+    int best_b = rows_per_block;
+    for (int b = 1; b <= 16 && b * n2 <= kLdsComplex; b *= 2) {
+        if (b > 1 && (n1 + 1) / 2 < b) break;
+        rows_per_block = b;
+        for (int w = 0; w < 3; ++w) QP_TRY(solve_rows_fwd_plain(*this, scratch_real, spec[0], s));
+        QP_HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < 20; ++r) {
+            QP_TRY(solve_rows_fwd_plain(*this, scratch_real, spec[0], s));
+            QP_TRY(solve_rows_inv_unscaled(*this, spec[0], scratch_real, s));
+        }
+        QP_HIP_CHECK(hipEventRecord(e1, s));
+        QP_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        QP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_b = b;
+        }
+    }
+    rows_per_block = best_b;
+    best = 1e30f;
+    int best_c = cols_per_block;
+    for (int c = 1; c <= 16 && c * n1 <= kLdsComplex; c *= 2) {
+        if (c > 1 && nh < c) break;
+        cols_per_block = c;
+        for (int w = 0; w < 3; ++w) QP_TRY(solve_cols(*this, 10.0, 1, s, nullptr, 0));
+        QP_HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < 20; ++r) QP_TRY(solve_cols(*this, 10.0, 1, s, nullptr, 0));
+        QP_HIP_CHECK(hipEventRecord(e1, s));
+        QP_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        QP_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_c = c;
+        }
+    }
+    cols_per_block = best_c;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cache[key] = std::make_pair(rows_per_block, cols_per_block);
+    return QP_OK;
+}
+
 void Solver2d::destroy() {
     ax1.destroy();
     ax2.destroy();
@@ -97,8 +159,15 @@ void Solver2d::destroy() {
 }
 
 // ------------------------------------------------------------ launches
-size_t rows_lds(const Solver2d &S) { return 2 * sizeof(double2) * (size_t)S.rows_per_block * S.n2; }
-size_t cols_lds(const Solver2d &S) { return 2 * sizeof(double2) * (size_t)S.cols_per_block * S.n1; }
+// LDS of a launch: two data buffers, the twiddle table (and for the column
+// pass the n1 eigenvalue terms).
+size_t rows_lds(const Solver2d &S) {
+    return 2 * sizeof(double2) * (size_t)S.rows_per_block * S.n2 + sizeof(double2) * (size_t)S.n2;
+}
+size_t cols_lds(const Solver2d &S) {
+    return 2 * sizeof(double2) * (size_t)S.cols_per_block * S.n1 + sizeof(double2) * (size_t)S.n1 +
+           sizeof(double) * (size_t)S.n1;
+}
 int rows_blocks(const Solver2d &S) {
     const int npairs = (S.n1 + 1) / 2;
     return (npairs + S.rows_per_block - 1) / S.rows_per_block;

@@ -19,10 +19,18 @@ __device__ __forceinline__ double2 tw_at(const double2 *tw, int t, int sign) {
     return w;
 }
 
+// Copy the axis twiddle table to LDS (call before the barrier that precedes
+// the first lds_fft).
+__device__ __forceinline__ void lds_load_tw(double2 *tw_lds, const FftAxisDev &ax) {
+    for (int t = threadIdx.x; t < ax.n; t += blockDim.x) tw_lds[t] = ax.tw[t];
+}
+
 // Transform `batch` sequences of length ax.n stored contiguously in `a`;
-// `b` is scratch of the same size.  All threads of the block must call it.
-// Returns the buffer holding the result (a or b).
-__device__ inline double2 *lds_fft(double2 *a, double2 *b, int batch, const FftAxisDev &ax, int sign) {
+// `b` is scratch of the same size; `twl` is the LDS copy of the twiddles.
+// All threads of the block must call it.  Returns the buffer holding the
+// result (a or b).
+__device__ inline double2 *lds_fft(double2 *a, double2 *b, int batch, const FftAxisDev &ax, const double2 *twl,
+                                   int sign) {
     const int n = ax.n;
     int Ns = 1;
     for (int s = 0; s < ax.nstages; ++s) {
@@ -38,9 +46,9 @@ __device__ inline double2 *lds_fft(double2 *a, double2 *b, int batch, const FftA
                 const int k = j % Ns;
                 double2 v0 = in[j], v1 = in[j + nR], v2 = in[j + 2 * nR], v3 = in[j + 3 * nR];
                 if (k != 0) {
-                    v1 = cmul(v1, tw_at(ax.tw, k * tws, sign));
-                    v2 = cmul(v2, tw_at(ax.tw, 2 * k * tws, sign));
-                    v3 = cmul(v3, tw_at(ax.tw, 3 * k * tws, sign));
+                    v1 = cmul(v1, tw_at(twl, k * tws, sign));
+                    v2 = cmul(v2, tw_at(twl, 2 * k * tws, sign));
+                    v3 = cmul(v3, tw_at(twl, 3 * k * tws, sign));
                 }
                 const double2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
                 const double2 t2 = cadd(v1, v3), t3 = csub(v1, v3);
@@ -60,7 +68,7 @@ __device__ inline double2 *lds_fft(double2 *a, double2 *b, int batch, const FftA
                 double2 *out = b + bb * n;
                 const int k = j % Ns;
                 double2 v0 = in[j], v1 = in[j + nR];
-                if (k != 0) v1 = cmul(v1, tw_at(ax.tw, k * tws, sign));
+                if (k != 0) v1 = cmul(v1, tw_at(twl, k * tws, sign));
                 const int o = (j / Ns) * Ns * 2 + k;
                 out[o] = cadd(v0, v1);
                 out[o + Ns] = csub(v0, v1);
@@ -78,7 +86,7 @@ __device__ inline double2 *lds_fft(double2 *a, double2 *b, int batch, const FftA
                 double2 acc = in[j];
                 for (int r = 1; r < R; ++r) {
                     const int ti = (int)(((long long)r * kq) % n);
-                    acc = cadd(acc, cmul(in[j + r * nR], tw_at(ax.tw, ti, sign)));
+                    acc = cadd(acc, cmul(in[j + r * nR], tw_at(twl, ti, sign)));
                 }
                 out[(j / Ns) * Ns * R + k + q * Ns] = acc;
             }

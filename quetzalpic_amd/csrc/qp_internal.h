@@ -30,7 +30,7 @@ void set_error(const std::string &msg);
 // ------------------------------------------------------------ FFT plan
 constexpr int kMaxStages = 24;
 constexpr int kMaxN = 1024;          // longest transform the LDS kernels take
-constexpr int kLdsComplex = 1536;    // complex doubles per LDS buffer (24 KiB; two buffers)
+constexpr int kLdsComplex = 1024;    // complex doubles per LDS data buffer (16 KiB; two buffers)
 
 // One transform axis as the kernels see it (passed by value).
 struct FftAxisDev {
@@ -59,6 +59,7 @@ struct Solver2d {
     int cols_per_block = 1;
     int init(int n1, int n2, double d1, double d2);
     int set_steps(double d1, double d2);
+    int autotune(hipStream_t s, double *scratch_real);
     void destroy();
 };
 

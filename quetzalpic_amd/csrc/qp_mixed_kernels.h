@@ -30,7 +30,7 @@ constexpr double kInvTwo32 = 1.0 / 4294967296.0;
 struct MxGeom {
     int n1, n2;
     double d1, d2, ly, lz, dx;
-    double dx_d1, dx_d2;  // dx/dy, dx/dz: position update in cell units
+    double inv_d1, inv_d2;  // 1/dy, 1/dz: the fused path multiplies instead of dividing
 };
 
 struct MxPart {
@@ -146,29 +146,30 @@ __device__ __forceinline__ double mx_gather_hy(const double *a, const Pos &q, in
     return gather4(a, n2, j1, j2, k1, k2, yf, zf);
 }
 // Staggered-derivative gathers (src/array_utils.cpp:240-288).
-__device__ __forceinline__ double mx_yder(const double *a, const Pos &q, int n1, int n2, double d1) {
+// The half-node differences are interpolated first and scaled once by 1/d.
+__device__ __forceinline__ double mx_yder(const double *a, const Pos &q, int n1, int n2, double inv_d1) {
     int j1, j2, k1, k2;
     double yf, zf;
     ax_half(q.j, q.fy, n1, j1, j2, yf);
     const int j3 = j2 + 1 == n1 ? 0 : j2 + 1;
     ax_node(q.k, q.fz, n2, k1, k2, zf);
-    const double g11 = (a[(size_t)n2 * j2 + k1] - a[(size_t)n2 * j1 + k1]) / d1;
-    const double g21 = (a[(size_t)n2 * j3 + k1] - a[(size_t)n2 * j2 + k1]) / d1;
-    const double g12 = (a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j1 + k2]) / d1;
-    const double g22 = (a[(size_t)n2 * j3 + k2] - a[(size_t)n2 * j2 + k2]) / d1;
-    return g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf;
+    const double g11 = a[(size_t)n2 * j2 + k1] - a[(size_t)n2 * j1 + k1];
+    const double g21 = a[(size_t)n2 * j3 + k1] - a[(size_t)n2 * j2 + k1];
+    const double g12 = a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j1 + k2];
+    const double g22 = a[(size_t)n2 * j3 + k2] - a[(size_t)n2 * j2 + k2];
+    return (g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf) * inv_d1;
 }
-__device__ __forceinline__ double mx_zder(const double *a, const Pos &q, int n1, int n2, double d2) {
+__device__ __forceinline__ double mx_zder(const double *a, const Pos &q, int n1, int n2, double inv_d2) {
     int j1, j2, k1, k2;
     double yf, zf;
     ax_node(q.j, q.fy, n1, j1, j2, yf);
     ax_half(q.k, q.fz, n2, k1, k2, zf);
     const int k3 = k2 + 1 == n2 ? 0 : k2 + 1;
-    const double g11 = (a[(size_t)n2 * j1 + k2] - a[(size_t)n2 * j1 + k1]) / d2;
-    const double g21 = (a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j2 + k1]) / d2;
-    const double g12 = (a[(size_t)n2 * j1 + k3] - a[(size_t)n2 * j1 + k2]) / d2;
-    const double g22 = (a[(size_t)n2 * j2 + k3] - a[(size_t)n2 * j2 + k2]) / d2;
-    return g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf;
+    const double g11 = a[(size_t)n2 * j1 + k2] - a[(size_t)n2 * j1 + k1];
+    const double g21 = a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j2 + k1];
+    const double g12 = a[(size_t)n2 * j1 + k3] - a[(size_t)n2 * j1 + k2];
+    const double g22 = a[(size_t)n2 * j2 + k3] - a[(size_t)n2 * j2 + k2];
+    return (g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf) * inv_d2;
 }
 
 // ------------------------------------------------------ LDS deposit window
@@ -187,6 +188,19 @@ __device__ __forceinline__ int rel_idx(int j, int ref, int n) {
     if (d < -(n / 2)) d += n;
     else if (d >= n - n / 2) d -= n;
     return d;
+}
+
+// Sum over the block (blockDim.x a multiple of 64, <= 1024); result in every thread.
+__device__ inline double block_sum(double v) {
+    __shared__ double part[16];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) part[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += part[i];
+    return s;
 }
 
 __device__ __forceinline__ int wave_min(int v) {

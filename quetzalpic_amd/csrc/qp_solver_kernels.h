@@ -73,8 +73,9 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *s
     if (solver_skip(ctl, skip_mode)) return;
     double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
     extern __shared__ double2 lds[];
-    double2 *a = lds, *b = lds + (size_t)B * n2;
+    double2 *a = lds, *b = lds + (size_t)B * n2, *twl = lds + (size_t)2 * B * n2;
     const int pair0 = blockIdx.x * B;
+    lds_load_tw(twl, ax);
     for (int t = threadIdx.x; t < B * n2; t += blockDim.x) {
         const int bb = t / n2, k = t - bb * n2;
         const int ra = 2 * (pair0 + bb), rb = ra + 1;
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *s
         a[t] = make_double2(xa, xb);
     }
     __syncthreads();
-    const double2 *r = lds_fft(a, b, B, ax, -1);
+    const double2 *r = lds_fft(a, b, B, ax, twl, -1);
     for (int t = threadIdx.x; t < B * nh; t += blockDim.x) {
         const int bb = t / nh, k = t - bb * nh;
         const int ra = 2 * (pair0 + bb), rb = ra + 1;
@@ -111,22 +112,25 @@ static __global__ __launch_bounds__(kSolverThreads) void k_cols_solve(double2 *s
     if (solver_skip(ctl, skip_mode)) return;
     extern __shared__ double2 lds[];
     double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
-    double2 *a = lds, *b = lds + (size_t)C * n1;
+    double2 *a = lds, *b = lds + (size_t)C * n1, *twl = lds + (size_t)2 * C * n1;
+    double *lam1l = reinterpret_cast<double *>(twl + n1);
     const int l0 = blockIdx.x * C;
+    lds_load_tw(twl, ax);
+    for (int t = threadIdx.x; t < n1; t += blockDim.x) lam1l[t] = lam1[t];
     for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
         const int c = t % C, j = t / C, l = l0 + c;
         a[c * n1 + j] = l < nh ? spec[(size_t)j * nh + l] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     if (mode == 1) {
-        double2 *o = lds_fft(a, b, C, ax, +1);
+        double2 *o = lds_fft(a, b, C, ax, twl, +1);
         for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
             const int c = t % C, j = t / C, l = l0 + c;
             if (l < nh) spec[(size_t)j * nh + l] = o[c * n1 + j];
         }
         return;
     }
-    double2 *r = lds_fft(a, b, C, ax, -1);
+    double2 *r = lds_fft(a, b, C, ax, twl, -1);
     if (mode == 0) {
         for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
             const int c = t % C, j = t / C, l = l0 + c;
@@ -141,14 +145,14 @@ static __global__ __launch_bounds__(kSolverThreads) void k_cols_solve(double2 *s
         if (m == 0 && l == 0) {
             v = make_double2(0.0, 0.0);
         } else {
-            const double mult = -D + lam1[m] + lam2[l];
+            const double mult = -D + lam1l[m] + lam2[l];
             v.x /= mult;
             v.y /= mult;
         }
         r[t] = v;
     }
     __syncthreads();
-    double2 *o = lds_fft(r, r == a ? b : a, C, ax, +1);
+    double2 *o = lds_fft(r, r == a ? b : a, C, ax, twl, +1);
     for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
         const int c = t % C, j = t / C, l = l0 + c;
         if (l < nh) spec[(size_t)j * nh + l] = o[c * n1 + j];
@@ -166,8 +170,9 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_inv(const double2 *spec
     const double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
     sink.begin();
     extern __shared__ double2 lds[];
-    double2 *a = lds, *b = lds + (size_t)B * n2;
+    double2 *a = lds, *b = lds + (size_t)B * n2, *twl = lds + (size_t)2 * B * n2;
     const int pair0 = blockIdx.x * B;
+    lds_load_tw(twl, ax);
     for (int t = threadIdx.x; t < B * n2; t += blockDim.x) {
         const int bb = t / n2, k = t - bb * n2;
         const int ra = 2 * (pair0 + bb), rb = ra + 1;
@@ -190,7 +195,7 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_inv(const double2 *spec
         a[t] = z;
     }
     __syncthreads();
-    const double2 *r = lds_fft(a, b, B, ax, +1);
+    const double2 *r = lds_fft(a, b, B, ax, twl, +1);
     for (int t = threadIdx.x; t < B * n2; t += blockDim.x) {
         const int bb = t / n2, k = t - bb * n2;
         const int ra = 2 * (pair0 + bb), rb = ra + 1;

@@ -27,10 +27,12 @@ struct Wake {
     size_t np = 0;
     size_t cap = 0;  // allocated particle capacity
     bool particles_set = false;
+    bool particles_dirty = true;  // host arrays changed since the device initial state was built
     // host copy of the initial particles (FP64), re-uploaded every pass
     std::vector<double> h_y, h_z, h_py, h_pz, h_w;
 
     PartFP64 pf;
+    double *init64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // y, z, py, pz, n of a pass
 
     // grids, ny*nz FP64 each
     double *g_base = nullptr;

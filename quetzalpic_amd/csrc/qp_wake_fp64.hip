@@ -305,19 +305,24 @@ int wake_upload_fp64(Wake *w) {
     hipStream_t s = w->stream();
     PartFP64 &P = w->pf;
     if (w->cap < np || !P.y) {
-        double **arrs[] = {&P.y, &P.z, &P.py, &P.pz, &P.n, &P.gamma, &P.px, &P.ym, &P.zm, &P.pyn, &P.pzn};
+        double **arrs[] = {&P.y, &P.z, &P.py, &P.pz, &P.n, &P.gamma, &P.px, &P.ym, &P.zm, &P.pyn, &P.pzn,
+                           &w->init64[0], &w->init64[1], &w->init64[2], &w->init64[3], &w->init64[4]};
         for (double **a : arrs) {
             if (*a) (void)hipFree(*a);
             QP_HIP_CHECK(hipMalloc(a, sizeof(double) * (np ? np : 1)));
         }
         w->cap = np;
+        w->particles_dirty = true;
     }
     const size_t b = sizeof(double) * np;
-    QP_HIP_CHECK(hipMemcpyAsync(P.y, w->h_y.data(), b, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(P.z, w->h_z.data(), b, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(P.py, w->h_py.data(), b, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(P.pz, w->h_pz.data(), b, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(P.n, w->h_w.data(), b, hipMemcpyHostToDevice, s));
+    if (w->particles_dirty) {
+        const std::vector<double> *src[5] = {&w->h_y, &w->h_z, &w->h_py, &w->h_pz, &w->h_w};
+        for (int i = 0; i < 5; ++i)
+            QP_HIP_CHECK(hipMemcpy(w->init64[i], src[i]->data(), b, hipMemcpyHostToDevice));
+        w->particles_dirty = false;
+    }
+    double *dst[5] = {P.y, P.z, P.py, P.pz, P.n};
+    for (int i = 0; i < 5; ++i) QP_HIP_CHECK(hipMemcpyAsync(dst[i], w->init64[i], b, hipMemcpyDeviceToDevice, s));
     // init_wake_particles: py_next = pz_next = 0, gamma = 0 (src/System_3d.cpp:740-742)
     QP_HIP_CHECK(hipMemsetAsync(P.pyn, 0, b, s));
     QP_HIP_CHECK(hipMemsetAsync(P.pzn, 0, b, s));

@@ -80,8 +80,9 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
             P.py[p] = pyv;
             P.pz[p] = pzv;
             const double ps = mx_gather(psim, q, n1, n2);
-            double Y = (q.j + q.fy) + (g.dx * (double)pyv / (1 + ps)) / g.d1;
-            double Z = (q.k + q.fz) + (g.dx * (double)pzv / (1 + ps)) / g.d2;
+            const double s = g.dx / (1 + ps);
+            double Y = (q.j + q.fy) + (s * (double)pyv) * g.inv_d1;
+            double Z = (q.k + q.fz) + (s * (double)pzv) * g.inv_d2;
             Y = wrap_cells(Y, n1);
             Z = wrap_cells(Z, n2);
             int j, k;
@@ -121,8 +122,9 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
 }
 
 // ------------------------------------------------------------------ B phase
-__device__ __forceinline__ double gamma_of(double p2, double a, double ps) {
-    return 0.5 * (1 + p2 + a + (1 + ps) * (1 + ps)) / (1 + ps);
+// gamma = (1 + p^2 + a + (1+psi)^2) / (2 (1+psi)) (src/System_3d.cpp:373-374), with 1/(1+psi) given.
+__device__ __forceinline__ double gamma_of(double p2, double a, double ps, double inv1ps) {
+    return 0.5 * (1 + p2 + a + (1 + ps) * (1 + ps)) * inv1ps;
 }
 
 __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, const double *psi, const double *by,
@@ -156,32 +158,34 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         const double py = P.py[p], pz = P.pz[p];
         const double a = mx_gather(asq, q, n1, n2);
         const double ps = mx_gather(psi, q, n1, n2);
+        const double inv1ps = 1.0 / (1 + ps);  // the FP64 1/(1+psi) factor
         double gm;
         if (first_iter) {
-            gm = gamma_of(py * py + pz * pz, a, ps);  // S4 (:373-374)
+            gm = gamma_of(py * py + pz * pz, a, ps, inv1ps);  // S4 (:373-374)
         } else {
             const double pys = P.pys[p], pzs = P.pzs[p];  // P6 of the previous iteration (:503-505)
-            gm = gamma_of(0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs), a, ps);
+            gm = gamma_of(0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs), a, ps, inv1ps);
         }
-        const double day = mx_yder(asq, q, n1, n2, g.d1);
-        const double daz = mx_zder(asq, q, n1, n2, g.d2);
-        const double dpy = mx_yder(psi, q, n1, n2, g.d1);
-        const double dpz = mx_zder(psi, q, n1, n2, g.d2);
+        const double day = mx_yder(asq, q, n1, n2, g.inv_d1);
+        const double daz = mx_zder(asq, q, n1, n2, g.inv_d2);
+        const double dpy = mx_yder(psi, q, n1, n2, g.inv_d1);
+        const double dpz = mx_zder(psi, q, n1, n2, g.inv_d2);
         const double byp = mx_gather_hz(by, q, n1, n2);
         const double bzp = mx_gather_hy(bz, q, n1, n2);
         const double dx = g.dx;
-        double pyn = py - dx * 0.5 * day / (1 + ps);  // P1 (:427-432)
-        pyn += dx * gm * dpy / (1 + ps);
+        const double s = dx * inv1ps;
+        double pyn = py - s * 0.5 * day;  // P1 (:427-432)
+        pyn += s * gm * dpy;
         pyn -= dx * bzp;
-        double pzn = pz - dx * 0.5 * daz / (1 + ps);
-        pzn += dx * gm * dpz / (1 + ps);
+        double pzn = pz - s * 0.5 * daz;
+        pzn += s * gm * dpz;
         pzn += dx * byp;
         const float pynf = (float)pyn, pznf = (float)pzn;
         P.pys[p] = pynf;
         P.pzs[p] = pznf;
         // P2 (:439-442) with the stored p*
-        double Ym = (q.j + q.fy) + (0.5 * dx * (double)pynf / (1 + ps)) / g.d1;
-        double Zm = (q.k + q.fz) + (0.5 * dx * (double)pznf / (1 + ps)) / g.d2;
+        double Ym = (q.j + q.fy) + (0.5 * s * (double)pynf) * g.inv_d1;
+        double Zm = (q.k + q.fz) + (0.5 * s * (double)pznf) * g.inv_d2;
         qm = pos_from_cells(wrap_cells(Ym, n1), wrap_cells(Zm, n2), n1, n2);
     }
     if (threadIdx.x == 0) {
@@ -230,21 +234,25 @@ __global__ __launch_bounds__(kPT) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, 
         const float pysf = P.pys[p], pzsf = P.pzs[p];
         const double pys = pysf, pzs = pzsf;
         const double ps = mx_gather(psi, q, n1, n2);
-        const double dx = g.dx;
-        double Ym = (q.j + q.fy) + (0.5 * dx * pys / (1 + ps)) / g.d1;
-        double Zm = (q.k + q.fz) + (0.5 * dx * pzs / (1 + ps)) / g.d2;
+        const double inv1ps = 1.0 / (1 + ps);
+        const double s = g.dx * inv1ps;
+        double Ym = (q.j + q.fy) + (0.5 * s * pys) * g.inv_d1;
+        double Zm = (q.k + q.fz) + (0.5 * s * pzs) * g.inv_d2;
         qm = pos_from_cells(wrap_cells(Ym, n1), wrap_cells(Zm, n2), n1, n2);
         const double psm = mx_gather(psim, qm, n1, n2);  // P4 (:483-485)
-        vjy = w * pys / (1 + psm);
-        vjz = w * pzs / (1 + psm);
+        const double wm = w / (1 + psm);
+        vjy = wm * pys;
+        vjz = wm * pzs;
         const double a = mx_gather(asq, q, n1, n2);  // P6 (:501-506)
         const double p2 = 0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs);
-        const double gm = gamma_of(p2, a, ps);
-        const double px = 0.5 * (1 + p2 + a - (1 + ps) * (1 + ps)) / (1 + ps);
-        const double vx = px / gm;  // P7 (:523-526)
-        vjx = w * vx / (1 - vx);
-        vrho = w / (1 - vx);
-        vchi = -w / (1 - vx) / gm;
+        const double gm = gamma_of(p2, a, ps, inv1ps);
+        const double px = 0.5 * (1 + p2 + a - (1 + ps) * (1 + ps)) * inv1ps;
+        // P7 (:523-526): vx = px/gamma, 1/(1-vx) = gamma/(gamma-px) = gamma/(1+psi)
+        const double vx = px / gm;
+        const double inv1mvx = 1.0 / (1 - vx);
+        vjx = w * vx * inv1mvx;
+        vrho = w * inv1mvx;
+        vchi = -w * inv1mvx / gm;
     }
     if (threadIdx.x == 0) {
         sh[4 * (kPT / 64)] = q.j;
@@ -302,12 +310,14 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
         const double py = P.py[p], pz = P.pz[p];
         const double ps = mx_gather(psi, q, n1, n2);
         const double a = mx_gather(asq, q, n1, n2);
-        const double gm = gamma_of(py * py + pz * pz, a, ps);
-        const double px = 0.5 * (1 + py * py + pz * pz + a - (1 + ps) * (1 + ps)) / (1 + ps);
+        const double inv1ps = 1.0 / (1 + ps);
+        const double gm = gamma_of(py * py + pz * pz, a, ps, inv1ps);
+        const double px = 0.5 * (1 + py * py + pz * pz + a - (1 + ps) * (1 + ps)) * inv1ps;
         const double vx = px / gm;
-        vjx = w * vx / (1 - vx);
-        vrho = w / (1 - vx);
-        vchi = -w / (1 - vx) / gm;
+        const double inv1mvx = 1.0 / (1 - vx);
+        vjx = w * vx * inv1mvx;
+        vrho = w * inv1mvx;
+        vchi = -w * inv1mvx / gm;
     }
     if (threadIdx.x == 0) {
         sh[4 * (kPT / 64)] = q.j;
@@ -415,19 +425,16 @@ struct SinkPsiFinal {
     double thr;
     double b;
     __device__ void begin() {
-        __shared__ double shb;
-        if (threadIdx.x == 0) {
-            double s1 = 0.0, s2 = 0.0;
-            for (int i = 0; i < ncb; ++i) {
-                s1 += bpart[2 * i];
-                s2 += bpart[2 * i + 1];
-            }
-            const double g00 = bpart[2 * ncb];
-            const double sraw = s1 + (double)n2 * g00 - s2;
-            shb = sraw / n1 / n2 / (n1 + n2 - 1);
+        double s1 = 0.0, s2 = 0.0;
+        for (int i = threadIdx.x; i < ncb; i += blockDim.x) {
+            s1 += bpart[2 * i];
+            s2 += bpart[2 * i + 1];
         }
-        __syncthreads();
-        b = shb;
+        s1 = block_sum(s1);
+        s2 = block_sum(s2);
+        const double g00 = bpart[2 * ncb];
+        const double sraw = s1 + (double)n2 * g00 - s2;
+        b = sraw / n1 / n2 / (n1 + n2 - 1);
     }
     __device__ void operator()(int j, int k, double v) const {
         double r = v / n1 / n2 - b;
@@ -462,14 +469,17 @@ __global__ __launch_bounds__(kSolverThreads) void k_cols_psi(double2 *spec, int 
                                                              double *bpart, const Ctl *ctl) {
     if (solver_skip(ctl, 1)) return;
     extern __shared__ double2 lds2[];
-    double2 *a = lds2, *b = lds2 + (size_t)C * n1;
+    double2 *a = lds2, *b = lds2 + (size_t)C * n1, *twl = lds2 + (size_t)2 * C * n1;
+    double *lam1l = reinterpret_cast<double *>(twl + n1);
     const int l0 = blockIdx.x * C;
+    lds_load_tw(twl, ax);
+    for (int t = threadIdx.x; t < n1; t += blockDim.x) lam1l[t] = lam1[t];
     for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
         const int c = t % C, j = t / C, l = l0 + c;
         a[c * n1 + j] = l < nh ? spec[(size_t)j * nh + l] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    double2 *r = lds_fft(a, b, C, ax, -1);
+    double2 *r = lds_fft(a, b, C, ax, twl, -1);
     for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
         const int c = t / n1, m = t - c * n1, l = l0 + c;
         if (l >= nh) continue;
@@ -477,40 +487,31 @@ __global__ __launch_bounds__(kSolverThreads) void k_cols_psi(double2 *spec, int 
         if (m == 0 && l == 0) {
             v = make_double2(0.0, 0.0);
         } else {
-            const double mult = -0.0 + lam1[m] + lam2[l];
+            const double mult = -0.0 + lam1l[m] + lam2[l];
             v.x /= mult;
             v.y /= mult;
         }
         r[t] = v;
     }
     __syncthreads();
-    double2 *o = lds_fft(r, r == a ? b : a, C, ax, +1);
+    double2 *o = lds_fft(r, r == a ? b : a, C, ax, twl, +1);
     for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
         const int c = t % C, j = t / C, l = l0 + c;
         if (l < nh) spec[(size_t)j * nh + l] = o[c * n1 + j];
     }
-    // partial sums, one column per thread, then thread 0 combines
-    __shared__ double colp[2 * 64];
-    if (threadIdx.x < C) {
-        const int c = threadIdx.x, l = l0 + c;
-        double cs = 0.0, g0 = 0.0;
-        if (l < nh) {
-            for (int j = 0; j < n1; ++j) cs += o[c * n1 + j].x;
-            g0 = o[c * n1].x;
-            const double wl = (l == 0 || 2 * l == n2) ? 1.0 : 2.0;
-            cs *= wl;
-            g0 *= wl;
-        }
-        colp[2 * c] = cs;
-        colp[2 * c + 1] = g0;
+    // partial sums over the whole block: sum_c w_l sum_j Re G[j][l] and sum_c w_l Re G[0][l]
+    double s1 = 0.0, s2 = 0.0;
+    for (int t = threadIdx.x; t < C * n1; t += blockDim.x) {
+        const int c = t / n1, j = t - c * n1, l = l0 + c;
+        if (l >= nh) continue;
+        const double wl = (l == 0 || 2 * l == n2) ? 1.0 : 2.0;
+        const double v = wl * o[t].x;
+        s1 += v;
+        if (j == 0) s2 += v;
     }
-    __syncthreads();
+    s1 = block_sum(s1);
+    s2 = block_sum(s2);
     if (threadIdx.x == 0) {
-        double s1 = 0.0, s2 = 0.0;
-        for (int c = 0; c < C; ++c) {
-            s1 += colp[2 * c];
-            s2 += colp[2 * c + 1];
-        }
         bpart[2 * blockIdx.x] = s1;
         bpart[2 * blockIdx.x + 1] = s2;
         if (blockIdx.x == 0) bpart[2 * gridDim.x] = o[0].x;
@@ -591,8 +592,9 @@ __global__ void k_sort_copyback(MxPart P, MxPart Q, int *perm, const int *perm2)
 
 // ------------------------------------------------------------------ host
 struct MixedState {
-    MxPart P{}, Q{};
-    int *perm = nullptr, *perm2 = nullptr;
+    MxPart P{}, Q{}, I{};   // live, re-binning scratch, initial state of a pass
+    int *perm = nullptr, *perm2 = nullptr, *perm0 = nullptr;
+    bool init_valid = false;
     unsigned *count = nullptr, *cursor = nullptr;
     double *bpart = nullptr;
     double *jbuf[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][y/z]
@@ -630,8 +632,9 @@ void wake_destroy_mixed(Wake *w) {
     if (!m) return;
     free_part(m->P);
     free_part(m->Q);
-    void *arrs[] = {m->perm, m->perm2, m->count, m->cursor, m->bpart, m->jbuf[0][0], m->jbuf[0][1], m->jbuf[1][0],
-                    m->jbuf[1][1]};
+    free_part(m->I);
+    void *arrs[] = {m->perm, m->perm2, m->perm0, m->count, m->cursor, m->bpart, m->jbuf[0][0], m->jbuf[0][1],
+                    m->jbuf[1][0], m->jbuf[1][1]};
     for (void *a : arrs)
         if (a) (void)hipFree(a);
     for (auto &g : m->gexec)
@@ -662,18 +665,23 @@ static int ensure_state(Wake *w) {
     if (m->cap < w->np || !m->P.cell) {
         free_part(m->P);
         free_part(m->Q);
+        free_part(m->I);
         if (m->perm) (void)hipFree(m->perm);
         if (m->perm2) (void)hipFree(m->perm2);
+        if (m->perm0) (void)hipFree(m->perm0);
         QP_TRY(alloc_part(m->P, w->np));
         QP_TRY(alloc_part(m->Q, w->np));
+        QP_TRY(alloc_part(m->I, w->np));
         QP_HIP_CHECK(hipMalloc(&m->perm, sizeof(int) * (w->np ? w->np : 1)));
         QP_HIP_CHECK(hipMalloc(&m->perm2, sizeof(int) * (w->np ? w->np : 1)));
+        QP_HIP_CHECK(hipMalloc(&m->perm0, sizeof(int) * (w->np ? w->np : 1)));
         m->cap = w->np;
+        m->init_valid = false;
         for (auto &g : m->gexec)
             if (g) (void)hipGraphExecDestroy(g);
         m->gexec[0] = m->gexec[1] = nullptr;
     }
-    m->P.np = m->Q.np = (int)w->np;
+    m->P.np = m->Q.np = m->I.np = (int)w->np;
     return QP_OK;
 }
 
@@ -692,9 +700,11 @@ static inline void to_cells(double y, double d, int n, int &j, uint32_t &o) {
     o = (uint32_t)u;
 }
 
-int wake_upload_mixed(Wake *w) {
-    QP_TRY(ensure_state(w));
-    MixedState *m = mstate(w);
+// Convert the host FP64 particles to the device layout once (sorted by
+// 16x16-cell tile, row-major inside a tile) into the pass's initial-state
+// buffers; every pass then starts with device-to-device copies, as the
+// reference re-initialises its particles each pass (src/System_3d.cpp:262).
+static int build_initial_state(Wake *w, MixedState *m) {
     const size_t np = w->np;
     const int n1 = w->p.ny, n2 = w->p.nz;
     const int ntk = (n2 + kTile - 1) / kTile;
@@ -722,17 +732,35 @@ int wake_upload_mixed(Wake *w) {
         pz[s] = (float)w->h_pz[i];
         wt[s] = (float)w->h_w[i];
     }
+    QP_HIP_CHECK(hipMemcpy(m->I.cell, c2.data(), 4 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.oy, y2.data(), 4 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.oz, z2.data(), 4 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.py, py.data(), 4 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.pz, pz.data(), 4 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 4 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->perm0, m->h_perm.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    m->init_valid = true;
+    return QP_OK;
+}
+
+int wake_upload_mixed(Wake *w) {
+    QP_TRY(ensure_state(w));
+    MixedState *m = mstate(w);
+    if (w->particles_dirty || !m->init_valid) {
+        QP_TRY(build_initial_state(w, m));
+        w->particles_dirty = false;
+    }
+    const size_t np = w->np;
     hipStream_t s = w->stream();
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.cell, c2.data(), 4 * np, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.oy, y2.data(), 4 * np, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.oz, z2.data(), 4 * np, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.py, py.data(), 4 * np, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.pz, pz.data(), 4 * np, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.w, wt.data(), 4 * np, hipMemcpyHostToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.cell, m->I.cell, 4 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.oy, m->I.oy, 4 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.oz, m->I.oz, 4 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.py, m->I.py, 4 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.pz, m->I.pz, 4 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.w, m->I.w, 4 * np, hipMemcpyDeviceToDevice, s));
     QP_HIP_CHECK(hipMemsetAsync(m->P.pys, 0, 4 * np, s));  // py_next = pz_next = 0 (src/System_3d.cpp:740-741)
     QP_HIP_CHECK(hipMemsetAsync(m->P.pzs, 0, 4 * np, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->perm, m->h_perm.data(), sizeof(int) * np, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipStreamSynchronize(s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->perm, m->perm0, sizeof(int) * np, hipMemcpyDeviceToDevice, s));
     return QP_OK;
 }
 
@@ -899,7 +927,7 @@ int wake_run_mixed(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
     R.np = (int)w->np;
     R.pb = pblocks(R.np);
     R.gb = (int)((R.nn + 255) / 256);
-    R.g = MxGeom{p.ny, p.nz, p.dy, p.dz, p.ly, p.lz, p.dx, p.dx / p.dy, p.dx / p.dz};
+    R.g = MxGeom{p.ny, p.nz, p.dy, p.dz, p.ly, p.lz, p.dx, 1.0 / p.dy, 1.0 / p.dz};
     for (int sl = 0; sl < 2; ++sl) R.si[sl] = MxSliceIn{w->a_sqr3, w->asq2, w->rho_b3, w->jx_b3, sl};
     R.ldsC = sizeof(double) * 5 * kWinCap;
     R.ldsS = sizeof(double) * 3 * kWinCap;

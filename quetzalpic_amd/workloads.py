@@ -1,0 +1,94 @@
+"""Synthetic workloads of the quasistatic slice loop (plasma units).
+
+Each workload is a periodic transverse box [0, L)^2 with the driver on its
+centre, ppcy = ppcz = 2, K = 5 predictor-corrector iterations.  Mapping of the
+target configurations to the reference's input keys
+(src/Config_reader.cpp:109-136): ysigma = zsigma = 2 w0 (full 1/e^2 widths),
+xsigma = sqrt(8) sigma_xi, x0 = lx - 3 sigma_xi.  The grid rule is
+n = (int)(l/d), d = l/n (src/System_3d.cpp:87-95).
+"""
+import math
+
+import numpy as np
+
+
+class Workload:
+    def __init__(self, name, lx, dx, L, n, a0, w0, sigma, profile=("constant", -1.0), ppc=2, K=5,
+                 thermal=0.0, seed=1234, note=""):
+        self.name = name
+        self.lx, self.dx0 = lx, dx
+        self.L, self.n0 = L, n
+        self.a0, self.w0, self.sigma = a0, w0, sigma
+        self.profile = profile
+        self.ppc = ppc
+        self.K = K
+        self.thermal = thermal
+        self.seed = seed
+        self.note = note
+        # grid rule (src/System_3d.cpp:87-95)
+        self.nx = int(lx / dx)
+        self.ny = self.nz = int(L / (L / n))
+        self.dx = lx / self.nx
+        self.dy = self.dz = L / self.ny
+
+    @property
+    def laser(self):
+        """(a0, xsigma, ysigma, zsigma, x0, y0, z0) in the reference's keys."""
+        return (self.a0, math.sqrt(8.0) * self.sigma, 2 * self.w0, 2 * self.w0, self.lx - 3 * self.sigma,
+                self.L / 2, self.L / 2)
+
+    def profile_value(self, y, z):
+        kind = self.profile[0]
+        if kind == "constant":
+            return np.full(np.broadcast(y, z).shape, self.profile[1])
+        if kind == "channel":  # -(base + curv r^2) about the box centre (extension, no reference counterpart)
+            base, curv = self.profile[1], self.profile[2]
+            r2 = (y - self.L / 2) ** 2 + (z - self.L / 2) ** 2
+            return -(base + curv * r2)
+        raise ValueError(kind)
+
+    def particles(self):
+        """init_wake_particles (src/System_3d.cpp:724-748): lattice
+        ((i+1/2) dy/ppcy, (j+1/2) dz/ppcz), weight profile/ppcy/ppcz, i outer."""
+        ppc = self.ppc
+        i = np.arange(ppc * self.ny, dtype=np.float64)
+        j = np.arange(ppc * self.nz, dtype=np.float64)
+        y = np.repeat((i + 0.5) * self.dy / ppc, len(j))
+        z = np.tile((j + 0.5) * self.dz / ppc, len(i))
+        v = self.profile_value(y, z)
+        keep = v != 0
+        y, z, v = y[keep], z[keep], v[keep]
+        w = v / float(ppc) / float(ppc)
+        py = pz = None
+        if self.thermal > 0:
+            rng = np.random.default_rng(self.seed)
+            py = rng.normal(0.0, self.thermal, len(y))
+            pz = rng.normal(0.0, self.thermal, len(y))
+        return y, z, w, py, pz
+
+    @property
+    def n_particles(self):
+        return (self.ppc * self.ny) * (self.ppc * self.nz)
+
+    def describe(self):
+        return {
+            "workload": self.name, "grid": f"{self.ny}x{self.nz}", "box": self.L, "slices": self.nx,
+            "dxi": self.dx, "particles": self.n_particles, "ppc": self.ppc * self.ppc, "K": self.K,
+            "a0": self.a0, "w0": self.w0, "sigma_xi": self.sigma, "profile": list(self.profile),
+            "note": self.note,
+        }
+
+
+def get(name):
+    if name == "C1":
+        return Workload("C1: 64x64 over [-10,10]^2, 200 slices dxi=0.05, uniform n0=1, 2x2 ppc, laser a0=1 w0=3 "
+                        "sigma=1.5", lx=10.0, dx=0.05, L=20.0, n=64, a0=1.0, w0=3.0, sigma=1.5)
+    if name == "C2":
+        return Workload("C2: 256x256 over [-12,12]^2, 4 ppc (262144 particles), 1000 slices dxi=0.02, channel "
+                        "n=1+0.05r^2, laser a0=2 w0=4 sigma=2", lx=20.0, dx=0.02, L=24.0, n=256, a0=2.0, w0=4.0,
+                        sigma=2.0, profile=("channel", 1.0, 0.05))
+    if name == "C3":
+        return Workload("C3: 512x512 over [-15,15]^2, 4 ppc (1048576 particles), 2000 slices dxi=0.01, n0=1, "
+                        "laser a0=4 w0=3.5 sigma=1.5, thermal p N(0,1e-3)", lx=20.0, dx=0.01, L=30.0, n=512,
+                        a0=4.0, w0=3.5, sigma=1.5, thermal=1e-3)
+    raise KeyError(name)
