@@ -1,0 +1,45 @@
+// Config_reader.h -- TOML input deck -> System_parameters with the reference's
+// keys, defaults and errors (src/Config_reader.h:9-14, src/Config_reader.cpp:18-258).
+#pragma once
+
+#include <exception>
+#include <iostream>
+#include <string>
+
+#include "System_parameters.h"
+#include "toml_lite.h"
+
+class Config_exception : public std::exception {
+public:
+    explicit Config_exception(std::string message) : message_(std::move(message)) {}
+    const char *what() const noexcept override { return message_.c_str(); }
+
+private:
+    std::string message_;
+};
+
+class Config_reader {
+public:
+    Config_reader(const std::string &filename, std::ostream &out);
+    // Parse TOML text directly (no file).
+    static Config_reader from_string(const std::string &text, std::ostream &out);
+    System_parameters get_parameters() { return params; }
+
+private:
+    Config_reader(toml_lite::TablePtr config, std::ostream &out);
+    void read_all();
+    template <class T>
+    T read_value(const std::string &name, const toml_lite::TablePtr &table);
+    template <class T>
+    T read_value(const std::string &name, T default_value, const toml_lite::TablePtr &table);
+    void init_laser();
+    Bunch_parameters parse_bunch_table(const toml_lite::TablePtr &t);
+    void init_bunch();
+    void init_output_parameters();
+    void init_plasma_profile();
+    void init_device();
+
+    System_parameters params;
+    toml_lite::TablePtr config;
+    std::ostream &out;
+};
