@@ -107,8 +107,7 @@ int qp_fourier2d_forward(qp_fourier2d f, const double *in_real, double *out_comp
     hipStream_t s = f->ctx->stream;
     const Solver2d &S = f->S;
     QP_HIP_CHECK(hipMemcpyAsync(f->d_buf, in_real, sizeof(double) * S.n1 * S.n2, hipMemcpyHostToDevice, s));
-    QP_TRY(solve_rows_fwd_plain(S, f->d_buf, S.spec[0], s));
-    // column forward only: reuse the column kernel path through a pure forward launch
+    QP_TRY(fourier_rows_fwd(S, f->d_buf, S.spec[0], s));
     QP_TRY(solve_cols_forward_only(S, s));
     QP_HIP_CHECK(hipMemcpyAsync(out_complex, S.spec[0], sizeof(double2) * S.n1 * S.nh, hipMemcpyDeviceToHost, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
@@ -121,7 +120,7 @@ int qp_fourier2d_backward(qp_fourier2d f, const double *in_complex, double *out_
     const Solver2d &S = f->S;
     QP_HIP_CHECK(hipMemcpyAsync(S.spec[0], in_complex, sizeof(double2) * S.n1 * S.nh, hipMemcpyHostToDevice, s));
     QP_TRY(solve_cols_inverse_only(S, s));
-    QP_TRY(solve_rows_inv_unscaled(S, S.spec[0], f->d_buf, s));
+    QP_TRY(fourier_rows_inv(S, S.spec[0], f->d_buf, s));
     QP_HIP_CHECK(hipMemcpyAsync(out_real, f->d_buf, sizeof(double) * S.n1 * S.n2, hipMemcpyDeviceToHost, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     return QP_OK;
@@ -317,7 +316,8 @@ int qp_wake_describe(qp_wake h, char *buf, size_t len) {
     const Wake *w = &h->w;
     std::string s = "{\"rows_per_block\": " + std::to_string(w->S.rows_per_block) +
                     ", \"cols_per_block\": " + std::to_string(w->S.cols_per_block) +
-                    ", \"precision\": " + std::to_string(w->p.precision) + "}";
+                    ", \"precision\": " + std::to_string(w->p.precision) +
+                    ", \"fft\": \"" + (w->S.wave ? "wave" : "generic") + "\"}";
     std::strncpy(buf, s.c_str(), len - 1);
     buf[len - 1] = 0;
     return QP_OK;

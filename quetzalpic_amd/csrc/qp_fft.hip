@@ -6,14 +6,20 @@
 //   -D + (2cos(2 pi j/n1) - 2)/d1^2 + (2cos(2 pi k/n2) - 2)/d2^2 -> c2r.
 // Three launches: rows forward (two real rows packed per complex transform),
 // columns forward + divide + inverse (one block owns whole columns), rows
-// inverse (Hermitian rows packed two per complex transform).
+// inverse (Hermitian rows packed two per complex transform).  Power-of-two
+// grids from 16 to 1024 per side use the register-resident kernels of
+// qp_wave_solver.h; other sizes, and Fourier2d's transforms (FFTW's r2c
+// layout), use the generic LDS kernels of qp_solver_kernels.h.
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <utility>
 #include <vector>
 
 #include "qp_fft_lds.h"
 #include "qp_solver_kernels.h"
+#include "qp_wave_solver.h"
 
 namespace qp {
 
@@ -75,6 +81,9 @@ int Solver2d::init(int n1_, int n2_, double d1_, double d2_) {
     QP_HIP_CHECK(hipMalloc(&d_lam2, sizeof(double) * nh));
     QP_TRY(set_steps(d1_, d2_));
     for (int i = 0; i < 2; ++i) QP_HIP_CHECK(hipMalloc(&spec[i], sizeof(double2) * (size_t)n1 * nh));
+    // QP_FFT=generic keeps the generic kernels for every size (A/B comparisons).
+    const char *sel = getenv("QP_FFT");
+    wave = (wf_supported(n1) && wf_supported(n2) && !(sel && strcmp(sel, "generic") == 0)) ? 1 : 0;
     const int npairs = (n1 + 1) / 2;
     int b = npairs / 64;
     if (b < 1) b = 1;
@@ -90,6 +99,7 @@ int Solver2d::init(int n1_, int n2_, double d1_, double d2_) {
 // Pick rows_per_block / cols_per_block for this grid by timing the row and
 // column passes (20 launches each) on scratch data; cached per (n1, n2).
 int Solver2d::autotune(hipStream_t s, double *scratch_real) {
+    if (wave) return QP_OK;  // the register-resident kernels have a fixed shape
     static std::map<std::pair<int, int>, std::pair<int, int>> cache;
     auto key = std::make_pair(n1, n2);
     auto it = cache.find(key);
@@ -173,8 +183,8 @@ int rows_blocks(const Solver2d &S) {
     return (npairs + S.rows_per_block - 1) / S.rows_per_block;
 }
 
-int solve_rows_fwd_plain_ctl(const Solver2d &S, const double *src, double2 *spec, hipStream_t s,
-                             const Ctl *ctl, int skip_mode) {
+static int gen_rows_fwd(const Solver2d &S, const double *src, double2 *spec, hipStream_t s, const Ctl *ctl,
+                        int skip_mode) {
     SrcPlain f{src, S.n2};
     hipLaunchKernelGGL(k_rows_fwd<SrcPlain>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, f,
                        spec, spec, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, skip_mode);
@@ -182,8 +192,18 @@ int solve_rows_fwd_plain_ctl(const Solver2d &S, const double *src, double2 *spec
     return QP_OK;
 }
 
+int solve_rows_fwd_plain_ctl(const Solver2d &S, const double *src, double2 *spec, hipStream_t s,
+                             const Ctl *ctl, int skip_mode) {
+    if (S.wave) return wlaunch_rows_fwd(S, SrcPlain{src, S.n2}, spec, spec, 1, s, ctl, skip_mode);
+    return gen_rows_fwd(S, src, spec, s, ctl, skip_mode);
+}
+
 int solve_rows_fwd_plain(const Solver2d &S, const double *src, double2 *spec, hipStream_t s) {
     return solve_rows_fwd_plain_ctl(S, src, spec, s, nullptr, 0);
+}
+
+int fourier_rows_fwd(const Solver2d &S, const double *src, double2 *spec, hipStream_t s) {
+    return gen_rows_fwd(S, src, spec, s, nullptr, 0);
 }
 
 static int cols_launch(const Solver2d &S, double D, int nspec, hipStream_t s, const Ctl *ctl, int skip_mode,
@@ -197,6 +217,7 @@ static int cols_launch(const Solver2d &S, double D, int nspec, hipStream_t s, co
 }
 
 int solve_cols(const Solver2d &S, double D, int nspec, hipStream_t s, const Ctl *ctl, int skip_mode) {
+    if (S.wave) return wlaunch_cols(S, D, nspec, nullptr, s, ctl, skip_mode);
     return cols_launch(S, D, nspec, s, ctl, skip_mode, 2);
 }
 
@@ -204,7 +225,7 @@ int solve_cols_forward_only(const Solver2d &S, hipStream_t s) { return cols_laun
 
 int solve_cols_inverse_only(const Solver2d &S, hipStream_t s) { return cols_launch(S, 0.0, 1, s, nullptr, 0, 1); }
 
-int solve_rows_inv_unscaled(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s) {
+int fourier_rows_inv(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s) {
     SinkRaw f{dst, S.n2};
     hipLaunchKernelGGL(k_rows_inv<SinkRaw>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
                        spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)nullptr, 0);
@@ -212,9 +233,15 @@ int solve_rows_inv_unscaled(const Solver2d &S, const double2 *spec, double *dst,
     return QP_OK;
 }
 
+int solve_rows_inv_unscaled(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s) {
+    if (S.wave) return wlaunch_rows_inv(S, spec, spec, SinkRaw{dst, S.n2}, 1, s, (const Ctl *)nullptr, 0);
+    return fourier_rows_inv(S, spec, dst, s);
+}
+
 int solve_rows_inv_plain_ctl(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s,
                              const Ctl *ctl) {
     SinkScaled f{dst, S.n1, S.n2};
+    if (S.wave) return wlaunch_rows_inv(S, spec, spec, f, 1, s, ctl, ctl ? 1 : 0);
     hipLaunchKernelGGL(k_rows_inv<SinkScaled>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
                        spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, ctl ? 1 : 0);
     QP_HIP_CHECK(hipGetLastError());
@@ -228,6 +255,7 @@ int solve_rows_inv_plain(const Solver2d &S, const double2 *spec, double *dst, hi
 int solve_rows_inv_bcheck(const Solver2d &S, const double2 *spec, double *dst, double thr, Ctl *ctl, int slice,
                           hipStream_t s) {
     SinkBCheck f{dst, S.n1, S.n2, thr, ctl, slice};
+    if (S.wave) return wlaunch_rows_inv(S, spec, spec, f, 1, s, (const Ctl *)ctl, 2);
     hipLaunchKernelGGL(k_rows_inv<SinkBCheck>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
                        spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)ctl, 2);
     QP_HIP_CHECK(hipGetLastError());

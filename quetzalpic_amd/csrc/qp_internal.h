@@ -57,6 +57,10 @@ struct Solver2d {
     double2 *spec[2] = {nullptr, nullptr};  // n1 x nh half spectra (two for paired solves)
     int rows_per_block = 1;   // complex row transforms (= 2 real rows) per block
     int cols_per_block = 1;
+    // 1: both sides 2^p in [16, 1024] -> register-resident FFT kernels with the
+    // column-major spectrum spec[l * n1 + j]; 0: generic kernels, row-major
+    // spec[j * nh + l] (FFTW's r2c layout).  Fourier2d always uses the latter.
+    int wave = 0;
     int init(int n1, int n2, double d1, double d2);
     int set_steps(double d1, double d2);
     int autotune(hipStream_t s, double *scratch_real);
@@ -206,6 +210,9 @@ int rows_blocks(const Solver2d &S);
 int solve_cols_forward_only(const Solver2d &S, hipStream_t s);
 int solve_cols_inverse_only(const Solver2d &S, hipStream_t s);
 int solve_rows_inv_unscaled(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s);
+// Fourier2d transforms in FFTW's layout (generic kernels whatever S.wave says).
+int fourier_rows_fwd(const Solver2d &S, const double *src, double2 *spec, hipStream_t s);
+int fourier_rows_inv(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s);
 int solve_rows_fwd_plain_ctl(const Solver2d &S, const double *src, double2 *spec, hipStream_t s,
                              const Ctl *ctl, int skip_mode);
 int solve_rows_inv_plain_ctl(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s,

@@ -262,6 +262,11 @@ __device__ __forceinline__ int win_slot(const Win &w, int j, int k, int n1, int 
     return dj * w.wk + dk;
 }
 
+// LDS-typed pointer: atomics through it are ds_add_f64.  With a generic
+// pointer the compiler may merge the LDS and global branches of dep4 into one
+// flat atomic, which is much slower on LDS.
+typedef __attribute__((address_space(3))) double lds_f64;
+
 // One CIC deposit of value v at (j1..j2, k1..k2) into grid g (LDS window or global).
 __device__ __forceinline__ void dep4(double *lds, const Win &w, double *glob, int n1, int n2, int j1, int j2, int k1,
                                      int k2, double yf, double zf, double v) {
@@ -271,10 +276,11 @@ __device__ __forceinline__ void dep4(double *lds, const Win &w, double *glob, in
         const int s11 = win_slot(w, j1, k1, n1, n2), s21 = win_slot(w, j2, k1, n1, n2);
         const int s12 = win_slot(w, j1, k2, n1, n2), s22 = win_slot(w, j2, k2, n1, n2);
         if (s11 >= 0 && s21 >= 0 && s12 >= 0 && s22 >= 0) {
-            atomicAdd(&lds[s11], a11);
-            atomicAdd(&lds[s21], a21);
-            atomicAdd(&lds[s12], a12);
-            atomicAdd(&lds[s22], a22);
+            lds_f64 *l = (lds_f64 *)lds;
+            __hip_atomic_fetch_add(l + s11, a11, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(l + s21, a21, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(l + s12, a12, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(l + s22, a22, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             return;
         }
     }

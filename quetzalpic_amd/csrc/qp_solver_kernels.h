@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *s
 // (mode 2); mode 0 = forward only, mode 1 = inverse only (Fourier2d's separate
 // forward/backward transforms).  blockIdx.y selects spec0 / spec1 (paired
 // solves with the same D).
-static __global__ __launch_bounds__(kSolverThreads) void k_cols_solve(double2 *spec0, double2 *spec1, int n1, int nh,
+[[maybe_unused]] static __global__ __launch_bounds__(kSolverThreads) void k_cols_solve(double2 *spec0, double2 *spec1, int n1, int nh,
                                                                FftAxisDev ax, const double *lam1,
                                                                const double *lam2, double D, int C,
                                                                const Ctl *ctl, int skip_mode, int mode) {

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "qp_mixed_kernels.h"
+#include "qp_wave_solver.h"
 
 namespace qp {
 
@@ -649,7 +650,8 @@ static int ensure_state(Wake *w) {
     if (!m) {
         m = new MixedState();
         w->mixed = m;
-        const int ncb = (w->S.nh + w->S.cols_per_block - 1) / w->S.cols_per_block;
+        const int ncb = w->S.wave ? wcols_blocks(w->S.n1, w->S.nh)
+                                  : (w->S.nh + w->S.cols_per_block - 1) / w->S.cols_per_block;
         if (w->S.cols_per_block > 64) {
             set_error("column block too wide for the border partial sums");
             return QP_ERR_STATE;
@@ -814,8 +816,22 @@ int psi_solve(MxRun &R, hipStream_t s, int slot, double *dst) {
     Wake *w = R.w;
     const Solver2d &S = w->S;
     MixedState *m = R.m;
-    prof_begin(w, "solve");
     SrcPsi src{w->src, w->rho_ion, w->rho_b3, w->jx_b3, w->ctl, slot, S.n2, R.nn};
+    SinkPsiFinal snk{dst, m->bpart, m->ncb, S.n1, S.n2, w->p.psi_threshold - 1, 0.0};
+    if (S.wave) {
+        prof_begin(w, "solve");
+        QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[0], 1, s, (const Ctl *)w->ctl, 1));
+        prof_end(w, "solve");
+        prof_begin(w, "solve");
+        QP_TRY(wlaunch_cols(S, 0.0, 1, m->bpart, s, (const Ctl *)w->ctl, 1));
+        prof_end(w, "solve");
+        prof_begin(w, "solve");
+        QP_TRY(wlaunch_rows_inv(S, (const double2 *)S.spec[0], (const double2 *)S.spec[0], snk, 1, s,
+                                (const Ctl *)w->ctl, 1));
+        prof_end(w, "solve");
+        return QP_OK;
+    }
+    prof_begin(w, "solve");
     MLAUNCH(k_rows_fwd<SrcPsi>, rows_blocks(S), kSolverThreads, rows_lds(S), src, S.spec[0], S.spec[0], S.n1, S.n2,
             S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 1);
     prof_end(w, "solve");
@@ -824,7 +840,6 @@ int psi_solve(MxRun &R, hipStream_t s, int slot, double *dst) {
             (const double *)S.d_lam1, (const double *)S.d_lam2, S.cols_per_block, m->bpart, (const Ctl *)w->ctl);
     prof_end(w, "solve");
     prof_begin(w, "solve");
-    SinkPsiFinal snk{dst, m->bpart, m->ncb, S.n1, S.n2, w->p.psi_threshold - 1, 0.0};
     MLAUNCH(k_rows_inv<SinkPsiFinal>, rows_blocks(S), kSolverThreads, rows_lds(S), (const double2 *)S.spec[0],
             (const double2 *)S.spec[0], snk, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 1);
     prof_end(w, "solve");
@@ -838,8 +853,22 @@ int b_solve(MxRun &R, hipStream_t s, int slot) {
     const qp_wake_params &p = w->p;
     double *jy = m->jbuf[1 - slot][0], *jz = m->jbuf[1 - slot][1];
     double *jyn = m->jbuf[slot][0], *jzn = m->jbuf[slot][1];
-    prof_begin(w, "solve");
     SrcB src{w->by, w->bz, w->jx, jy, jz, jyn, jzn, S.n1, S.n2, p.D, p.dx, p.dy, p.dz};
+    SinkBPair snk{w->by, w->bz, S.n1, S.n2, p.b_threshold, w->ctl, slot};
+    if (S.wave) {
+        prof_begin(w, "solve");
+        QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[1], 2, s, (const Ctl *)w->ctl, 2));
+        prof_end(w, "solve");
+        prof_begin(w, "solve");
+        QP_TRY(wlaunch_cols(S, p.D, 2, nullptr, s, (const Ctl *)w->ctl, 2));
+        prof_end(w, "solve");
+        prof_begin(w, "solve");
+        QP_TRY(wlaunch_rows_inv(S, (const double2 *)S.spec[0], (const double2 *)S.spec[1], snk, 2, s,
+                                (const Ctl *)w->ctl, 2));
+        prof_end(w, "solve");
+        return QP_OK;
+    }
+    prof_begin(w, "solve");
     hipLaunchKernelGGL(k_rows_fwd<SrcB>, dim3(rows_blocks(S), 2), dim3(kSolverThreads), rows_lds(S), s, src,
                        S.spec[0], S.spec[1], S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 2);
     QP_HIP_CHECK(hipGetLastError());
@@ -848,7 +877,6 @@ int b_solve(MxRun &R, hipStream_t s, int slot) {
     QP_TRY(solve_cols(S, p.D, 2, s, w->ctl, 2));
     prof_end(w, "solve");
     prof_begin(w, "solve");
-    SinkBPair snk{w->by, w->bz, S.n1, S.n2, p.b_threshold, w->ctl, slot};
     hipLaunchKernelGGL(k_rows_inv<SinkBPair>, dim3(rows_blocks(S), 2), dim3(kSolverThreads), rows_lds(S), s,
                        (const double2 *)S.spec[0], (const double2 *)S.spec[1], snk, S.n1, S.n2, S.nh, S.ax2.dev,
                        S.rows_per_block, (const Ctl *)w->ctl, 2);

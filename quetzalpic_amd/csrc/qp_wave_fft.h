@@ -1,0 +1,236 @@
+// qp_wave_fft.h -- register-resident complex FFTs of power-of-two length N in
+// [16, 1024] (device code).
+//
+// E elements per lane (E in {4, 8, 16}), L = N/E lanes per transform, a
+// block of TB threads carries T = TB/L transforms.  Lane j of a transform
+// holds element j + L*m in register m, on entry and on exit (natural order
+// both ways), so a forward transform, a pointwise operation and an inverse
+// transform chain without an exchange.  Stockham passes: the first has radix
+// E and works on the registers; each later pass with sub-transform size Ns
+// and radix R = min(E, N/Ns) has butterflies jj = j + L b (b < E/R) reading
+// in[jj + r N/R], twiddled by w^{k r N/(Ns R)} (k = jj mod Ns), writing
+// out[(jj/Ns) Ns R + k + q Ns]; the last pass lands in registers.  Passes
+// exchange through two LDS buffers (one barrier per pass).
+#pragma once
+
+#include <type_traits>
+
+#include "qp_internal.h"
+
+namespace qp {
+
+template <int N, int E, int TB>
+struct WfShape {
+    static_assert(N >= 16 && N <= 1024 && (N & (N - 1)) == 0, "wave FFT length must be 2^p in [16, 1024]");
+    static_assert(E == 4 || E == 8 || E == 16, "elements per lane must be 4, 8 or 16");
+    static constexpr int L = N / E;                      // lanes per transform
+    static_assert(TB % 64 == 0 && TB >= L && TB % L == 0, "block must hold whole transforms");
+    static constexpr int T = TB / L;                     // transforms per block
+    static constexpr int PADN = N + N / 16;              // padded LDS length of one transform
+    static constexpr int LDS = T * PADN;
+    // two exchange buffers + the twiddle table within 64 KiB of LDS
+    static_assert((2 * LDS + N) * 16 <= 65536, "transforms per block exceed the LDS budget");
+};
+
+// One pad slot every 16 elements keeps 16-byte LDS accesses of consecutive
+// lanes on distinct banks.
+__device__ __forceinline__ int wf_pad(int i) { return i + (i >> 4); }
+
+// cos / sin of 2 pi m / 16 (cos pi/8, sin pi/8, sqrt(1/2)).
+constexpr double kC1 = 0.92387953251128675613;
+constexpr double kS1 = 0.38268343236508977173;
+constexpr double kH = 0.70710678118654752440;
+constexpr double cos16(int m) {
+    switch (m & 15) {
+        case 0: return 1.0;
+        case 1: return kC1;
+        case 2: return kH;
+        case 3: return kS1;
+        case 4: return 0.0;
+        case 5: return -kS1;
+        case 6: return -kH;
+        case 7: return -kC1;
+        case 8: return -1.0;
+        case 9: return -kC1;
+        case 10: return -kH;
+        case 11: return -kS1;
+        case 12: return 0.0;
+        case 13: return kS1;
+        case 14: return kH;
+        default: return kC1;
+    }
+}
+constexpr double sin16(int m) { return cos16(m - 4); }
+
+// (S i) * a
+template <int S>
+__device__ __forceinline__ double2 mul_si(double2 a) {
+    return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// a * e^{S 2 pi i M / 16}
+template <int M, int S>
+__device__ __forceinline__ double2 w16(double2 a) {
+    constexpr int m = M & 15;
+    if constexpr (m == 0) {
+        return a;
+    } else if constexpr (m == 4) {
+        return mul_si<S>(a);
+    } else if constexpr (m == 8) {
+        return make_double2(-a.x, -a.y);
+    } else if constexpr (m == 12) {
+        return mul_si<-S>(a);
+    } else {
+        constexpr double c = cos16(m);
+        constexpr double s = S * sin16(m);
+        return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, double2 &v3) {
+    const double2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
+    const double2 t2 = cadd(v1, v3), t3 = mul_si<S>(csub(v1, v3));
+    v0 = cadd(t0, t2);
+    v1 = cadd(t1, t3);
+    v2 = csub(t0, t2);
+    v3 = csub(t1, t3);
+}
+
+// In-register DFT of length R in {1, 2, 4, 8, 16}: u[q] <- sum_r u[r] e^{S 2 pi i r q / R}.
+// Radix 8 and 16 split as r = r1 + (R/4 or 4) r2 with internal twiddles.
+template <int R, int S>
+__device__ __forceinline__ void dft(double2 (&u)[R]) {
+    if constexpr (R == 2) {
+        const double2 a = u[0], b = u[1];
+        u[0] = cadd(a, b);
+        u[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        dft4<S>(u[0], u[1], u[2], u[3]);
+    } else if constexpr (R == 8) {
+        // r = r1 + 2 r2, q = q2 + 4 q1
+        double2 a0 = u[0], a1 = u[2], a2 = u[4], a3 = u[6];
+        double2 b0 = u[1], b1 = u[3], b2 = u[5], b3 = u[7];
+        dft4<S>(a0, a1, a2, a3);
+        dft4<S>(b0, b1, b2, b3);
+        b1 = w16<2, S>(b1);
+        b2 = w16<4, S>(b2);
+        b3 = w16<6, S>(b3);
+        u[0] = cadd(a0, b0);
+        u[4] = csub(a0, b0);
+        u[1] = cadd(a1, b1);
+        u[5] = csub(a1, b1);
+        u[2] = cadd(a2, b2);
+        u[6] = csub(a2, b2);
+        u[3] = cadd(a3, b3);
+        u[7] = csub(a3, b3);
+    } else if constexpr (R == 16) {
+        // r = r1 + 4 r2, q = q2 + 4 q1
+        double2 a[4][4];
+#pragma unroll
+        for (int r1 = 0; r1 < 4; ++r1) {
+            a[r1][0] = u[r1];
+            a[r1][1] = u[r1 + 4];
+            a[r1][2] = u[r1 + 8];
+            a[r1][3] = u[r1 + 12];
+            dft4<S>(a[r1][0], a[r1][1], a[r1][2], a[r1][3]);
+        }
+        a[1][1] = w16<1, S>(a[1][1]);
+        a[1][2] = w16<2, S>(a[1][2]);
+        a[1][3] = w16<3, S>(a[1][3]);
+        a[2][1] = w16<2, S>(a[2][1]);
+        a[2][2] = w16<4, S>(a[2][2]);
+        a[2][3] = w16<6, S>(a[2][3]);
+        a[3][1] = w16<3, S>(a[3][1]);
+        a[3][2] = w16<6, S>(a[3][2]);
+        a[3][3] = w16<9, S>(a[3][3]);
+#pragma unroll
+        for (int q2 = 0; q2 < 4; ++q2) {
+            double2 c0 = a[0][q2], c1 = a[1][q2], c2 = a[2][q2], c3 = a[3][q2];
+            dft4<S>(c0, c1, c2, c3);
+            u[q2] = c0;
+            u[q2 + 4] = c1;
+            u[q2 + 8] = c2;
+            u[q2 + 12] = c3;
+        }
+    }
+}
+
+// Twiddle e^{S 2 pi i t / N} from the table e^{-2 pi i t / N}.
+template <int S>
+__device__ __forceinline__ double2 wf_tw(const double2 *twl, int t) {
+    double2 w = twl[t];
+    if constexpr (S > 0) w.y = -w.y;
+    return w;
+}
+
+// Passes after the first: bin holds sub-transforms of size NS.
+template <int N, int E, int S, int NS>
+__device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2 *bout, const double2 *twl, int j) {
+    constexpr int L = N / E;
+    constexpr int REM = N / NS;
+    constexpr int R = REM < E ? REM : E;
+    constexpr bool LAST = (R == REM);
+    constexpr int NB = E / R;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int jj = j + L * b;
+        const int k = jj & (NS - 1);
+        double2 u[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = bin[wf_pad(jj + r * (N / R))];
+        if (k != 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) u[r] = cmul(u[r], wf_tw<S>(twl, k * r * (N / (NS * R))));
+        }
+        dft<R, S>(u);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) v[b + NB * q] = u[q];
+        } else {
+            const int base = (jj / NS) * NS * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) bout[wf_pad(base + q * NS)] = u[q];
+        }
+    }
+    if constexpr (!LAST) {
+        __syncthreads();
+        wf_passes<N, E, S, NS * R>(v, bout, bin, twl, j);
+    }
+}
+
+// Transform the E registers of every lane (see the header comment).  buf0 /
+// buf1: this transform's LDS regions (PADN each); twl: the N-entry table
+// e^{-2 pi i t/N} in LDS, written before the call (the first barrier inside
+// orders it).  Every thread of the block must call it.
+template <int N, int E, int S>
+__device__ __forceinline__ void wave_fft(double2 (&v)[E], double2 *buf0, double2 *buf1, const double2 *twl, int j) {
+    dft<E, S>(v);
+    if constexpr (N > E) {
+        __syncthreads();  // earlier readers of buf0 are done
+#pragma unroll
+        for (int q = 0; q < E; ++q) buf0[wf_pad(E * j + q)] = v[q];
+        __syncthreads();
+        wf_passes<N, E, S, E>(v, buf0, buf1, twl, j);
+    }
+}
+
+// Run f(std::integral_constant<int, N>) for the wave-FFT length n; returns
+// false when n is not 2^p in [16, 1024].
+template <class F>
+inline bool wf_dispatch(int n, F &&f) {
+    switch (n) {
+        case 16: f(std::integral_constant<int, 16>{}); return true;
+        case 32: f(std::integral_constant<int, 32>{}); return true;
+        case 64: f(std::integral_constant<int, 64>{}); return true;
+        case 128: f(std::integral_constant<int, 128>{}); return true;
+        case 256: f(std::integral_constant<int, 256>{}); return true;
+        case 512: f(std::integral_constant<int, 512>{}); return true;
+        case 1024: f(std::integral_constant<int, 1024>{}); return true;
+        default: return false;
+    }
+}
+
+inline bool wf_supported(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+
+}  // namespace qp

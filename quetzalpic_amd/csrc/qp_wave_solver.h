@@ -1,0 +1,263 @@
+// qp_wave_solver.h -- the three launches of the periodic 2D solve built on the
+// register-resident FFTs of qp_wave_fft.h (both grid sides 2^p in [16, 1024]).
+//
+// Spectrum layout: column-major half spectrum spec[l * n1 + j] (l < nh =
+// n2/2 + 1 along the row axis, j < n1), so the column pass reads and writes
+// contiguous columns and the row passes move 2T consecutive rows per
+// frequency.  Same sources / sinks as the generic kernels of
+// qp_solver_kernels.h (functors with operator()(j, k) and begin()).
+#pragma once
+
+#include "qp_solver_kernels.h"
+#include "qp_wave_fft.h"
+
+namespace qp {
+
+// Launch shape per transform length: 4 elements per lane (more waves in
+// flight beat fewer exchanges: at 256^2 the psi solve in a dependent chain
+// measured 17.7 us with E = 4 against 23.1 us with E = 8 and 34.2 us with
+// E = 16) and the threads per block that measured best at 256^2, 512^2 and
+// 1024^2 (128, 256, 256); smaller grids use one wavefront per block.
+template <int N>
+struct WfCfg {
+    static constexpr int E = 4;
+    static constexpr int TB = N >= 512 ? 256 : (N == 256 ? 128 : 64);
+};
+
+// Sum of three values over a block of TB threads (every thread gets the sums).
+template <int TB>
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+    }
+    if constexpr (TB > 64) {
+        __shared__ double part[3][TB / 64];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        __syncthreads();
+        if (lane == 0) {
+            part[0][wv] = a;
+            part[1][wv] = b;
+            part[2][wv] = c;
+        }
+        __syncthreads();
+        a = b = c = 0.0;
+        for (int i = 0; i < TB / 64; ++i) {
+            a += part[0][i];
+            b += part[1][i];
+            c += part[2][i];
+        }
+    }
+}
+
+// Rows forward: rows (2p, 2p+1) packed as z = x_a + i x_b, complex FFT along
+// n2, X_a[k] = (Z[k] + conj Z[-k])/2, X_b[k] = (Z[k] - conj Z[-k])/(2i).
+// blockIdx.y selects spec0 / spec1.
+template <int N2, int E, int TB, class Src>
+__global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, double2 *spec1, int n1,
+                                                  const double2 *tw2, const Ctl *ctl, int skip_mode) {
+    using W = WfShape<N2, E, TB>;
+    if (solver_skip(ctl, skip_mode)) return;
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N2];
+    double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
+    const int tt = tid / W::L, j = tid % W::L;
+    const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
+    double2 v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int k = j + W::L * r;
+        double xa = 0.0, xb = 0.0;
+        if (ra < n1) xa = src(ra, k);
+        if (rb < n1) xb = src(rb, k);
+        v[r] = make_double2(xa, xb);
+    }
+    wave_fft<N2, E, -1>(v, sb0 + tt * W::PADN, sb1 + tt * W::PADN, twl, j);
+    // Z in natural order to LDS (after the barrier no lane reads sb0 any more)
+    double2 *zb = sb0;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) zb[tt * W::PADN + wf_pad(j + W::L * m)] = v[m];
+    __syncthreads();
+    // outputs: thread = t2 + T kk writes frequencies kk + L i of row pair t2
+    const int t2 = tid % W::T, kk = tid / W::T;
+    const int qa = 2 * (blockIdx.x * W::T + t2), qb = qa + 1;
+    if (qa >= n1) return;
+    const double2 *z = zb + t2 * W::PADN;
+#pragma unroll
+    for (int i = 0; i <= E / 2; ++i) {
+        const int k = kk + W::L * i;
+        if (k > N2 / 2) break;
+        const double2 Z = z[wf_pad(k)];
+        const double2 Zm = z[wf_pad((N2 - k) & (N2 - 1))];
+        const double cr = Zm.x, ci = -Zm.y;
+        spec[(size_t)k * n1 + qa] = make_double2(0.5 * (Z.x + cr), 0.5 * (Z.y + ci));
+        if (qb < n1) spec[(size_t)k * n1 + qb] = make_double2(0.5 * (Z.y - ci), -0.5 * (Z.x - cr));
+    }
+}
+
+// Columns: forward along n1, (0,0) := 0, divide by -D + lam1[m] + lam2[l],
+// inverse.  With bpart != nullptr (psi solves) also the border-mean partial
+// sums of the later c2r output x (unnormalised):
+//   sum_j x[j][0]      = sum_l c_l n1 Re H[0][l]
+//   x[0][0]            = sum_l c_l Re G[0][l],  G[0][l] = sum_m H[m][l]
+//   sum_k x[0][k]      = n2 Re G[0][0]
+// (H = divided spectrum, G = its inverse column transform, c_l = 1 for l = 0
+// and l = n2/2, else 2); per block: bpart[2b] = first sum, bpart[2b+1] =
+// second, and block 0 writes Re G[0][0] to bpart[2 gridDim.x].
+template <int N1, int E, int TB>
+__global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, int nh, int n2, const double2 *tw1,
+                                              const double *lam1, const double *lam2, double D, double *bpart,
+                                              const Ctl *ctl, int skip_mode) {
+    using W = WfShape<N1, E, TB>;
+    if (solver_skip(ctl, skip_mode)) return;
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N1];
+    double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N1; t += TB) twl[t] = tw1[t];
+    const int tt = tid / W::L, j = tid % W::L;
+    const int l = blockIdx.x * W::T + tt;
+    const bool valid = l < nh;
+    double2 v[E];
+    double lm[E];
+    const double2 *col = spec + (size_t)l * N1;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        v[r] = valid ? col[j + W::L * r] : make_double2(0.0, 0.0);
+        lm[r] = lam1[j + W::L * r];
+    }
+    const double l2 = valid ? lam2[l] : 0.0;
+    double2 *b0 = sb0 + tt * W::PADN, *b1 = sb1 + tt * W::PADN;
+    wave_fft<N1, E, -1>(v, b0, b1, twl, j);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        if (!valid || (l == 0 && j == 0 && m == 0)) {
+            v[m] = make_double2(0.0, 0.0);
+        } else {
+            const double mult = -D + lm[m] + l2;
+            v[m].x /= mult;
+            v[m].y /= mult;
+        }
+    }
+    if (bpart) {
+        const double wl = (l == 0 || 2 * l == n2) ? 1.0 : 2.0;
+        double g = 0.0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) g += v[m].x;
+        double s1 = (valid && j == 0) ? wl * (double)N1 * v[0].x : 0.0;
+        double s2 = valid ? wl * g : 0.0;
+        double g00 = (l == 0) ? g : 0.0;
+        block_sum3<TB>(s1, s2, g00);
+        if (tid == 0) {
+            bpart[2 * blockIdx.x] = s1;
+            bpart[2 * blockIdx.x + 1] = s2;
+            if (blockIdx.x == 0) bpart[2 * gridDim.x] = g00;
+        }
+    }
+    wave_fft<N1, E, +1>(v, b0, b1, twl, j);
+    if (!valid) return;
+    double2 *out = spec + (size_t)l * N1;
+#pragma unroll
+    for (int m = 0; m < E; ++m) out[j + W::L * m] = v[m];
+}
+
+// Rows inverse (c2r): Z[k] = X[k] + i Y[k] with X[-k] = conj X[k]; the
+// imaginary parts of the k = 0 and k = n2/2 terms are ignored (FFTW c2r
+// semantics).  Re z -> row 2p, Im z -> row 2p+1, handed to the sink.
+template <int N2, int E, int TB, class Sink>
+__global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const double2 *spec1, Sink sink, int n1,
+                                                  const double2 *tw2, const Ctl *ctl, int skip_mode) {
+    using W = WfShape<N2, E, TB>;
+    if (solver_skip(ctl, skip_mode)) return;
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N2];
+    const double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+    sink.begin();
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
+    const int tt = tid / W::L, j = tid % W::L;
+    const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
+    double2 v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int k = j + W::L * r;
+        const bool cj = k > N2 / 2;
+        const int kk = cj ? N2 - k : k;
+        double2 X = make_double2(0.0, 0.0), Y = X;
+        if (ra < n1) X = spec[(size_t)kk * n1 + ra];
+        if (rb < n1) Y = spec[(size_t)kk * n1 + rb];
+        if (cj) {
+            X.y = -X.y;
+            Y.y = -Y.y;
+        }
+        if (kk == 0 || 2 * kk == N2) {
+            X.y = 0.0;
+            Y.y = 0.0;
+        }
+        v[r] = make_double2(X.x - Y.y, X.y + Y.x);
+    }
+    wave_fft<N2, E, +1>(v, sb0 + tt * W::PADN, sb1 + tt * W::PADN, twl, j);
+    if (ra >= n1) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int k = j + W::L * m;
+        sink(ra, k, v[m].x);
+        if (rb < n1) sink(rb, k, v[m].y);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+// Blocks along x of the row passes / the column pass for transforms per block T.
+inline int wrows_blocks_t(int n1, int T) { return ((n1 + 1) / 2 + T - 1) / T; }
+inline int wcols_blocks_t(int nh, int T) { return (nh + T - 1) / T; }
+
+inline int wcols_blocks(int n1, int nh) {
+    int nb = 0;
+    wf_dispatch(n1, [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        nb = wcols_blocks_t(nh, WfShape<N, WfCfg<N>::E, WfCfg<N>::TB>::T);
+    });
+    return nb;
+}
+
+template <class Src>
+int wlaunch_rows_fwd(const Solver2d &S, const Src &src, double2 *spec0, double2 *spec1, int nspec, hipStream_t s,
+                     const Ctl *ctl, int skip_mode) {
+    wf_dispatch(S.n2, [&](auto NC) {
+        constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
+        const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec);
+        hipLaunchKernelGGL((kw_rows_fwd<N, E, TB, Src>), grid, dim3(TB), 0, s, src, spec0, spec1, S.n1,
+                           (const double2 *)S.ax2.dev.tw, ctl, skip_mode);
+    });
+    QP_HIP_CHECK(hipGetLastError());
+    return QP_OK;
+}
+
+inline int wlaunch_cols(const Solver2d &S, double D, int nspec, double *bpart, hipStream_t s, const Ctl *ctl,
+                        int skip_mode) {
+    wf_dispatch(S.n1, [&](auto NC) {
+        constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
+        const dim3 grid(wcols_blocks_t(S.nh, WfShape<N, E, TB>::T), nspec);
+        hipLaunchKernelGGL((kw_cols<N, E, TB>), grid, dim3(TB), 0, s, S.spec[0], S.spec[1], S.nh, S.n2,
+                           (const double2 *)S.ax1.dev.tw, (const double *)S.d_lam1, (const double *)S.d_lam2, D,
+                           bpart, ctl, skip_mode);
+    });
+    QP_HIP_CHECK(hipGetLastError());
+    return QP_OK;
+}
+
+template <class Sink>
+int wlaunch_rows_inv(const Solver2d &S, const double2 *spec0, const double2 *spec1, const Sink &sink, int nspec,
+                     hipStream_t s, const Ctl *ctl, int skip_mode) {
+    wf_dispatch(S.n2, [&](auto NC) {
+        constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
+        const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec);
+        hipLaunchKernelGGL((kw_rows_inv<N, E, TB, Sink>), grid, dim3(TB), 0, s, spec0, spec1, sink, S.n1,
+                           (const double2 *)S.ax2.dev.tw, ctl, skip_mode);
+    });
+    QP_HIP_CHECK(hipGetLastError());
+    return QP_OK;
+}
+
+}  // namespace qp

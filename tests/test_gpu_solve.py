@@ -54,3 +54,23 @@ def test_solve2d_single_mode_closed_form(qp_ctx):
     got = f.solve_poisson_equation(np.ones((n1, n2)), d1, d2, 10.0)
     assert np.abs(got).max() < 1e-12
     f.close()
+
+
+@pytest.mark.parametrize("n1,n2", [(16, 16), (32, 64), (256, 256), (128, 1024), (1024, 1024), (512, 32)])
+def test_wave_and_generic_kernels_agree(qp_ctx, n1, n2, monkeypatch):
+    """Power-of-two grids use the one-wavefront kernels; QP_FFT=generic forces the
+    generic LDS kernels on the same grid.  Both must give the same solve."""
+    from quetzalpic_amd import capi
+    rng = np.random.default_rng(n1 + 3 * n2)
+    src = rng.standard_normal((n1, n2))
+    d1, d2 = 24.0 / n1, 24.0 / n2
+    fw = capi.Fourier2d(qp_ctx, n1, n2)
+    monkeypatch.setenv("QP_FFT", "generic")
+    fg = capi.Fourier2d(qp_ctx, n1, n2)
+    monkeypatch.delenv("QP_FFT")
+    for D in (0.0, 10.0):
+        a = fw.solve_poisson_equation(src, d1, d2, D)
+        b = fg.solve_poisson_equation(src, d1, d2, D)
+        assert rel_err(a, b)[0] < 1e-14
+    fw.close()
+    fg.close()

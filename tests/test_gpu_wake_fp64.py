@@ -38,3 +38,14 @@ def test_fp64_wake_matches_oracle(qp_ctx, oracle, K):
     for f in FIELDS:
         mx, rms = rel_err(got[f], want[f])
         assert mx < 1e-10, (f, mx, rms)
+
+
+def test_fp64_wake_generic_fft_grid(qp_ctx, oracle):
+    """A 24 x 24 grid is not a power of two: the solves take the generic LDS kernels."""
+    g, y, z, w, a = small_case(oracle, n=24)
+    want, st0, _ = oracle.wake_run(g, y, z, w, a)
+    got, st1, _ = run_gpu(qp_ctx, g, y, z, w, a)
+    assert st0 == st1 == 0
+    for f in FIELDS:
+        mx, rms = rel_err(got[f], want[f])
+        assert mx < 1e-10, (f, mx, rms)

@@ -4,6 +4,8 @@ Bar (from the target statement): max and RMS relative error <= 1e-5 on psi and
 on e_x (the longitudinal field; the reference's `ex`).  The other fields are
 reported and held to looser bounds set by the FP32 particle state.
 """
+import json
+
 import numpy as np
 import pytest
 
@@ -52,6 +54,18 @@ def test_mixed_matches_oracle(qp_ctx, oracle, K):
     g, y, z, w, a = case(oracle, K=K)
     want, st0, _ = oracle.wake_run(g, y, z, w, a)
     wk, got, st1, _ = run_mixed(qp_ctx, g, y, z, w, a)
+    assert json.loads(wk.describe())["fft"] == "wave"
+    wk.close()
+    assert st0 == st1 == 0
+    check(got, want)
+
+
+def test_mixed_generic_fft_grid(qp_ctx, oracle):
+    """A 24 x 24 grid is not a power of two: the solves take the generic LDS kernels."""
+    g, y, z, w, a = case(oracle, n=24)
+    want, st0, _ = oracle.wake_run(g, y, z, w, a)
+    wk, got, st1, _ = run_mixed(qp_ctx, g, y, z, w, a)
+    assert json.loads(wk.describe())["fft"] == "generic"
     wk.close()
     assert st0 == st1 == 0
     check(got, want)
