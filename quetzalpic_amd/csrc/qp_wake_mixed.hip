@@ -110,12 +110,16 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
     const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
     win_zero(lds, 1, win);
     __syncthreads();
-    if (valid) {
+    {
         int j1, j2, k1, k2;
         double yf, zf;
         ax_node(q.j, q.fy, n1, j1, j2, yf);
         ax_node(q.k, q.fz, n2, k1, k2, zf);
-        dep4(lds, win, src_acc, n1, n2, j1, j2, k1, k2, yf, zf, -w);
+        double a[4];
+        cic_w(-w, yf, zf, a);
+        bool act = valid;
+        quad_fold<4>(j1 * n2 + k1, act, a);
+        if (act) dep4w(lds, win, src_acc, n1, n2, j1, j2, k1, k2, a[0], a[1], a[2], a[3]);
     }
     __syncthreads();
     double *gl[1] = {src_acc};
@@ -200,12 +204,16 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
     win_zero(lds, 1, win);
     __syncthreads();
-    if (valid) {  // P3 (:453-457)
+    {  // P3 (:453-457)
         int j1, j2, k1, k2;
         double yf, zf;
         ax_node(qm.j, qm.fy, n1, j1, j2, yf);
         ax_node(qm.k, qm.fz, n2, k1, k2, zf);
-        dep4(lds, win, src_acc, n1, n2, j1, j2, k1, k2, yf, zf, -w);
+        double a[4];
+        cic_w(-w, yf, zf, a);
+        bool act = valid;
+        quad_fold<4>(j1 * n2 + k1, act, a);
+        if (act) dep4w(lds, win, src_acc, n1, n2, j1, j2, k1, k2, a[0], a[1], a[2], a[3]);
     }
     __syncthreads();
     double *gl[1] = {src_acc};
@@ -278,12 +286,24 @@ __global__ __launch_bounds__(kPT) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, 
     const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
     win_zero(lds, 5, win);
     __syncthreads();
-    if (valid) {
-        dep4(lds, win, jyn, n1, n2, ya1, ya2, za1, za2, yaf, zaf, vjy);
-        dep4(lds + kWinCap, win, jzn, n1, n2, yb1, yb2, zb1, zb2, ybf, zbf, vjz);
-        dep4(lds + 2 * kWinCap, win, rho, n1, n2, yc1, yc2, zc1, zc2, ycf, zcf, vrho);
-        dep4(lds + 3 * kWinCap, win, jx, n1, n2, yc1, yc2, zc1, zc2, ycf, zcf, vjx);
-        dep4(lds + 4 * kWinCap, win, chi, n1, n2, yc1, yc2, zc1, zc2, ycf, zcf, vchi);
+    {
+        double aa[4], ab[4], ac[12];
+        cic_w(vjy, yaf, zaf, aa);
+        cic_w(vjz, ybf, zbf, ab);
+        cic_w(vrho, ycf, zcf, ac);
+        cic_w(vjx, ycf, zcf, ac + 4);
+        cic_w(vchi, ycf, zcf, ac + 8);
+        bool fa = valid, fb = valid, fc = valid;
+        quad_fold<4>(ya1 * n2 + za1, fa, aa);
+        quad_fold<4>(yb1 * n2 + zb1, fb, ab);
+        quad_fold<12>(yc1 * n2 + zc1, fc, ac);
+        if (fa) dep4w(lds, win, jyn, n1, n2, ya1, ya2, za1, za2, aa[0], aa[1], aa[2], aa[3]);
+        if (fb) dep4w(lds + kWinCap, win, jzn, n1, n2, yb1, yb2, zb1, zb2, ab[0], ab[1], ab[2], ab[3]);
+        if (fc) {
+            dep4w(lds + 2 * kWinCap, win, rho, n1, n2, yc1, yc2, zc1, zc2, ac[0], ac[1], ac[2], ac[3]);
+            dep4w(lds + 3 * kWinCap, win, jx, n1, n2, yc1, yc2, zc1, zc2, ac[4], ac[5], ac[6], ac[7]);
+            dep4w(lds + 4 * kWinCap, win, chi, n1, n2, yc1, yc2, zc1, zc2, ac[8], ac[9], ac[10], ac[11]);
+        }
     }
     __syncthreads();
     double *gl[5] = {jyn, jzn, rho, jx, chi};
@@ -335,10 +355,18 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
     const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
     win_zero(lds, 3, win);
     __syncthreads();
-    if (valid) {
-        dep4(lds, win, rho, n1, n2, y1, y2, z1, z2, yf, zf, vrho);
-        dep4(lds + kWinCap, win, jx, n1, n2, y1, y2, z1, z2, yf, zf, vjx);
-        dep4(lds + 2 * kWinCap, win, chi, n1, n2, y1, y2, z1, z2, yf, zf, vchi);
+    {
+        double ac[12];
+        cic_w(vrho, yf, zf, ac);
+        cic_w(vjx, yf, zf, ac + 4);
+        cic_w(vchi, yf, zf, ac + 8);
+        bool act = valid;
+        quad_fold<12>(y1 * n2 + z1, act, ac);
+        if (act) {
+            dep4w(lds, win, rho, n1, n2, y1, y2, z1, z2, ac[0], ac[1], ac[2], ac[3]);
+            dep4w(lds + kWinCap, win, jx, n1, n2, y1, y2, z1, z2, ac[4], ac[5], ac[6], ac[7]);
+            dep4w(lds + 2 * kWinCap, win, chi, n1, n2, y1, y2, z1, z2, ac[8], ac[9], ac[10], ac[11]);
+        }
     }
     __syncthreads();
     double *gl[3] = {rho, jx, chi};
