@@ -27,19 +27,20 @@ def log(*a):
 
 def bytes_per_particle(cls, K):
     """Algorithmic HBM bytes per particle and launch of a particle phase
-    (24 B persistent state S, 8 B scratch T = p*):
-      A: read S + T, write position + momenta (20 B)          = 52
-      B: read S (+ T after the first iteration), write T      = 32 | 40
-      C: read S + T                                           = 32
-      S: read S                                               = 24"""
+    (persistent state S = 36 B: u32 cell, 2 x u32 offsets, 2 x f64 momenta,
+    f32 weight; scratch T = 8 B: 2 x f32 increments p* - p):
+      A: read S + T, write position + momenta (28 B)          = 72
+      B: read S (+ T after the first iteration), write T      = 44 | 52
+      C: read S + T                                           = 44
+      S: read S                                               = 36"""
     if cls == "push_A":
-        return 52.0
+        return 72.0
     if cls == "push_B":
-        return (32.0 + 40.0 * (K - 1)) / K if K > 0 else 32.0
+        return (44.0 + 52.0 * (K - 1)) / K if K > 0 else 44.0
     if cls == "push_C":
-        return 32.0
+        return 44.0
     if cls == "push_S":
-        return 24.0
+        return 36.0
     raise KeyError(cls)
 
 

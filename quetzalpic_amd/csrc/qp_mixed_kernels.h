@@ -1,11 +1,13 @@
 // qp_mixed_kernels.h -- device code of the fused mixed-precision slice step.
 //
-// Particle layout (structure of arrays, 24 B persistent + 8 B scratch):
+// Particle layout (structure of arrays, 36 B persistent + 8 B scratch):
 //   cell  uint32  (j << 16) | k          node-origin cell of the particle
 //   oy,oz uint32  in-cell offset in units of 2^-32 cell (Y = j + oy 2^-32)
-//   py,pz float   transverse momenta
+//   py,pz double  transverse momenta (they accumulate over every slice)
 //   w     float   macro-particle weight (wake_particle_2d::n)
-//   pys,pzs float predicted momenta p* of the current iteration (scratch)
+//   dpy,dpz float increment p* - p of the current iteration's predicted
+//                 momenta (scratch): its rounding is relative to the small
+//                 increment, not to p
 // Arithmetic is FP64 throughout (FP64 is cheap next to the memory traffic);
 // only the stored state is narrowed.  Positions are advanced in FP64 cell
 // units and re-split with round-to-nearest into cell + offset, so the
@@ -35,7 +37,8 @@ struct MxGeom {
 
 struct MxPart {
     uint32_t *cell, *oy, *oz;
-    float *py, *pz, *w, *pys, *pzs;
+    double *py, *pz;
+    float *w, *dpy, *dpz;
     int np;
 };
 

@@ -77,13 +77,15 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
         q = load_pos(P, p);
         w = P.w[p];
         if (advance) {
-            const float pyv = P.pys[p], pzv = P.pzs[p];
+            // commit p <- p* (advance 2: K = 0, where p* keeps its initial value 0)
+            const double pyv = advance == 2 ? 0.0 : P.py[p] + (double)P.dpy[p];
+            const double pzv = advance == 2 ? 0.0 : P.pz[p] + (double)P.dpz[p];
             P.py[p] = pyv;
             P.pz[p] = pzv;
             const double ps = mx_gather(psim, q, n1, n2);
             const double s = g.dx / (1 + ps);
-            double Y = (q.j + q.fy) + (s * (double)pyv) * g.inv_d1;
-            double Z = (q.k + q.fz) + (s * (double)pzv) * g.inv_d2;
+            double Y = (q.j + q.fy) + (s * pyv) * g.inv_d1;
+            double Z = (q.k + q.fz) + (s * pzv) * g.inv_d2;
             Y = wrap_cells(Y, n1);
             Z = wrap_cells(Z, n2);
             int j, k;
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         if (first_iter) {
             gm = gamma_of(py * py + pz * pz, a, ps, inv1ps);  // S4 (:373-374)
         } else {
-            const double pys = P.pys[p], pzs = P.pzs[p];  // P6 of the previous iteration (:503-505)
+            const double pys = py + (double)P.dpy[p], pzs = pz + (double)P.dpz[p];  // P6 of the previous iteration (:503-505)
             gm = gamma_of(0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs), a, ps, inv1ps);
         }
         const double day = mx_yder(asq, q, n1, n2, g.inv_d1);
@@ -185,12 +187,13 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         double pzn = pz - s * 0.5 * daz;
         pzn += s * gm * dpz;
         pzn += dx * byp;
-        const float pynf = (float)pyn, pznf = (float)pzn;
-        P.pys[p] = pynf;
-        P.pzs[p] = pznf;
-        // P2 (:439-442) with the stored p*
-        double Ym = (q.j + q.fy) + (0.5 * s * (double)pynf) * g.inv_d1;
-        double Zm = (q.k + q.fz) + (0.5 * s * (double)pznf) * g.inv_d2;
+        const float dyf = (float)(pyn - py), dzf = (float)(pzn - pz);
+        P.dpy[p] = dyf;
+        P.dpz[p] = dzf;
+        // P2 (:439-442) with the stored p* = p + increment
+        const double pyst = py + (double)dyf, pzst = pz + (double)dzf;
+        double Ym = (q.j + q.fy) + (0.5 * s * pyst) * g.inv_d1;
+        double Zm = (q.k + q.fz) + (0.5 * s * pzst) * g.inv_d2;
         qm = pos_from_cells(wrap_cells(Ym, n1), wrap_cells(Zm, n2), n1, n2);
     }
     if (threadIdx.x == 0) {
@@ -240,8 +243,7 @@ __global__ __launch_bounds__(kPT) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, 
         q = load_pos(P, p);
         const double w = P.w[p];
         const double py = P.py[p], pz = P.pz[p];
-        const float pysf = P.pys[p], pzsf = P.pzs[p];
-        const double pys = pysf, pzs = pzsf;
+        const double pys = py + (double)P.dpy[p], pzs = pz + (double)P.dpz[p];
         const double ps = mx_gather(psi, q, n1, n2);
         const double inv1ps = 1.0 / (1 + ps);
         const double s = g.dx * inv1ps;
@@ -600,8 +602,8 @@ __global__ void k_sort_scatter(MxPart P, MxPart Q, const unsigned *offset, unsig
     Q.py[d] = P.py[p];
     Q.pz[d] = P.pz[p];
     Q.w[d] = P.w[p];
-    Q.pys[d] = P.pys[p];
-    Q.pzs[d] = P.pzs[p];
+    Q.dpy[d] = P.dpy[p];
+    Q.dpz[d] = P.dpz[p];
     perm2[d] = perm[p];
 }
 
@@ -614,8 +616,8 @@ __global__ void k_sort_copyback(MxPart P, MxPart Q, int *perm, const int *perm2)
     P.py[p] = Q.py[p];
     P.pz[p] = Q.pz[p];
     P.w[p] = Q.w[p];
-    P.pys[p] = Q.pys[p];
-    P.pzs[p] = Q.pzs[p];
+    P.dpy[p] = Q.dpy[p];
+    P.dpz[p] = Q.dpz[p];
     perm[p] = perm2[p];
 }
 
@@ -637,7 +639,7 @@ struct MixedState {
 static MixedState *mstate(Wake *w) { return static_cast<MixedState *>(w->mixed); }
 
 static void free_part(MxPart &P) {
-    void *arrs[] = {P.cell, P.oy, P.oz, P.py, P.pz, P.w, P.pys, P.pzs};
+    void *arrs[] = {P.cell, P.oy, P.oz, P.py, P.pz, P.w, P.dpy, P.dpz};
     for (void *a : arrs)
         if (a) (void)hipFree(a);
     P = MxPart{};
@@ -648,11 +650,11 @@ static int alloc_part(MxPart &P, size_t n) {
     QP_HIP_CHECK(hipMalloc(&P.cell, 4 * n));
     QP_HIP_CHECK(hipMalloc(&P.oy, 4 * n));
     QP_HIP_CHECK(hipMalloc(&P.oz, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.py, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.pz, 4 * n));
+    QP_HIP_CHECK(hipMalloc(&P.py, 8 * n));
+    QP_HIP_CHECK(hipMalloc(&P.pz, 8 * n));
     QP_HIP_CHECK(hipMalloc(&P.w, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.pys, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.pzs, 4 * n));
+    QP_HIP_CHECK(hipMalloc(&P.dpy, 4 * n));
+    QP_HIP_CHECK(hipMalloc(&P.dpz, 4 * n));
     return QP_OK;
 }
 
@@ -752,21 +754,22 @@ static int build_initial_state(Wake *w, MixedState *m) {
     std::iota(m->h_perm.begin(), m->h_perm.end(), 0);
     std::stable_sort(m->h_perm.begin(), m->h_perm.end(), [&](int a, int b) { return key[a] < key[b]; });
     std::vector<uint32_t> c2(np), y2(np), z2(np);
-    std::vector<float> py(np), pz(np), wt(np);
+    std::vector<double> py(np), pz(np);
+    std::vector<float> wt(np);
     for (size_t s = 0; s < np; ++s) {
         const int i = m->h_perm[s];
         c2[s] = cell[i];
         y2[s] = oy[i];
         z2[s] = oz[i];
-        py[s] = (float)w->h_py[i];
-        pz[s] = (float)w->h_pz[i];
+        py[s] = w->h_py[i];
+        pz[s] = w->h_pz[i];
         wt[s] = (float)w->h_w[i];
     }
     QP_HIP_CHECK(hipMemcpy(m->I.cell, c2.data(), 4 * np, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->I.oy, y2.data(), 4 * np, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->I.oz, z2.data(), 4 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.py, py.data(), 4 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.pz, pz.data(), 4 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.py, py.data(), 8 * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.pz, pz.data(), 8 * np, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 4 * np, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->perm0, m->h_perm.data(), sizeof(int) * np, hipMemcpyHostToDevice));
     m->init_valid = true;
@@ -785,11 +788,13 @@ int wake_upload_mixed(Wake *w) {
     QP_HIP_CHECK(hipMemcpyAsync(m->P.cell, m->I.cell, 4 * np, hipMemcpyDeviceToDevice, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->P.oy, m->I.oy, 4 * np, hipMemcpyDeviceToDevice, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->P.oz, m->I.oz, 4 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.py, m->I.py, 4 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.pz, m->I.pz, 4 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.py, m->I.py, 8 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.pz, m->I.pz, 8 * np, hipMemcpyDeviceToDevice, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->P.w, m->I.w, 4 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemsetAsync(m->P.pys, 0, 4 * np, s));  // py_next = pz_next = 0 (src/System_3d.cpp:740-741)
-    QP_HIP_CHECK(hipMemsetAsync(m->P.pzs, 0, 4 * np, s));
+    // py_next = pz_next = 0 (src/System_3d.cpp:740-741): the first slice reads no
+    // increment, and with K = 0 the commit writes p = 0 directly
+    QP_HIP_CHECK(hipMemsetAsync(m->P.dpy, 0, 4 * np, s));
+    QP_HIP_CHECK(hipMemsetAsync(m->P.dpz, 0, 4 * np, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->perm, m->perm0, sizeof(int) * np, hipMemcpyDeviceToDevice, s));
     return QP_OK;
 }
@@ -803,13 +808,13 @@ int wake_download_mixed(Wake *w, double *y, double *z, double *py, double *pz) {
     const size_t np = w->np;
     QP_HIP_CHECK(hipStreamSynchronize(w->stream()));
     std::vector<uint32_t> c(np), oy(np), oz(np);
-    std::vector<float> fy(np), fz(np);
+    std::vector<double> fy(np), fz(np);
     std::vector<int> perm(np);
     QP_HIP_CHECK(hipMemcpy(c.data(), m->P.cell, 4 * np, hipMemcpyDeviceToHost));
     QP_HIP_CHECK(hipMemcpy(oy.data(), m->P.oy, 4 * np, hipMemcpyDeviceToHost));
     QP_HIP_CHECK(hipMemcpy(oz.data(), m->P.oz, 4 * np, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(fy.data(), m->P.py, 4 * np, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(fz.data(), m->P.pz, 4 * np, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(fy.data(), m->P.py, 8 * np, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(fz.data(), m->P.pz, 8 * np, hipMemcpyDeviceToHost));
     QP_HIP_CHECK(hipMemcpy(perm.data(), m->perm, sizeof(int) * np, hipMemcpyDeviceToHost));
     for (size_t s = 0; s < np; ++s) {
         const int i = perm[s];
@@ -926,7 +931,7 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     const int laser = (w->laser_set && !w->a_sqr3) ? 1 : 0;
     prof_begin(w, "push_A");
     MLAUNCH(k_mx_A, R.pb, kPT, 0, m->P, R.g, si, (const double *)w->psim, w->src, w->asq2, w->laser, laser, w->rho,
-            w->jx, w->chi, s_phase ? 1 : 0, first ? 0 : 1, w->ctl);
+            w->jx, w->chi, s_phase ? 1 : 0, first ? 0 : (p.K == 0 ? 2 : 1), w->ctl);
     prof_end(w, "push_A");
     QP_TRY(psi_solve(R, s, slot, psi));
     if (s_phase) {
