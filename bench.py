@@ -54,9 +54,10 @@ def solve_bytes(n1, n2):
 
 
 def read_traffic(kernel_cls):
-    """Per-launch HBM KiB of a kernel from the committed rocprofv3 --pmc summary
+    """Per-launch HBM bytes of a kernel from the committed rocprofv3 --pmc summary
     (profiles/traffic_<class>_C2.txt: lines 'FETCH_KIB <v>', 'WRITE_KIB <v>',
-    'COPY_FACTOR <f>'); None when absent."""
+    'FETCH_FACTOR <f>', 'WRITE_FACTOR <f>', the factors from a device copy of
+    known size); None when absent."""
     p = os.path.join(ROOT, "profiles", f"traffic_{kernel_cls}_C2.txt")
     if not os.path.exists(p):
         return None
@@ -71,8 +72,8 @@ def read_traffic(kernel_cls):
                     pass
     if "FETCH_KIB" not in vals or "WRITE_KIB" not in vals:
         return None
-    fac = vals.get("COPY_FACTOR", 1.0)
-    return (vals["FETCH_KIB"] + vals["WRITE_KIB"]) * 1024.0 * fac
+    return 1024.0 * (vals["FETCH_KIB"] * vals.get("FETCH_FACTOR", 1.0) +
+                     vals["WRITE_KIB"] * vals.get("WRITE_FACTOR", 1.0))
 
 
 def calib_copy(mib):

@@ -380,7 +380,8 @@ void prof_begin(Wake *w, const char *cls) {
     const size_t need = 2 * (size_t)(w->prof_count + 1);
     while (w->prof_ev.size() < need) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
+        // no system-scope fence on record: these events only time device work
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
         w->prof_ev.push_back(e);
     }
     (void)hipEventRecord(w->prof_ev[2 * w->prof_count], w->stream());
