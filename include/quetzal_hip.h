@@ -134,6 +134,28 @@ int qp_wake_describe(qp_wake w, char *buf, size_t len);
 /* Copy the current particle state back (FP64 SoA; any pointer may be NULL). */
 int qp_wake_get_particles(qp_wake w, size_t np, double *y, double *z, double *py, double *pz);
 
+/* ---- Batches of independent wakes (no reference counterpart) --------------
+ * The reference runs one System_3d per process; its callers that sweep many
+ * plasma / laser settings run many wakes.  A batch holds nwakes members with
+ * the geometry and iteration settings of *params (QP_PREC_MIXED only).  The
+ * members are advanced in lockstep, and every launch carries all of them.  Each
+ * member has its own particles, Gaussian laser, fields and stop rule, and
+ * gives the same result as the member run alone in its own handle.  The
+ * single-wake calls above act on member 0; qp_wake_run requires nwakes == 1. */
+int qp_wake_create_batch(qp_ctx ctx, const qp_wake_params *params, int nwakes, qp_wake *out);
+int qp_wake_batch_size(qp_wake w, int *nwakes);
+int qp_wake_set_particles_at(qp_wake w, int member, size_t np, const double *y, const double *z,
+                             const double *py, const double *pz, const double *weight);
+int qp_wake_set_laser_gaussian_at(qp_wake w, int member, double a0, double xsigma, double ysigma,
+                                  double zsigma, double x0, double y0, double z0);
+/* Per-slice callback of a batch: member index, then the view (as qp_slice_cb). */
+typedef int (*qp_batch_cb)(int member, const qp_slice_view *view, void *user);
+/* One pass for every member; stop_slices (nwakes entries, may be NULL) gets each
+ * member's stop slice or -1.  Returns QP_STOPPED if any member stopped. */
+int qp_wake_run_batch(qp_wake w, qp_batch_cb cb, void *user, int *stop_slices);
+int qp_wake_get_particles_at(qp_wake w, int member, size_t np, double *y, double *z, double *py,
+                             double *pz);
+
 #ifdef __cplusplus
 }
 #endif

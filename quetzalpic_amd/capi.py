@@ -41,6 +41,7 @@ class SliceView(ctypes.Structure):
 
 
 SLICE_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(SliceView), ctypes.c_void_p)
+BATCH_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.POINTER(SliceView), ctypes.c_void_p)
 
 _lib = None
 
@@ -77,6 +78,12 @@ def load():
         "qp_wake_profile_kernel": ([vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
         "qp_wake_get_particles": ([vp, ctypes.c_size_t, _dp, _dp, _dp, _dp], ctypes.c_int),
         "qp_wake_describe": ([vp, ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
+        "qp_wake_create_batch": ([vp, ctypes.POINTER(WakeParams), ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
+        "qp_wake_batch_size": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "qp_wake_set_particles_at": ([vp, ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "qp_wake_set_laser_gaussian_at": ([vp, ctypes.c_int] + [ctypes.c_double] * 7, ctypes.c_int),
+        "qp_wake_run_batch": ([vp, BATCH_CB, vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "qp_wake_get_particles_at": ([vp, ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -255,6 +262,97 @@ class Wake:
         y, z, py, pz = (np.zeros(self.np) for _ in range(4))
         _check(load().qp_wake_get_particles(self.h, self.np, _ptr(y), _ptr(z), _ptr(py), _ptr(pz)),
                "qp_wake_get_particles")
+        return y, z, py, pz
+
+    def close(self):
+        if self.h:
+            load().qp_wake_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WakeBatch:
+    """nwakes independent wakes with one geometry, advanced together so every
+    launch carries all of them (qp_wake_create_batch; mixed precision).  Each
+    member has its own particles, Gaussian laser, fields and stop rule."""
+
+    def __init__(self, ctx, nwakes, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
+                 b_threshold=100.0, use_graph=1):
+        self.ctx = ctx
+        self.nwakes = nwakes
+        self.shape = (nx, ny, nz)
+        self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
+                                 QP_PREC_MIXED, use_graph, 0)
+        self.h = ctypes.c_void_p()
+        _check(load().qp_wake_create_batch(ctx.h, ctypes.byref(self.params), nwakes, ctypes.byref(self.h)),
+               "qp_wake_create_batch")
+        self.nps = [0] * nwakes
+        ctx._adopt(self)
+
+    def set_particles(self, member, y, z, w, py=None, pz=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (y, z, py, pz, w)]
+        y, z, py, pz, w = arrs
+        self.nps[member] = len(y)
+        _check(load().qp_wake_set_particles_at(self.h, member, len(y), _ptr(y), _ptr(z), _ptr(py), _ptr(pz),
+                                               _ptr(w)), "qp_wake_set_particles_at")
+
+    def set_laser_gaussian(self, member, a0, xsigma, ysigma, zsigma, x0, y0, z0):
+        _check(load().qp_wake_set_laser_gaussian_at(self.h, member, a0, xsigma, ysigma, zsigma, x0, y0, z0),
+               "qp_wake_set_laser_gaussian_at")
+
+    def run(self, fields=None, callback=None):
+        """One pass of every member.  Returns (list of per-member field dicts as
+        Wake.run, status, list of stop slices)."""
+        nx, ny, nz = self.shape
+        outs = [{k: np.zeros(self.shape) for k in fields} if fields else {} for _ in range(self.nwakes)]
+        stops = (ctypes.c_int * self.nwakes)()
+        if fields or callback:
+            nn = ny * nz
+
+            def _cb(member, vp, _user):
+                v = vp.contents
+                out = outs[member]
+                for k in out:
+                    if v.kind == QP_SLICE_STOPPED and k not in ("by", "bz", "chi"):
+                        continue
+                    out[k][v.slice] = np.ctypeslib.as_array(getattr(v, k), shape=(nn,)).reshape(ny, nz)
+                if callback:
+                    return int(callback(member, v) or 0)
+                return 0
+
+            cb = BATCH_CB(_cb)
+        else:
+            cb = BATCH_CB()
+        st = _check(load().qp_wake_run_batch(self.h, cb, None, stops), "qp_wake_run_batch")
+        return outs, st, list(stops)
+
+    def last_time_ms(self):
+        ms = ctypes.c_double()
+        _check(load().qp_wake_last_time_ms(self.h, ctypes.byref(ms)), "qp_wake_last_time_ms")
+        return ms.value
+
+    def profile_kernel(self, cls):
+        n = ctypes.c_int()
+        us = ctypes.c_double()
+        _check(load().qp_wake_profile_kernel(self.h, cls.encode(), ctypes.byref(n), ctypes.byref(us)),
+               "qp_wake_profile_kernel")
+        return n.value, us.value
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        _check(load().qp_wake_describe(self.h, buf, 512), "qp_wake_describe")
+        return buf.value.decode()
+
+    def get_particles(self, member):
+        n = self.nps[member]
+        y, z, py, pz = (np.zeros(n) for _ in range(4))
+        _check(load().qp_wake_get_particles_at(self.h, member, n, _ptr(y), _ptr(z), _ptr(py), _ptr(pz)),
+               "qp_wake_get_particles_at")
         return y, z, py, pz
 
     def close(self):

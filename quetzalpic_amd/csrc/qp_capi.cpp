@@ -1,4 +1,5 @@
 // qp_capi.cpp -- the extern "C" boundary (include/quetzal_hip.h).
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -161,7 +162,12 @@ int qp_solve2d(qp_fourier2d f, double d1, double d2, double D, const double *src
 
 // ----------------------------------------------------------------- wake
 int qp_wake_create(qp_ctx ctx, const qp_wake_params *params, qp_wake *out) {
+    return qp_wake_create_batch(ctx, params, 1, out);
+}
+
+int qp_wake_create_batch(qp_ctx ctx, const qp_wake_params *params, int nwakes, qp_wake *out) {
     ARG_CHECK(ctx && params && out, "qp_wake_create: null argument");
+    ARG_CHECK(nwakes >= 1, "qp_wake_create_batch: nwakes must be >= 1");
     const qp_wake_params &p = *params;
     ARG_CHECK(p.nx >= 1 && p.ny >= 2 && p.nz >= 2, "qp_wake_create: need nx >= 1, ny >= 2, nz >= 2");
     ARG_CHECK(p.ny <= kMaxN && p.nz <= kMaxN, "qp_wake_create: ny, nz must be <= 1024");
@@ -170,12 +176,16 @@ int qp_wake_create(qp_ctx ctx, const qp_wake_params *params, qp_wake *out) {
     // Magnetic field iterations should be non-negative (src/System_3d.cpp:77-79)
     ARG_CHECK(p.K >= 0, "Magnetic field iterations should be non-negative");
     ARG_CHECK(p.precision == QP_PREC_FP64 || p.precision == QP_PREC_MIXED, "qp_wake_create: unknown precision");
+    ARG_CHECK(nwakes == 1 || p.precision == QP_PREC_MIXED, "qp_wake_create_batch: batches need QP_PREC_MIXED");
+    ARG_CHECK(nwakes == 1 || p.resort_every == 0, "qp_wake_create_batch: re-binning is for single wakes only");
     QP_HIP_CHECK(hipSetDevice(ctx->device));
     auto *h = new (std::nothrow) qp_wake_s();
     if (!h) return QP_ERR_ALLOC;
     Wake *w = &h->w;
     w->ctx = ctx;
     w->p = p;
+    w->nw = nwakes;
+    w->mem.assign(nwakes, WakeMember{});
     int st = wake_alloc_common(w);
     if (st != QP_OK) {
         qp_wake_destroy(h);
@@ -197,6 +207,7 @@ int qp_wake_destroy(qp_wake h) {
     if (w->rho_b3) (void)hipFree(w->rho_b3);
     if (w->jx_b3) (void)hipFree(w->jx_b3);
     if (w->ctl) (void)hipFree(w->ctl);
+    if (w->d_laser) (void)hipFree(w->d_laser);
     if (w->h_out) (void)hipHostFree(w->h_out);
     if (w->ev0) (void)hipEventDestroy(w->ev0);
     if (w->ev1) (void)hipEventDestroy(w->ev1);
@@ -207,24 +218,32 @@ int qp_wake_destroy(qp_wake h) {
 
 int qp_wake_set_particles(qp_wake h, size_t np, const double *y, const double *z, const double *py,
                           const double *pz, const double *weight) {
+    return qp_wake_set_particles_at(h, 0, np, y, z, py, pz, weight);
+}
+
+int qp_wake_set_particles_at(qp_wake h, int member, size_t np, const double *y, const double *z, const double *py,
+                             const double *pz, const double *weight) {
     ARG_CHECK(h, "null wake");
     ARG_CHECK(np == 0 || (y && z && weight), "qp_wake_set_particles: y, z, weight required");
     Wake *w = &h->w;
+    ARG_CHECK(member >= 0 && member < w->nw, "qp_wake_set_particles_at: member out of range");
+    ARG_CHECK(np < (size_t)INT_MAX / 2, "qp_wake_set_particles: too many particles");
     for (size_t i = 0; i < np; ++i) {
         if (!(y[i] >= 0 && y[i] < w->p.ly && z[i] >= 0 && z[i] < w->p.lz)) {
             set_error("qp_wake_set_particles: particle " + std::to_string(i) + " outside [0, ly) x [0, lz)");
             return QP_ERR_ARG;
         }
     }
-    w->np = np;
-    w->h_y.assign(y, y + np);
-    w->h_z.assign(z, z + np);
-    w->h_w.assign(weight, weight + np);
-    w->h_py.assign(np, 0.0);
-    w->h_pz.assign(np, 0.0);
-    if (py) w->h_py.assign(py, py + np);
-    if (pz) w->h_pz.assign(pz, pz + np);
-    w->particles_set = true;
+    WakeMember &m = w->mem[member];
+    m.np = np;
+    m.h_y.assign(y, y + np);
+    m.h_z.assign(z, z + np);
+    m.h_w.assign(weight, weight + np);
+    m.h_py.assign(np, 0.0);
+    m.h_pz.assign(np, 0.0);
+    if (py) m.h_py.assign(py, py + np);
+    if (pz) m.h_pz.assign(pz, pz + np);
+    m.particles_set = true;
     w->particles_dirty = true;
     return QP_OK;
 }
@@ -244,19 +263,26 @@ static int upload3d(Wake *w, double **dst, const double *src) {
 int qp_wake_set_a_sqr(qp_wake h, const double *a_sqr) {
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;
+    ARG_CHECK(w->nw == 1, "qp_wake_set_a_sqr: 3D arrays are for single wakes (use the Gaussian laser)");
     QP_TRY(upload3d(w, &w->a_sqr3, a_sqr));
-    w->laser_set = false;
+    w->mem[0].laser_set = false;
     return QP_OK;
 }
 
 int qp_wake_set_laser_gaussian(qp_wake h, double a0, double xsigma, double ysigma, double zsigma, double x0,
                                double y0, double z0) {
+    return qp_wake_set_laser_gaussian_at(h, 0, a0, xsigma, ysigma, zsigma, x0, y0, z0);
+}
+
+int qp_wake_set_laser_gaussian_at(qp_wake h, int member, double a0, double xsigma, double ysigma, double zsigma,
+                                  double x0, double y0, double z0) {
     ARG_CHECK(h, "null wake");
     ARG_CHECK(xsigma > 0 && ysigma > 0 && zsigma > 0, "laser widths must be positive");
     Wake *w = &h->w;
+    ARG_CHECK(member >= 0 && member < w->nw, "qp_wake_set_laser_gaussian_at: member out of range");
     const double s8 = std::sqrt(8.0);
-    w->laser = LaserGauss{0.5 * a0 * a0, xsigma / s8, ysigma / s8, zsigma / s8, x0, y0, z0};
-    w->laser_set = true;
+    w->mem[member].laser = LaserGauss{0.5 * a0 * a0, xsigma / s8, ysigma / s8, zsigma / s8, x0, y0, z0};
+    w->mem[member].laser_set = true;
     if (w->a_sqr3) (void)hipFree(w->a_sqr3);
     w->a_sqr3 = nullptr;
     return QP_OK;
@@ -265,18 +291,55 @@ int qp_wake_set_laser_gaussian(qp_wake h, double a0, double xsigma, double ysigm
 int qp_wake_set_bunch_sources(qp_wake h, const double *rho_b, const double *jx_b) {
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;
+    ARG_CHECK(w->nw == 1 || (!rho_b && !jx_b), "qp_wake_set_bunch_sources: bunch sources are for single wakes");
     QP_TRY(upload3d(w, &w->rho_b3, rho_b));
     QP_TRY(upload3d(w, &w->jx_b3, jx_b));
     return QP_OK;
 }
 
+namespace {
+struct SingleCb {
+    qp_slice_cb cb;
+    void *user;
+};
+int single_cb_adapter(int, const qp_slice_view *v, void *user) {
+    const SingleCb *a = static_cast<const SingleCb *>(user);
+    return a->cb(v, a->user);
+}
+bool all_particles_set(const Wake *w) {
+    for (const WakeMember &m : w->mem)
+        if (!m.particles_set) return false;
+    return true;
+}
+}  // namespace
+
 int qp_wake_run(qp_wake h, qp_slice_cb cb, void *user, int *stop_slice) {
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;
-    ARG_CHECK(w->particles_set, "qp_wake_run: particles not set");
+    ARG_CHECK(w->nw == 1, "qp_wake_run: batch handle (use qp_wake_run_batch)");
+    ARG_CHECK(all_particles_set(w), "qp_wake_run: particles not set");
     QP_HIP_CHECK(hipSetDevice(w->ctx->device));
     if (w->p.precision == QP_PREC_FP64) return wake_run_fp64(w, cb, user, stop_slice);
-    return wake_run_mixed(w, cb, user, stop_slice);
+    SingleCb a{cb, user};
+    return wake_run_mixed(w, cb ? single_cb_adapter : nullptr, &a, stop_slice);
+}
+
+int qp_wake_run_batch(qp_wake h, qp_batch_cb cb, void *user, int *stop_slices) {
+    ARG_CHECK(h, "null wake");
+    Wake *w = &h->w;
+    ARG_CHECK(all_particles_set(w), "qp_wake_run_batch: particles not set for every member");
+    QP_HIP_CHECK(hipSetDevice(w->ctx->device));
+    if (w->p.precision == QP_PREC_FP64) {
+        ARG_CHECK(!cb, "qp_wake_run_batch: FP64 handles take qp_wake_run");
+        return wake_run_fp64(w, nullptr, nullptr, stop_slices);
+    }
+    return wake_run_mixed(w, cb, user, stop_slices);
+}
+
+int qp_wake_batch_size(qp_wake h, int *nwakes) {
+    ARG_CHECK(h && nwakes, "null argument");
+    *nwakes = h->w.nw;
+    return QP_OK;
 }
 
 int qp_wake_last_time_ms(qp_wake h, double *ms) {
@@ -288,7 +351,7 @@ int qp_wake_last_time_ms(qp_wake h, double *ms) {
 int qp_wake_profile_kernel(qp_wake h, const char *kernel_class, int *launches, double *mean_us) {
     ARG_CHECK(h && kernel_class && launches && mean_us, "null argument");
     Wake *w = &h->w;
-    ARG_CHECK(w->particles_set, "qp_wake_profile_kernel: particles not set");
+    ARG_CHECK(all_particles_set(w), "qp_wake_profile_kernel: particles not set");
     w->prof_class = kernel_class;
     w->prof_count = 0;
     int st;
@@ -314,7 +377,8 @@ int qp_wake_profile_kernel(qp_wake h, const char *kernel_class, int *launches, d
 int qp_wake_describe(qp_wake h, char *buf, size_t len) {
     ARG_CHECK(h && buf && len > 0, "null argument");
     const Wake *w = &h->w;
-    std::string s = "{\"rows_per_block\": " + std::to_string(w->S.rows_per_block) +
+    std::string s = "{\"members\": " + std::to_string(w->nw) +
+                    ", \"rows_per_block\": " + std::to_string(w->S.rows_per_block) +
                     ", \"cols_per_block\": " + std::to_string(w->S.cols_per_block) +
                     ", \"precision\": " + std::to_string(w->p.precision) +
                     ", \"fft\": \"" + (w->S.wave ? "wave" : "generic") + "\"}";
@@ -324,11 +388,16 @@ int qp_wake_describe(qp_wake h, char *buf, size_t len) {
 }
 
 int qp_wake_get_particles(qp_wake h, size_t np, double *y, double *z, double *py, double *pz) {
+    return qp_wake_get_particles_at(h, 0, np, y, z, py, pz);
+}
+
+int qp_wake_get_particles_at(qp_wake h, int member, size_t np, double *y, double *z, double *py, double *pz) {
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;
-    ARG_CHECK(np == w->np, "qp_wake_get_particles: count mismatch");
+    ARG_CHECK(member >= 0 && member < w->nw, "qp_wake_get_particles_at: member out of range");
+    ARG_CHECK(np == w->mem[member].np, "qp_wake_get_particles: count mismatch");
     if (w->p.precision == QP_PREC_FP64) return wake_download_fp64(w, y, z, py, pz);
-    return wake_download_mixed(w, y, z, py, pz);
+    return wake_download_mixed(w, member, y, z, py, pz);
 }
 
 }  // extern "C"
@@ -339,17 +408,20 @@ namespace qp {
 int wake_alloc_common(Wake *w) {
     const qp_wake_params &p = w->p;
     const size_t nn = w->nn();
-    QP_TRY(w->S.init(p.ny, p.nz, p.dy, p.dz));
+    const size_t nw = (size_t)w->nw;
+    QP_TRY(w->S.init(p.ny, p.nz, p.dy, p.dz, w->nw));
     const int ngrids = 26;
-    QP_HIP_CHECK(hipMalloc(&w->g_base, sizeof(double) * nn * ngrids));
-    QP_HIP_CHECK(hipMemset(w->g_base, 0, sizeof(double) * nn * ngrids));
+    QP_HIP_CHECK(hipMalloc(&w->g_base, sizeof(double) * nn * nw * ngrids));
+    QP_HIP_CHECK(hipMemset(w->g_base, 0, sizeof(double) * nn * nw * ngrids));
     double **grids[] = {&w->src, &w->psi, &w->psi_prev, &w->psim, &w->rho, &w->jx, &w->jy, &w->jz, &w->jyn,
                         &w->jzn, &w->djy, &w->djz, &w->rho_ion, &w->by, &w->bz, &w->chi, &w->ex, &w->ey,
                         &w->ez, &w->src2, &w->asq2, &w->psi_b[0], &w->psi_b[1], &w->zero2};
     int gi = 0;
-    for (double **g : grids) *g = w->g_base + nn * (gi++);
-    QP_HIP_CHECK(hipMalloc(&w->ctl, sizeof(Ctl)));
-    QP_HIP_CHECK(hipMemset(w->ctl, 0, sizeof(Ctl)));
+    for (double **g : grids) *g = w->g_base + nn * nw * (gi++);
+    QP_HIP_CHECK(hipMalloc(&w->ctl, sizeof(Ctl) * nw));
+    QP_HIP_CHECK(hipMemset(w->ctl, 0, sizeof(Ctl) * nw));
+    QP_HIP_CHECK(hipMalloc(&w->d_laser, sizeof(LaserGauss) * nw));
+    QP_HIP_CHECK(hipMemset(w->d_laser, 0, sizeof(LaserGauss) * nw));
     QP_HIP_CHECK(hipHostMalloc(&w->h_out, sizeof(double) * nn * 11, hipHostMallocDefault));
     QP_HIP_CHECK(hipEventCreate(&w->ev0));
     QP_HIP_CHECK(hipEventCreate(&w->ev1));

@@ -71,7 +71,8 @@ int Solver2d::set_steps(double d1_, double d2_) {
     return QP_OK;
 }
 
-int Solver2d::init(int n1_, int n2_, double d1_, double d2_) {
+int Solver2d::init(int n1_, int n2_, double d1_, double d2_, int nbatch_) {
+    nbatch = nbatch_ < 1 ? 1 : nbatch_;
     n1 = n1_;
     n2 = n2_;
     nh = n2 / 2 + 1;
@@ -80,7 +81,7 @@ int Solver2d::init(int n1_, int n2_, double d1_, double d2_) {
     QP_HIP_CHECK(hipMalloc(&d_lam1, sizeof(double) * n1));
     QP_HIP_CHECK(hipMalloc(&d_lam2, sizeof(double) * nh));
     QP_TRY(set_steps(d1_, d2_));
-    for (int i = 0; i < 2; ++i) QP_HIP_CHECK(hipMalloc(&spec[i], sizeof(double2) * (size_t)n1 * nh));
+    for (int i = 0; i < 2; ++i) QP_HIP_CHECK(hipMalloc(&spec[i], sizeof(double2) * (size_t)n1 * nh * nbatch));
     // QP_FFT=generic keeps the generic kernels for every size (A/B comparisons).
     const char *sel = getenv("QP_FFT");
     wave = (wf_supported(n1) && wf_supported(n2) && !(sel && strcmp(sel, "generic") == 0)) ? 1 : 0;
@@ -187,7 +188,7 @@ static int gen_rows_fwd(const Solver2d &S, const double *src, double2 *spec, hip
                         int skip_mode) {
     SrcPlain f{src, S.n2};
     hipLaunchKernelGGL(k_rows_fwd<SrcPlain>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, f,
-                       spec, spec, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, skip_mode);
+                       spec, spec, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, skip_mode, 1, (size_t)0);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
@@ -211,7 +212,18 @@ static int cols_launch(const Solver2d &S, double D, int nspec, hipStream_t s, co
     const int nb = (S.nh + S.cols_per_block - 1) / S.cols_per_block;
     hipLaunchKernelGGL(k_cols_solve, dim3(nb, nspec), dim3(kSolverThreads), cols_lds(S), s, S.spec[0], S.spec[1],
                        S.n1, S.nh, S.ax1.dev, (const double *)S.d_lam1, (const double *)S.d_lam2, D,
-                       S.cols_per_block, ctl, skip_mode, mode);
+                       S.cols_per_block, ctl, skip_mode, mode, nspec, (size_t)0);
+    QP_HIP_CHECK(hipGetLastError());
+    return QP_OK;
+}
+
+int solve_cols_batch(const Solver2d &S, double D, int nspec, int nw, hipStream_t s, const Ctl *ctl,
+                     int skip_mode) {
+    if (S.wave) return wlaunch_cols(S, D, nspec, nullptr, s, ctl, skip_mode, nw);
+    const int nb = (S.nh + S.cols_per_block - 1) / S.cols_per_block;
+    hipLaunchKernelGGL(k_cols_solve, dim3(nb, nspec * nw), dim3(kSolverThreads), cols_lds(S), s, S.spec[0],
+                       S.spec[1], S.n1, S.nh, S.ax1.dev, (const double *)S.d_lam1, (const double *)S.d_lam2, D,
+                       S.cols_per_block, ctl, skip_mode, 2, nspec, (size_t)S.n1 * S.nh);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
@@ -228,7 +240,8 @@ int solve_cols_inverse_only(const Solver2d &S, hipStream_t s) { return cols_laun
 int fourier_rows_inv(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s) {
     SinkRaw f{dst, S.n2};
     hipLaunchKernelGGL(k_rows_inv<SinkRaw>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
-                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)nullptr, 0);
+                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)nullptr, 0, 1,
+                       (size_t)0);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
@@ -243,7 +256,7 @@ int solve_rows_inv_plain_ctl(const Solver2d &S, const double2 *spec, double *dst
     SinkScaled f{dst, S.n1, S.n2};
     if (S.wave) return wlaunch_rows_inv(S, spec, spec, f, 1, s, ctl, ctl ? 1 : 0);
     hipLaunchKernelGGL(k_rows_inv<SinkScaled>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
-                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, ctl ? 1 : 0);
+                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, ctl, ctl ? 1 : 0, 1, (size_t)0);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
@@ -257,7 +270,7 @@ int solve_rows_inv_bcheck(const Solver2d &S, const double2 *spec, double *dst, d
     SinkBCheck f{dst, S.n1, S.n2, thr, ctl, slice};
     if (S.wave) return wlaunch_rows_inv(S, spec, spec, f, 1, s, (const Ctl *)ctl, 2);
     hipLaunchKernelGGL(k_rows_inv<SinkBCheck>, dim3(rows_blocks(S)), dim3(kSolverThreads), rows_lds(S), s, spec,
-                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)ctl, 2);
+                       spec, f, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)ctl, 2, 1, (size_t)0);
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }

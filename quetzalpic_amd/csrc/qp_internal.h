@@ -54,14 +54,15 @@ struct Solver2d {
     FftAxis ax1, ax2;
     double *d_lam1 = nullptr;  // (2cos(2 pi j/n1) - 2)/d1/d1
     double *d_lam2 = nullptr;  // (2cos(2 pi k/n2) - 2)/d2/d2
-    double2 *spec[2] = {nullptr, nullptr};  // n1 x nh half spectra (two for paired solves)
+    double2 *spec[2] = {nullptr, nullptr};  // n1 x nh half spectra (two for paired solves), nbatch each
+    int nbatch = 1;                           // wakes sharing this solver (spectra side by side)
     int rows_per_block = 1;   // complex row transforms (= 2 real rows) per block
     int cols_per_block = 1;
     // 1: both sides 2^p in [16, 1024] -> register-resident FFT kernels with the
     // column-major spectrum spec[l * n1 + j]; 0: generic kernels, row-major
     // spec[j * nh + l] (FFTW's r2c layout).  Fourier2d always uses the latter.
     int wave = 0;
-    int init(int n1, int n2, double d1, double d2);
+    int init(int n1, int n2, double d1, double d2, int nbatch = 1);
     int set_steps(double d1, double d2);
     int autotune(hipStream_t s, double *scratch_real);
     void destroy();
@@ -85,8 +86,10 @@ struct Ctl {
 
 __device__ __forceinline__ bool ctl_skip(const Ctl *c) { return c->stop_acc != 0; }
 __device__ __forceinline__ bool ctl_skip_b(const Ctl *c) { return c->stop_now != 0; }
+// c is the control block of the launching block's wake; several blocks of one
+// wake may latch (same value).
 __device__ __forceinline__ void ctl_latch(Ctl *c) {
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) c->stop_now = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->stop_now = 1;
 }
 
 // ------------------------------------------------------------ complex math
@@ -200,6 +203,9 @@ __device__ __forceinline__ void wrap_coords(double &y, double &z, double ly, dou
 // All launches go to `s`.  `src_fn` kinds select what the row-forward pass reads.
 int solve_rows_fwd_plain(const Solver2d &S, const double *src, double2 *spec, hipStream_t s);
 int solve_cols(const Solver2d &S, double D, int nspec, hipStream_t s, const Ctl *ctl, int skip_mode);
+// Column pass of nw wakes side by side (spectra at + wake * n1 * nh, control blocks at ctl + wake).
+int solve_cols_batch(const Solver2d &S, double D, int nspec, int nw, hipStream_t s, const Ctl *ctl,
+                     int skip_mode);
 int solve_rows_inv_plain(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s);
 // Plain-path B solve epilogue: store v/n1/n2 and flag |B| > thr or NaN.
 int solve_rows_inv_bcheck(const Solver2d &S, const double2 *spec, double *dst, double thr, Ctl *ctl,

@@ -35,12 +35,18 @@ struct MxGeom {
     double inv_d1, inv_d2;  // 1/dy, 1/dz: the fused path multiplies instead of dividing
 };
 
+// Member m of a batch owns slots [m * pstride, m * pstride + npw[m]) of every
+// array (npw null: one member with np particles).
 struct MxPart {
     uint32_t *cell, *oy, *oz;
     double *py, *pz;
     float *w, *dpy, *dpz;
     int np;
+    int pstride;
+    const int *npw;
 };
+
+__device__ __forceinline__ int mx_np(const MxPart &P, int wk) { return P.npw ? P.npw[wk] : P.np; }
 
 // Slice-dependent inputs resolved on the device from the slice counter.
 struct MxSliceIn {
@@ -51,8 +57,10 @@ struct MxSliceIn {
 };
 
 __device__ __forceinline__ int mx_slice(const Ctl *c, const MxSliceIn &si) { return c->slice_of[si.slot]; }
-__device__ __forceinline__ const double *mx_asq(const Ctl *c, const MxSliceIn &si, size_t nn) {
-    return si.a_sqr3 ? si.a_sqr3 + (size_t)mx_slice(c, si) * nn : si.asq2;
+// <a^2> plane of the block's member (go = member * nn; the 3D array and the
+// bunch sources exist for single wakes only).
+__device__ __forceinline__ const double *mx_asq(const Ctl *c, const MxSliceIn &si, size_t nn, size_t go) {
+    return si.a_sqr3 ? si.a_sqr3 + (size_t)mx_slice(c, si) * nn : si.asq2 + go;
 }
 
 // ----------------------------------------------------------- positions

@@ -2,6 +2,11 @@
 // Sources (what the row-forward pass reads) and sinks (what the row-inverse
 // pass does with each real output) are small functors so the fused slice
 // step can assemble sources and finalise fields inside the transform launches.
+//
+// Batches: blockIdx.y = wake * nspec + which.  `which` selects spec0 / spec1
+// (two independent solves of one wake, e.g. B_y and B_z), `wake` offsets the
+// spectra by sstride and the control block by one Ctl; sources and sinks get
+// (wake, which) and offset their own grids.
 #pragma once
 
 #include "qp_fft_lds.h"
@@ -11,6 +16,7 @@ namespace qp {
 constexpr int kSolverThreads = 256;
 
 // skip_mode: 0 never, 1 skip when stop_acc (and latch stop_now), 2 skip when stop_now.
+// ctl: the control block of this block's wake (or null).
 __device__ __forceinline__ bool solver_skip(const Ctl *ctl, int skip_mode) {
     if (!ctl || skip_mode == 0) return false;
     if (skip_mode == 1) {
@@ -26,7 +32,7 @@ __device__ __forceinline__ bool solver_skip(const Ctl *ctl, int skip_mode) {
 struct SrcPlain {
     const double *a;
     int n2;
-    __device__ double operator()(int j, int k) const { return a[(size_t)j * n2 + k]; }
+    __device__ double operator()(int, int, int j, int k) const { return a[(size_t)j * n2 + k]; }
 };
 
 // Real output v of the unnormalised c2r; the reference divides by n1 then n2
@@ -34,16 +40,16 @@ struct SrcPlain {
 struct SinkScaled {
     double *dst;
     int n1, n2;
-    __device__ void begin() {}
-    __device__ void operator()(int j, int k, double v) const { dst[(size_t)j * n2 + k] = v / n1 / n2; }
+    __device__ void begin(int) {}
+    __device__ void operator()(int, int, int j, int k, double v) const { dst[(size_t)j * n2 + k] = v / n1 / n2; }
 };
 
 // Raw unnormalised c2r output (Fourier2d::backward_transform semantics).
 struct SinkRaw {
     double *dst;
     int n2;
-    __device__ void begin() {}
-    __device__ void operator()(int j, int k, double v) const { dst[(size_t)j * n2 + k] = v; }
+    __device__ void begin(int) {}
+    __device__ void operator()(int, int, int j, int k, double v) const { dst[(size_t)j * n2 + k] = v; }
 };
 
 // B store with the stop rule (src/System_3d.cpp:541-552, 569-580).
@@ -53,13 +59,13 @@ struct SinkBCheck {
     double thr;
     Ctl *ctl;
     int slice;
-    __device__ void begin() {}
-    __device__ void operator()(int j, int k, double v) const {
+    __device__ void begin(int) {}
+    __device__ void operator()(int wk, int, int j, int k, double v) const {
         const double b = v / n1 / n2;
         dst[(size_t)j * n2 + k] = b;
         if (fabs(b) > thr || isnan(b)) {
-            ctl->stop_acc = 1;
-            ctl->stop_slice = slice;
+            ctl[wk].stop_acc = 1;
+            ctl[wk].stop_slice = slice;
         }
     }
 };
@@ -69,9 +75,10 @@ struct SinkBCheck {
 template <class Src>
 __global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *spec0, double2 *spec1, int n1,
                                                              int n2, int nh, FftAxisDev ax, int B, const Ctl *ctl,
-                                                             int skip_mode) {
-    if (solver_skip(ctl, skip_mode)) return;
-    double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+                                                             int skip_mode, int nspec, size_t sstride) {
+    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
+    double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     extern __shared__ double2 lds[];
     double2 *a = lds, *b = lds + (size_t)B * n2, *twl = lds + (size_t)2 * B * n2;
     const int pair0 = blockIdx.x * B;
@@ -80,8 +87,8 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *s
         const int bb = t / n2, k = t - bb * n2;
         const int ra = 2 * (pair0 + bb), rb = ra + 1;
         double xa = 0.0, xb = 0.0;
-        if (ra < n1) xa = src(ra, k);
-        if (rb < n1) xb = src(rb, k);
+        if (ra < n1) xa = src(wk, which, ra, k);
+        if (rb < n1) xb = src(wk, which, rb, k);
         a[t] = make_double2(xa, xb);
     }
     __syncthreads();
@@ -108,10 +115,12 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *s
 [[maybe_unused]] static __global__ __launch_bounds__(kSolverThreads) void k_cols_solve(double2 *spec0, double2 *spec1, int n1, int nh,
                                                                FftAxisDev ax, const double *lam1,
                                                                const double *lam2, double D, int C,
-                                                               const Ctl *ctl, int skip_mode, int mode) {
-    if (solver_skip(ctl, skip_mode)) return;
+                                                               const Ctl *ctl, int skip_mode, int mode, int nspec,
+                                                               size_t sstride) {
+    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
     extern __shared__ double2 lds[];
-    double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+    double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     double2 *a = lds, *b = lds + (size_t)C * n1, *twl = lds + (size_t)2 * C * n1;
     double *lam1l = reinterpret_cast<double *>(twl + n1);
     const int l0 = blockIdx.x * C;
@@ -165,10 +174,12 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_fwd(Src src, double2 *s
 template <class Sink>
 __global__ __launch_bounds__(kSolverThreads) void k_rows_inv(const double2 *spec0, const double2 *spec1, Sink sink,
                                                              int n1, int n2, int nh, FftAxisDev ax, int B,
-                                                             const Ctl *ctl, int skip_mode) {
-    if (solver_skip(ctl, skip_mode)) return;
-    const double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
-    sink.begin();
+                                                             const Ctl *ctl, int skip_mode, int nspec,
+                                                             size_t sstride) {
+    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
+    const double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
+    sink.begin(wk);
     extern __shared__ double2 lds[];
     double2 *a = lds, *b = lds + (size_t)B * n2, *twl = lds + (size_t)2 * B * n2;
     const int pair0 = blockIdx.x * B;
@@ -199,8 +210,8 @@ __global__ __launch_bounds__(kSolverThreads) void k_rows_inv(const double2 *spec
     for (int t = threadIdx.x; t < B * n2; t += blockDim.x) {
         const int bb = t / n2, k = t - bb * n2;
         const int ra = 2 * (pair0 + bb), rb = ra + 1;
-        if (ra < n1) sink(ra, k, r[t].x);
-        if (rb < n1) sink(rb, k, r[t].y);
+        if (ra < n1) sink(wk, which, ra, k, r[t].x);
+        if (rb < n1) sink(wk, which, rb, k, r[t].y);
     }
 }
 

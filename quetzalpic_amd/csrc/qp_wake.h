@@ -20,34 +20,41 @@ struct PartFP64 {
            *px = nullptr, *ym = nullptr, *zm = nullptr, *pyn = nullptr, *pzn = nullptr;
 };
 
+// Host data of one member of a batch of wakes (a single wake is a batch of
+// one): the initial particles (FP64, re-uploaded every pass) and the laser.
+struct WakeMember {
+    size_t np = 0;
+    bool particles_set = false;
+    std::vector<double> h_y, h_z, h_py, h_pz, h_w;
+    bool laser_set = false;
+    LaserGauss laser{};
+};
+
 struct Wake {
     qp_ctx_s *ctx = nullptr;
     qp_wake_params p{};
     Solver2d S;
-    size_t np = 0;
+    int nw = 1;                    // batch members (mixed path only when > 1)
+    std::vector<WakeMember> mem;   // nw entries
     size_t cap = 0;  // allocated particle capacity
-    bool particles_set = false;
     bool particles_dirty = true;  // host arrays changed since the device initial state was built
-    // host copy of the initial particles (FP64), re-uploaded every pass
-    std::vector<double> h_y, h_z, h_py, h_pz, h_w;
 
     PartFP64 pf;
     double *init64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // y, z, py, pz, n of a pass
 
-    // grids, ny*nz FP64 each
+    // grids: nw planes of ny*nz FP64 side by side each (member m at + m*ny*nz)
     double *g_base = nullptr;
     double *src = nullptr, *psi = nullptr, *psi_prev = nullptr, *psim = nullptr, *rho = nullptr,
            *jx = nullptr, *jy = nullptr, *jz = nullptr, *jyn = nullptr, *jzn = nullptr, *djy = nullptr,
            *djz = nullptr, *rho_ion = nullptr, *by = nullptr, *bz = nullptr, *chi = nullptr, *ex = nullptr,
            *ey = nullptr, *ez = nullptr, *src2 = nullptr, *asq2 = nullptr, *psi_b[2] = {nullptr, nullptr},
            *zero2 = nullptr;
-    // driver inputs
+    // driver inputs (the 3D arrays only for a single wake)
     double *a_sqr3 = nullptr;            // nx*ny*nz or null
-    bool laser_set = false;
-    LaserGauss laser{};
+    LaserGauss *d_laser = nullptr;       // nw members' lasers on the device
     double *rho_b3 = nullptr, *jx_b3 = nullptr;  // nx*ny*nz or null
 
-    Ctl *ctl = nullptr;
+    Ctl *ctl = nullptr;                  // nw control blocks
     double *h_out = nullptr;             // pinned host staging, 11 planes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -61,16 +68,24 @@ struct Wake {
     int prof_count = 0;
 
     size_t nn() const { return (size_t)p.ny * p.nz; }
+    size_t np() const { return mem[0].np; }  // particles of a single wake
+    bool any_laser() const {
+        for (const WakeMember &m : mem)
+            if (m.laser_set) return true;
+        return false;
+    }
     hipStream_t stream() const { return ctx->stream; }
 };
 
 int wake_alloc_common(Wake *w);
 int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice);
-int wake_run_mixed(Wake *w, qp_slice_cb cb, void *user, int *stop_slice);
+// Batch slice callback: member index, then the slice view.
+using batch_cb = int (*)(int member, const qp_slice_view *v, void *user);
+int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices);
 int wake_upload_fp64(Wake *w);
 int wake_upload_mixed(Wake *w);
 int wake_download_fp64(Wake *w, double *y, double *z, double *py, double *pz);
-int wake_download_mixed(Wake *w, double *y, double *z, double *py, double *pz);
+int wake_download_mixed(Wake *w, int member, double *y, double *z, double *py, double *pz);
 void wake_free_particles(Wake *w);
 void wake_destroy_mixed(Wake *w);
 

@@ -301,7 +301,7 @@ __global__ void k_advance(PartFP64 P, const double *psim, size_t np, G2 g, Ctl *
     } while (0)
 
 int wake_upload_fp64(Wake *w) {
-    const size_t np = w->np;
+    const size_t np = w->np();
     hipStream_t s = w->stream();
     PartFP64 &P = w->pf;
     if (w->cap < np || !P.y) {
@@ -316,7 +316,8 @@ int wake_upload_fp64(Wake *w) {
     }
     const size_t b = sizeof(double) * np;
     if (w->particles_dirty) {
-        const std::vector<double> *src[5] = {&w->h_y, &w->h_z, &w->h_py, &w->h_pz, &w->h_w};
+        const WakeMember &m0 = w->mem[0];
+        const std::vector<double> *src[5] = {&m0.h_y, &m0.h_z, &m0.h_py, &m0.h_pz, &m0.h_w};
         for (int i = 0; i < 5; ++i)
             QP_HIP_CHECK(hipMemcpy(w->init64[i], src[i]->data(), b, hipMemcpyHostToDevice));
         w->particles_dirty = false;
@@ -334,7 +335,7 @@ int wake_upload_fp64(Wake *w) {
 }
 
 int wake_download_fp64(Wake *w, double *y, double *z, double *py, double *pz) {
-    const size_t b = sizeof(double) * w->np;
+    const size_t b = sizeof(double) * w->np();
     QP_HIP_CHECK(hipStreamSynchronize(w->stream()));
     if (y) QP_HIP_CHECK(hipMemcpy(y, w->pf.y, b, hipMemcpyDeviceToHost));
     if (z) QP_HIP_CHECK(hipMemcpy(z, w->pf.z, b, hipMemcpyDeviceToHost));
@@ -363,7 +364,7 @@ static int solve_plain(Wake *w, const double *src, double *dst, double D, bool b
 int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
     const qp_wake_params &p = w->p;
     hipStream_t s = w->stream();
-    const size_t nn = w->nn(), np = w->np;
+    const size_t nn = w->nn(), np = w->np();
     const int n1 = p.ny, n2 = p.nz;
     const G2 g{n1, n2, p.dy, p.dz, p.ly, p.lz, p.dx};
     const double thr = p.psi_threshold - 1;
@@ -495,9 +496,9 @@ int wake_asq_slice(Wake *w, int i, const double **asq) {
         *asq = w->a_sqr3 + (size_t)i * w->nn();
         return QP_OK;
     }
-    if (w->laser_set) {
+    if (w->mem[0].laser_set) {
         hipStream_t s = w->stream();
-        LAUNCH(k_laser_slice, nblk(w->nn()), w->asq2, w->laser, i, p.dx, p.dy, p.dz, p.ny, p.nz);
+        LAUNCH(k_laser_slice, nblk(w->nn()), w->asq2, w->mem[0].laser, i, p.dx, p.dy, p.dz, p.ny, p.nz);
         *asq = w->asq2;
         return QP_OK;
     }

@@ -30,7 +30,6 @@
 
 namespace qp {
 
-static inline int pblocks(int np) { return np > 0 ? (np + kPT - 1) / kPT : 1; }
 
 #define MX_SKIP(ctl)              \
     if (ctl_skip(ctl)) {          \
@@ -43,14 +42,25 @@ static inline int pblocks(int np) { return np > 0 ? (np + kPT - 1) / kPT : 1; }
 // position into the psi source accumulator (:349-353).  The grid part
 // evaluates the laser plane of slice i and, for S-phase slices, initialises
 // rho/jx/chi (:381-388).
+// Batches: blockIdx.y is the member; its grids sit at + member * nn, its
+// particles at + member * pstride, its control block at ctl + member.
 __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, const double *psim, double *src_acc,
-                                              double *asq2, LaserGauss L, int laser, double *rho, double *jx,
+                                              double *asq2, const LaserGauss *Ls, int laser, double *rho, double *jx,
                                               double *chi, int init_rho, int advance, Ctl *ctl) {
+    const int wk = blockIdx.y;
+    ctl += wk;
     MX_SKIP(ctl);
     __shared__ double lds[kWinCap];
     __shared__ int sh[4 * (kPT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
-    const size_t nn = (size_t)n1 * n2;
+    const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
+    psim += go;
+    src_acc += go;
+    asq2 += go;
+    rho += go;
+    jx += go;
+    chi += go;
+    const LaserGauss L = Ls[wk];
     const int slice = mx_slice(ctl, si);
     if (laser || init_rho) {
         const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * nn : nullptr;
@@ -69,8 +79,9 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
             }
         }
     }
-    const int p = blockIdx.x * kPT + threadIdx.x;
-    const bool valid = p < P.np;
+    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const bool valid = lp < mx_np(P, wk);
+    const int p = wk * P.pstride + lp;
     Pos q{0, 0, 0.0, 0.0};
     double w = 0.0;
     if (valid) {
@@ -137,11 +148,22 @@ __device__ __forceinline__ double gamma_of(double p2, double a, double ps, doubl
 __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, const double *psi, const double *by,
                                               const double *bz, double *src_acc, double *rho, double *jx,
                                               double *chi, double *jyn, double *jzn, int first_iter, Ctl *ctl) {
+    const int wk = blockIdx.y;
+    ctl += wk;
     MX_SKIP(ctl);
     __shared__ double lds[kWinCap];
     __shared__ int sh[4 * (kPT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
-    const size_t nn = (size_t)n1 * n2;
+    const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
+    psi += go;
+    by += go;
+    bz += go;
+    src_acc += go;
+    rho += go;
+    jx += go;
+    chi += go;
+    jyn += go;
+    jzn += go;
     const int slice = mx_slice(ctl, si);
     {  // rho = rho_bunch(i), jx = jx_bunch(i), chi = 0, jy* = jz* = 0 before the C phase (:473-478, :511-518)
         const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * nn : nullptr;
@@ -154,9 +176,10 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
             jzn[t] = 0.0;
         }
     }
-    const double *asq = mx_asq(ctl, si, nn);
-    const int p = blockIdx.x * kPT + threadIdx.x;
-    const bool valid = p < P.np;
+    const double *asq = mx_asq(ctl, si, nn, go);
+    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const bool valid = lp < mx_np(P, wk);
+    const int p = wk * P.pstride + lp;
     Pos qm{0, 0, 0.0, 0.0};
     double w = 0.0;
     if (valid) {
@@ -228,15 +251,25 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
 __global__ __launch_bounds__(kPT) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, const double *psi,
                                               const double *psim, double *rho, double *jx, double *chi,
                                               double *jyn, double *jzn, Ctl *ctl) {
+    const int wk = blockIdx.y;
+    ctl += wk;
     MX_SKIP(ctl);
     extern __shared__ double ldsd[];
     double *lds = ldsd;
     __shared__ int sh[4 * (kPT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
-    const size_t nn = (size_t)n1 * n2;
-    const double *asq = mx_asq(ctl, si, nn);
-    const int p = blockIdx.x * kPT + threadIdx.x;
-    const bool valid = p < P.np;
+    const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
+    psi += go;
+    psim += go;
+    rho += go;
+    jx += go;
+    chi += go;
+    jyn += go;
+    jzn += go;
+    const double *asq = mx_asq(ctl, si, nn, go);
+    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const bool valid = lp < mx_np(P, wk);
+    const int p = wk * P.pstride + lp;
     Pos q{0, 0, 0.0, 0.0}, qm{0, 0, 0.0, 0.0};
     double vjy = 0, vjz = 0, vjx = 0, vrho = 0, vchi = 0;
     if (valid) {
@@ -316,15 +349,22 @@ __global__ __launch_bounds__(kPT) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, 
 // gamma/px (S4, :368-378) and the rho/jx/chi deposits (S5, :390-397).
 __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, const double *psi, double *rho,
                                               double *jx, double *chi, Ctl *ctl) {
+    const int wk = blockIdx.y;
+    ctl += wk;
     MX_SKIP(ctl);
     extern __shared__ double ldsd[];
     double *lds = ldsd;
     __shared__ int sh[4 * (kPT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
-    const size_t nn = (size_t)n1 * n2;
-    const double *asq = mx_asq(ctl, si, nn);
-    const int p = blockIdx.x * kPT + threadIdx.x;
-    const bool valid = p < P.np;
+    const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
+    psi += go;
+    rho += go;
+    jx += go;
+    chi += go;
+    const double *asq = mx_asq(ctl, si, nn, go);
+    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const bool valid = lp < mx_np(P, wk);
+    const int p = wk * P.pstride + lp;
     Pos q{0, 0, 0.0, 0.0};
     double vjx = 0, vrho = 0, vchi = 0;
     if (valid) {
@@ -377,8 +417,11 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
 
 // rho_ion = S(y,z)(-n) (src/System_3d.cpp:318-323), direct atomics (once per pass).
 __global__ __launch_bounds__(kPT) void k_mx_ion(MxPart P, MxGeom g, double *rho_ion) {
-    const int p = blockIdx.x * kPT + threadIdx.x;
-    if (p >= P.np) return;
+    const int wk = blockIdx.y;
+    const int lp = blockIdx.x * kPT + threadIdx.x;
+    if (lp >= mx_np(P, wk)) return;
+    const int p = wk * P.pstride + lp;
+    rho_ion += (size_t)wk * g.n1 * g.n2;
     const Pos q = load_pos(P, p);
     int j1, j2, k1, k2;
     double yf, zf;
@@ -390,15 +433,30 @@ __global__ __launch_bounds__(kPT) void k_mx_ion(MxPart P, MxGeom g, double *rho_
 // ------------------------------------------------------------- output step
 // e_x, e_y, e_z of slice i (src/System_3d.cpp:633-661); also hands the slice
 // counter to the next step.
-__global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi, const double *psi_prev,
+// psi of the slice in psi_b[slot], of the slice before in psi_b[1 - slot]; for
+// the final output_step(0) the slot follows each member's own stop slice.
+__global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi_b0, const double *psi_b1,
                          const double *by, const double *bz, int n1, int n2, double dx, double dy, double dz,
                          int with_ex, Ctl *ctl, int slot, int final_step) {
+    const int wk = blockIdx.y;
+    ctl += wk;
     if (!final_step) {
         MX_SKIP(ctl);
-    } else if (ctl->stop_acc) {
-        by = nullptr;
-        bz = nullptr;
+    } else {
+        slot = (ctl->stop_acc ? ctl->stop_slice : 0) & 1;
+        if (ctl->stop_acc) {
+            by = nullptr;
+            bz = nullptr;
+        }
     }
+    const size_t go = (size_t)wk * n1 * n2;
+    const double *psi = (slot ? psi_b1 : psi_b0) + go;
+    const double *psi_prev = (slot ? psi_b0 : psi_b1) + go;
+    ex += go;
+    ey += go;
+    ez += go;
+    if (by) by += go;
+    if (bz) bz += go;
     if (!final_step && blockIdx.x == 0 && threadIdx.x == 0) ctl->slice_of[1 - slot] = ctl->slice_of[slot] - 1;
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)n1 * n2) return;
@@ -411,6 +469,7 @@ __global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi, 
 }
 
 // ------------------------------------------------------- solver functors
+// All take (member, which, j, k): member planes sit at + member * nn.
 // psi source: jx_bunch(i) - rho_bunch(i) - rho_ion + deposits; clears the accumulator.
 struct SrcPsi {
     double *acc;
@@ -418,32 +477,33 @@ struct SrcPsi {
     const Ctl *ctl;
     int slot, n2;
     size_t nn;
-    __device__ double operator()(int j, int k) const {
-        const size_t t = (size_t)j * n2 + k;
-        const int slice = ctl->slice_of[slot];
+    __device__ double operator()(int wk, int, int j, int k) const {
+        const size_t t = (size_t)j * n2 + k, tw = (size_t)wk * nn + t;
+        const int slice = ctl[wk].slice_of[slot];
         const double jb = jx_b3 ? jx_b3[(size_t)slice * nn + t] : 0.0;
         const double rb = rho_b3 ? rho_b3[(size_t)slice * nn + t] : 0.0;
-        const double v = acc[t];
-        acc[t] = 0.0;
-        return (jb - rb - rho_ion[t]) + v;
+        const double v = acc[tw];
+        acc[tw] = 0.0;
+        return (jb - rb - rho_ion[tw]) + v;
     }
 };
 
-// B_y (blockIdx.y = 0, :530-536) and B_z (blockIdx.y = 1, :555-564) sources.
+// B_y (which = 0, :530-536) and B_z (which = 1, :555-564) sources.
 struct SrcB {
     const double *by, *bz, *jx, *jy, *jz, *jyn, *jzn;
     int n1, n2;
     double D, dx, dy, dz;
-    __device__ double operator()(int j, int k) const {
-        const size_t t = (size_t)j * n2 + k;
-        if (blockIdx.y == 0) {
-            const double jn = k < n2 - 1 ? jx[t + 1] : jx[(size_t)j * n2];
+    __device__ double operator()(int wk, int which, int j, int k) const {
+        const size_t go = (size_t)wk * n1 * n2;
+        const size_t t = go + (size_t)j * n2 + k;
+        if (which == 0) {
+            const double jn = k < n2 - 1 ? jx[t + 1] : jx[go + (size_t)j * n2];
             const double djz = (jz[t] - jzn[t]) / dx;
             return -D * by[t] - (jn - jx[t]) / dz + djz;
         }
         const double djy = (jy[t] - jyn[t]) / dx;
         if (j < n1 - 1) return -D * bz[t] + (jx[t + n2] - jx[t]) / dy - djy;
-        return -D * bz[k] + (jx[k] - jx[t]) / dy - djy;
+        return -D * bz[go + k] + (jx[go + k] - jx[t]) / dy - djy;
     }
 };
 
@@ -451,43 +511,45 @@ struct SrcB {
 // :30-47) computed from the column pass's partial sums, then the clamp.
 struct SinkPsiFinal {
     double *dst;
-    const double *bpart;  // [ncb] (P1, P2) pairs then G00
+    const double *bpart;  // per member (stride bstride): [ncb] (P1, P2) pairs then G00
     int ncb, n1, n2;
     double thr;
     double b;
-    __device__ void begin() {
+    size_t bstride;
+    __device__ void begin(int wk) {
+        const double *bp = bpart + wk * bstride;
         double s1 = 0.0, s2 = 0.0;
         for (int i = threadIdx.x; i < ncb; i += blockDim.x) {
-            s1 += bpart[2 * i];
-            s2 += bpart[2 * i + 1];
+            s1 += bp[2 * i];
+            s2 += bp[2 * i + 1];
         }
         s1 = block_sum(s1);
         s2 = block_sum(s2);
-        const double g00 = bpart[2 * ncb];
+        const double g00 = bp[2 * ncb];
         const double sraw = s1 + (double)n2 * g00 - s2;
         b = sraw / n1 / n2 / (n1 + n2 - 1);
     }
-    __device__ void operator()(int j, int k, double v) const {
+    __device__ void operator()(int wk, int, int j, int k, double v) const {
         double r = v / n1 / n2 - b;
         if (r < thr) r = thr;
-        dst[(size_t)j * n2 + k] = r;
+        dst[(size_t)wk * n1 * n2 + (size_t)j * n2 + k] = r;
     }
 };
 
-// B store + stop rule, paired: blockIdx.y selects B_y / B_z.
+// B store + stop rule, paired: which selects B_y / B_z.
 struct SinkBPair {
     double *by, *bz;
     int n1, n2;
     double thr;
     Ctl *ctl;
     int slot;
-    __device__ void begin() {}
-    __device__ void operator()(int j, int k, double v) const {
+    __device__ void begin(int) {}
+    __device__ void operator()(int wk, int which, int j, int k, double v) const {
         const double b = v / n1 / n2;
-        (blockIdx.y == 0 ? by : bz)[(size_t)j * n2 + k] = b;
+        (which == 0 ? by : bz)[(size_t)wk * n1 * n2 + (size_t)j * n2 + k] = b;
         if (fabs(b) > thr || isnan(b)) {
-            ctl->stop_acc = 1;
-            ctl->stop_slice = ctl->slice_of[slot];
+            ctl[wk].stop_acc = 1;
+            ctl[wk].stop_slice = ctl[wk].slice_of[slot];
         }
     }
 };
@@ -495,10 +557,16 @@ struct SinkBPair {
 // Column pass of the psi solves, with the partial sums the border mean needs:
 // after the inverse column transform G[j][l], the c2r row value at k = 0 is
 // sum_l c_l Re G[j][l] (c_0 = c_{n2/2} = 1, else 2) and sum_k x[0][k] = n2 Re G[0][0].
+// blockIdx.y is the member (spectra at + member * sstride, partial sums at +
+// member * bstride).
 __global__ __launch_bounds__(kSolverThreads) void k_cols_psi(double2 *spec, int n1, int nh, int n2, FftAxisDev ax,
                                                              const double *lam1, const double *lam2, int C,
-                                                             double *bpart, const Ctl *ctl) {
-    if (solver_skip(ctl, 1)) return;
+                                                             double *bpart, const Ctl *ctl, size_t sstride,
+                                                             size_t bstride) {
+    const int wk = blockIdx.y;
+    if (solver_skip(ctl + wk, 1)) return;
+    spec += wk * sstride;
+    bpart += wk * bstride;
     extern __shared__ double2 lds2[];
     double2 *a = lds2, *b = lds2 + (size_t)C * n1, *twl = lds2 + (size_t)2 * C * n1;
     double *lam1l = reinterpret_cast<double *>(twl + n1);
@@ -622,14 +690,19 @@ __global__ void k_sort_copyback(MxPart P, MxPart Q, int *perm, const int *perm2)
 }
 
 // ------------------------------------------------------------------ host
+// A batch of nw members (nw = 1 for a single wake).  Member m's particles take
+// slots [m * pstride, m * pstride + np_m) of every particle array; its grids,
+// spectra, partial sums and control block sit at member offsets as well.
 struct MixedState {
     MxPart P{}, Q{}, I{};   // live, re-binning scratch, initial state of a pass
-    int *perm = nullptr, *perm2 = nullptr, *perm0 = nullptr;
+    int *perm = nullptr, *perm2 = nullptr, *perm0 = nullptr;  // local particle index per slot (-1: padding)
+    int *npw = nullptr;     // particles per member (device)
+    int pstride = 0;
     bool init_valid = false;
     unsigned *count = nullptr, *cursor = nullptr;
     double *bpart = nullptr;
-    double *jbuf[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][y/z]
-    double *ex = nullptr;
+    size_t bstride = 0;     // partial sums per member
+    double *jbuf[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][y/z], nw planes each
     std::vector<int> h_perm;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int ncb = 0;
@@ -664,8 +737,8 @@ void wake_destroy_mixed(Wake *w) {
     free_part(m->P);
     free_part(m->Q);
     free_part(m->I);
-    void *arrs[] = {m->perm, m->perm2, m->perm0, m->count, m->cursor, m->bpart, m->jbuf[0][0], m->jbuf[0][1],
-                    m->jbuf[1][0], m->jbuf[1][1]};
+    void *arrs[] = {m->perm,       m->perm2,      m->perm0,      m->npw,        m->count,     m->cursor,
+                    m->bpart,      m->jbuf[0][0], m->jbuf[0][1], m->jbuf[1][0], m->jbuf[1][1]};
     for (void *a : arrs)
         if (a) (void)hipFree(a);
     for (auto &g : m->gexec)
@@ -674,9 +747,17 @@ void wake_destroy_mixed(Wake *w) {
     w->mixed = nullptr;
 }
 
+// Slots per member: the largest member rounded up to whole blocks.
+static int member_stride(const Wake *w) {
+    size_t mx = 0;
+    for (const WakeMember &mb : w->mem) mx = std::max(mx, mb.np);
+    const size_t st = ((mx + kPT - 1) / kPT) * kPT;
+    return (int)std::max<size_t>(st, kPT);
+}
+
 static int ensure_state(Wake *w) {
     MixedState *m = mstate(w);
-    const size_t nn = w->nn();
+    const size_t nn = w->nn(), nw = (size_t)w->nw;
     if (!m) {
         m = new MixedState();
         w->mixed = m;
@@ -687,33 +768,45 @@ static int ensure_state(Wake *w) {
             return QP_ERR_STATE;
         }
         m->ncb = ncb;
-        QP_HIP_CHECK(hipMalloc(&m->bpart, sizeof(double) * (2 * ncb + 2)));
+        m->bstride = 2 * (size_t)ncb + 2;
+        QP_HIP_CHECK(hipMalloc(&m->bpart, sizeof(double) * m->bstride * nw));
         for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) QP_HIP_CHECK(hipMalloc(&m->jbuf[a][b], sizeof(double) * nn));
+            for (int b = 0; b < 2; ++b) QP_HIP_CHECK(hipMalloc(&m->jbuf[a][b], sizeof(double) * nn * nw));
         const int ntk = (w->p.nz + kTile - 1) / kTile, ntj = (w->p.ny + kTile - 1) / kTile;
         QP_HIP_CHECK(hipMalloc(&m->count, sizeof(unsigned) * ntk * ntj));
         QP_HIP_CHECK(hipMalloc(&m->cursor, sizeof(unsigned) * ntk * ntj));
+        QP_HIP_CHECK(hipMalloc(&m->npw, sizeof(int) * nw));
     }
-    if (m->cap < w->np || !m->P.cell) {
+    const int pstride = member_stride(w);
+    const size_t need = nw * (size_t)pstride;
+    if (m->cap < need || !m->P.cell || m->pstride != pstride) {
         free_part(m->P);
         free_part(m->Q);
         free_part(m->I);
         if (m->perm) (void)hipFree(m->perm);
         if (m->perm2) (void)hipFree(m->perm2);
         if (m->perm0) (void)hipFree(m->perm0);
-        QP_TRY(alloc_part(m->P, w->np));
-        QP_TRY(alloc_part(m->Q, w->np));
-        QP_TRY(alloc_part(m->I, w->np));
-        QP_HIP_CHECK(hipMalloc(&m->perm, sizeof(int) * (w->np ? w->np : 1)));
-        QP_HIP_CHECK(hipMalloc(&m->perm2, sizeof(int) * (w->np ? w->np : 1)));
-        QP_HIP_CHECK(hipMalloc(&m->perm0, sizeof(int) * (w->np ? w->np : 1)));
-        m->cap = w->np;
+        QP_TRY(alloc_part(m->P, need));
+        QP_TRY(alloc_part(m->Q, need));
+        QP_TRY(alloc_part(m->I, need));
+        QP_HIP_CHECK(hipMalloc(&m->perm, sizeof(int) * need));
+        QP_HIP_CHECK(hipMalloc(&m->perm2, sizeof(int) * need));
+        QP_HIP_CHECK(hipMalloc(&m->perm0, sizeof(int) * need));
+        m->cap = need;
         m->init_valid = false;
         for (auto &g : m->gexec)
             if (g) (void)hipGraphExecDestroy(g);
         m->gexec[0] = m->gexec[1] = nullptr;
     }
-    m->P.np = m->Q.np = m->I.np = (int)w->np;
+    m->pstride = pstride;
+    std::vector<int> hn(nw);
+    for (size_t i = 0; i < nw; ++i) hn[i] = (int)w->mem[i].np;
+    QP_HIP_CHECK(hipMemcpy(m->npw, hn.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+    for (MxPart *q : {&m->P, &m->Q, &m->I}) {
+        q->np = (int)w->mem[0].np;
+        q->pstride = pstride;
+        q->npw = m->npw;
+    }
     return QP_OK;
 }
 
@@ -733,45 +826,51 @@ static inline void to_cells(double y, double d, int n, int &j, uint32_t &o) {
 }
 
 // Convert the host FP64 particles to the device layout once (sorted by
-// 16x16-cell tile, row-major inside a tile) into the pass's initial-state
-// buffers; every pass then starts with device-to-device copies, as the
-// reference re-initialises its particles each pass (src/System_3d.cpp:262).
+// 16x16-cell tile, then by cell) into the pass's initial-state buffers; every
+// pass then starts with device-to-device copies, as the reference
+// re-initialises its particles each pass (src/System_3d.cpp:262).
 static int build_initial_state(Wake *w, MixedState *m) {
-    const size_t np = w->np;
     const int n1 = w->p.ny, n2 = w->p.nz;
     const int ntk = (n2 + kTile - 1) / kTile;
-    std::vector<uint32_t> cell(np), oy(np), oz(np);
-    std::vector<long long> key(np);
-    for (size_t i = 0; i < np; ++i) {
-        int j, k;
-        to_cells(w->h_y[i], w->p.dy, n1, j, oy[i]);
-        to_cells(w->h_z[i], w->p.dz, n2, k, oz[i]);
-        cell[i] = ((uint32_t)j << 16) | (uint32_t)k;
-        const long long tile = (long long)(j / kTile) * ntk + (k / kTile);
-        key[i] = (tile * n1 + j) * (long long)n2 + k;
+    const size_t total = (size_t)w->nw * m->pstride;
+    std::vector<uint32_t> c2(total, 0), y2(total, 0), z2(total, 0);
+    std::vector<double> py(total, 0.0), pz(total, 0.0);
+    std::vector<float> wt(total, 0.0f);
+    m->h_perm.assign(total, -1);
+    for (int mi = 0; mi < w->nw; ++mi) {
+        const WakeMember &mb = w->mem[mi];
+        const size_t np = mb.np, base = (size_t)mi * m->pstride;
+        std::vector<uint32_t> cell(np), oy(np), oz(np);
+        std::vector<long long> key(np);
+        for (size_t i = 0; i < np; ++i) {
+            int j, k;
+            to_cells(mb.h_y[i], w->p.dy, n1, j, oy[i]);
+            to_cells(mb.h_z[i], w->p.dz, n2, k, oz[i]);
+            cell[i] = ((uint32_t)j << 16) | (uint32_t)k;
+            const long long tile = (long long)(j / kTile) * ntk + (k / kTile);
+            key[i] = (tile * n1 + j) * (long long)n2 + k;
+        }
+        std::vector<int> ord(np);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+        for (size_t s = 0; s < np; ++s) {
+            const int i = ord[s];
+            c2[base + s] = cell[i];
+            y2[base + s] = oy[i];
+            z2[base + s] = oz[i];
+            py[base + s] = mb.h_py[i];
+            pz[base + s] = mb.h_pz[i];
+            wt[base + s] = (float)mb.h_w[i];
+            m->h_perm[base + s] = i;
+        }
     }
-    m->h_perm.resize(np);
-    std::iota(m->h_perm.begin(), m->h_perm.end(), 0);
-    std::stable_sort(m->h_perm.begin(), m->h_perm.end(), [&](int a, int b) { return key[a] < key[b]; });
-    std::vector<uint32_t> c2(np), y2(np), z2(np);
-    std::vector<double> py(np), pz(np);
-    std::vector<float> wt(np);
-    for (size_t s = 0; s < np; ++s) {
-        const int i = m->h_perm[s];
-        c2[s] = cell[i];
-        y2[s] = oy[i];
-        z2[s] = oz[i];
-        py[s] = w->h_py[i];
-        pz[s] = w->h_pz[i];
-        wt[s] = (float)w->h_w[i];
-    }
-    QP_HIP_CHECK(hipMemcpy(m->I.cell, c2.data(), 4 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.oy, y2.data(), 4 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.oz, z2.data(), 4 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.py, py.data(), 8 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.pz, pz.data(), 8 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 4 * np, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->perm0, m->h_perm.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.cell, c2.data(), 4 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.oy, y2.data(), 4 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.oz, z2.data(), 4 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.py, py.data(), 8 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.pz, pz.data(), 8 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 4 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->perm0, m->h_perm.data(), sizeof(int) * total, hipMemcpyHostToDevice));
     m->init_valid = true;
     return QP_OK;
 }
@@ -783,40 +882,40 @@ int wake_upload_mixed(Wake *w) {
         QP_TRY(build_initial_state(w, m));
         w->particles_dirty = false;
     }
-    const size_t np = w->np;
+    const size_t n = (size_t)w->nw * m->pstride;
     hipStream_t s = w->stream();
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.cell, m->I.cell, 4 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.oy, m->I.oy, 4 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.oz, m->I.oz, 4 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.py, m->I.py, 8 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.pz, m->I.pz, 8 * np, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.w, m->I.w, 4 * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.cell, m->I.cell, 4 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.oy, m->I.oy, 4 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.oz, m->I.oz, 4 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.py, m->I.py, 8 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.pz, m->I.pz, 8 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.w, m->I.w, 4 * n, hipMemcpyDeviceToDevice, s));
     // py_next = pz_next = 0 (src/System_3d.cpp:740-741): the first slice reads no
     // increment, and with K = 0 the commit writes p = 0 directly
-    QP_HIP_CHECK(hipMemsetAsync(m->P.dpy, 0, 4 * np, s));
-    QP_HIP_CHECK(hipMemsetAsync(m->P.dpz, 0, 4 * np, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->perm, m->perm0, sizeof(int) * np, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemsetAsync(m->P.dpy, 0, 4 * n, s));
+    QP_HIP_CHECK(hipMemsetAsync(m->P.dpz, 0, 4 * n, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->perm, m->perm0, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
     return QP_OK;
 }
 
-int wake_download_mixed(Wake *w, double *y, double *z, double *py, double *pz) {
+int wake_download_mixed(Wake *w, int member, double *y, double *z, double *py, double *pz) {
     MixedState *m = mstate(w);
     if (!m) {
         set_error("mixed state not initialised");
         return QP_ERR_STATE;
     }
-    const size_t np = w->np;
+    const size_t np = w->mem[member].np, n = (size_t)m->pstride, base = (size_t)member * m->pstride;
     QP_HIP_CHECK(hipStreamSynchronize(w->stream()));
-    std::vector<uint32_t> c(np), oy(np), oz(np);
-    std::vector<double> fy(np), fz(np);
-    std::vector<int> perm(np);
-    QP_HIP_CHECK(hipMemcpy(c.data(), m->P.cell, 4 * np, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(oy.data(), m->P.oy, 4 * np, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(oz.data(), m->P.oz, 4 * np, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(fy.data(), m->P.py, 8 * np, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(fz.data(), m->P.pz, 8 * np, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(perm.data(), m->perm, sizeof(int) * np, hipMemcpyDeviceToHost));
-    for (size_t s = 0; s < np; ++s) {
+    std::vector<uint32_t> c(n), oy(n), oz(n);
+    std::vector<double> fy(n), fz(n);
+    std::vector<int> perm(n);
+    QP_HIP_CHECK(hipMemcpy(c.data(), m->P.cell + base, 4 * n, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(oy.data(), m->P.oy + base, 4 * n, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(oz.data(), m->P.oz + base, 4 * n, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(fy.data(), m->P.py + base, 8 * n, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(fz.data(), m->P.pz + base, 8 * n, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(perm.data(), m->perm + base, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < n; ++s) {
         const int i = perm[s];
         if (i < 0 || (size_t)i >= np) continue;
         if (y) y[i] = ((double)(c[s] >> 16) + oy[s] * kInvTwo32) * w->p.dy;
@@ -835,13 +934,14 @@ struct MxRun {
     MxGeom g;
     MxSliceIn si[2];
     size_t nn;
-    int np, pb, gb;
+    int nw, pb, gb;
+    size_t sstride;
     size_t ldsC, ldsS;
 };
 
 #define MLAUNCH(kern, grid, block, lds, ...)                                            \
     do {                                                                                \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, s, __VA_ARGS__);         \
+        hipLaunchKernelGGL(kern, grid, dim3(block), lds, s, __VA_ARGS__);               \
         QP_HIP_CHECK(hipGetLastError());                                                \
     } while (0)
 
@@ -849,32 +949,35 @@ int psi_solve(MxRun &R, hipStream_t s, int slot, double *dst) {
     Wake *w = R.w;
     const Solver2d &S = w->S;
     MixedState *m = R.m;
+    const int nw = R.nw;
     SrcPsi src{w->src, w->rho_ion, w->rho_b3, w->jx_b3, w->ctl, slot, S.n2, R.nn};
-    SinkPsiFinal snk{dst, m->bpart, m->ncb, S.n1, S.n2, w->p.psi_threshold - 1, 0.0};
+    SinkPsiFinal snk{dst, m->bpart, m->ncb, S.n1, S.n2, w->p.psi_threshold - 1, 0.0, m->bstride};
     if (S.wave) {
         prof_begin(w, "solve");
-        QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[0], 1, s, (const Ctl *)w->ctl, 1));
+        QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[0], 1, s, (const Ctl *)w->ctl, 1, nw));
         prof_end(w, "solve");
         prof_begin(w, "solve");
-        QP_TRY(wlaunch_cols(S, 0.0, 1, m->bpart, s, (const Ctl *)w->ctl, 1));
+        QP_TRY(wlaunch_cols(S, 0.0, 1, m->bpart, s, (const Ctl *)w->ctl, 1, nw, m->bstride));
         prof_end(w, "solve");
         prof_begin(w, "solve");
         QP_TRY(wlaunch_rows_inv(S, (const double2 *)S.spec[0], (const double2 *)S.spec[0], snk, 1, s,
-                                (const Ctl *)w->ctl, 1));
+                                (const Ctl *)w->ctl, 1, nw));
         prof_end(w, "solve");
         return QP_OK;
     }
     prof_begin(w, "solve");
-    MLAUNCH(k_rows_fwd<SrcPsi>, rows_blocks(S), kSolverThreads, rows_lds(S), src, S.spec[0], S.spec[0], S.n1, S.n2,
-            S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 1);
+    MLAUNCH(k_rows_fwd<SrcPsi>, dim3(rows_blocks(S), nw), kSolverThreads, rows_lds(S), src, S.spec[0], S.spec[0],
+            S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 1, 1, R.sstride);
     prof_end(w, "solve");
     prof_begin(w, "solve");
-    MLAUNCH(k_cols_psi, m->ncb, kSolverThreads, cols_lds(S), S.spec[0], S.n1, S.nh, S.n2, S.ax1.dev,
-            (const double *)S.d_lam1, (const double *)S.d_lam2, S.cols_per_block, m->bpart, (const Ctl *)w->ctl);
+    MLAUNCH(k_cols_psi, dim3(m->ncb, nw), kSolverThreads, cols_lds(S), S.spec[0], S.n1, S.nh, S.n2, S.ax1.dev,
+            (const double *)S.d_lam1, (const double *)S.d_lam2, S.cols_per_block, m->bpart, (const Ctl *)w->ctl,
+            R.sstride, m->bstride);
     prof_end(w, "solve");
     prof_begin(w, "solve");
-    MLAUNCH(k_rows_inv<SinkPsiFinal>, rows_blocks(S), kSolverThreads, rows_lds(S), (const double2 *)S.spec[0],
-            (const double2 *)S.spec[0], snk, S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 1);
+    MLAUNCH(k_rows_inv<SinkPsiFinal>, dim3(rows_blocks(S), nw), kSolverThreads, rows_lds(S),
+            (const double2 *)S.spec[0], (const double2 *)S.spec[0], snk, S.n1, S.n2, S.nh, S.ax2.dev,
+            S.rows_per_block, (const Ctl *)w->ctl, 1, 1, R.sstride);
     prof_end(w, "solve");
     return QP_OK;
 }
@@ -884,41 +987,40 @@ int b_solve(MxRun &R, hipStream_t s, int slot) {
     const Solver2d &S = w->S;
     MixedState *m = R.m;
     const qp_wake_params &p = w->p;
+    const int nw = R.nw;
     double *jy = m->jbuf[1 - slot][0], *jz = m->jbuf[1 - slot][1];
     double *jyn = m->jbuf[slot][0], *jzn = m->jbuf[slot][1];
     SrcB src{w->by, w->bz, w->jx, jy, jz, jyn, jzn, S.n1, S.n2, p.D, p.dx, p.dy, p.dz};
     SinkBPair snk{w->by, w->bz, S.n1, S.n2, p.b_threshold, w->ctl, slot};
     if (S.wave) {
         prof_begin(w, "solve");
-        QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[1], 2, s, (const Ctl *)w->ctl, 2));
+        QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[1], 2, s, (const Ctl *)w->ctl, 2, nw));
         prof_end(w, "solve");
         prof_begin(w, "solve");
-        QP_TRY(wlaunch_cols(S, p.D, 2, nullptr, s, (const Ctl *)w->ctl, 2));
+        QP_TRY(wlaunch_cols(S, p.D, 2, nullptr, s, (const Ctl *)w->ctl, 2, nw));
         prof_end(w, "solve");
         prof_begin(w, "solve");
         QP_TRY(wlaunch_rows_inv(S, (const double2 *)S.spec[0], (const double2 *)S.spec[1], snk, 2, s,
-                                (const Ctl *)w->ctl, 2));
+                                (const Ctl *)w->ctl, 2, nw));
         prof_end(w, "solve");
         return QP_OK;
     }
     prof_begin(w, "solve");
-    hipLaunchKernelGGL(k_rows_fwd<SrcB>, dim3(rows_blocks(S), 2), dim3(kSolverThreads), rows_lds(S), s, src,
-                       S.spec[0], S.spec[1], S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 2);
-    QP_HIP_CHECK(hipGetLastError());
+    MLAUNCH(k_rows_fwd<SrcB>, dim3(rows_blocks(S), 2 * nw), kSolverThreads, rows_lds(S), src, S.spec[0], S.spec[1],
+            S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 2, 2, R.sstride);
     prof_end(w, "solve");
     prof_begin(w, "solve");
-    QP_TRY(solve_cols(S, p.D, 2, s, w->ctl, 2));
+    QP_TRY(solve_cols_batch(S, p.D, 2, nw, s, w->ctl, 2));
     prof_end(w, "solve");
     prof_begin(w, "solve");
-    hipLaunchKernelGGL(k_rows_inv<SinkBPair>, dim3(rows_blocks(S), 2), dim3(kSolverThreads), rows_lds(S), s,
-                       (const double2 *)S.spec[0], (const double2 *)S.spec[1], snk, S.n1, S.n2, S.nh, S.ax2.dev,
-                       S.rows_per_block, (const Ctl *)w->ctl, 2);
-    QP_HIP_CHECK(hipGetLastError());
+    MLAUNCH(k_rows_inv<SinkBPair>, dim3(rows_blocks(S), 2 * nw), kSolverThreads, rows_lds(S),
+            (const double2 *)S.spec[0], (const double2 *)S.spec[1], snk, S.n1, S.n2, S.nh, S.ax2.dev,
+            S.rows_per_block, (const Ctl *)w->ctl, 2, 2, R.sstride);
     prof_end(w, "solve");
     return QP_OK;
 }
 
-// Enqueue the work of slice i (everything between two output steps).
+// Enqueue the work of slice i (everything between two output steps) for every member.
 int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     Wake *w = R.w;
     MixedState *m = R.m;
@@ -928,35 +1030,37 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     const bool s_phase = (i == 0) || (p.K == 0);
     const MxSliceIn &si = R.si[slot];
     double *psi = w->psi_b[slot];
-    const int laser = (w->laser_set && !w->a_sqr3) ? 1 : 0;
+    const int laser = (w->any_laser() && !w->a_sqr3) ? 1 : 0;
+    const dim3 pg(R.pb, R.nw);
     prof_begin(w, "push_A");
-    MLAUNCH(k_mx_A, R.pb, kPT, 0, m->P, R.g, si, (const double *)w->psim, w->src, w->asq2, w->laser, laser, w->rho,
-            w->jx, w->chi, s_phase ? 1 : 0, first ? 0 : (p.K == 0 ? 2 : 1), w->ctl);
+    MLAUNCH(k_mx_A, pg, kPT, 0, m->P, R.g, si, (const double *)w->psim, w->src, w->asq2,
+            (const LaserGauss *)w->d_laser, laser, w->rho, w->jx, w->chi, s_phase ? 1 : 0,
+            first ? 0 : (p.K == 0 ? 2 : 1), w->ctl);
     prof_end(w, "push_A");
     QP_TRY(psi_solve(R, s, slot, psi));
     if (s_phase) {
         prof_begin(w, "push_S");
-        MLAUNCH(k_mx_S, R.pb, kPT, R.ldsS, m->P, R.g, si, (const double *)psi, w->rho, w->jx, w->chi, w->ctl);
+        MLAUNCH(k_mx_S, pg, kPT, R.ldsS, m->P, R.g, si, (const double *)psi, w->rho, w->jx, w->chi, w->ctl);
         prof_end(w, "push_S");
         if (i == 0) return QP_OK;
     } else {
         for (int it = 0; it < p.K; ++it) {
             prof_begin(w, "push_B");
-            MLAUNCH(k_mx_B, R.pb, kPT, 0, m->P, R.g, si, (const double *)psi, (const double *)w->by,
+            MLAUNCH(k_mx_B, pg, kPT, 0, m->P, R.g, si, (const double *)psi, (const double *)w->by,
                     (const double *)w->bz, w->src, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1],
                     it == 0 ? 1 : 0, w->ctl);
             prof_end(w, "push_B");
             QP_TRY(psi_solve(R, s, slot, w->psim));
             prof_begin(w, "push_C");
-            MLAUNCH(k_mx_C, R.pb, kPT, R.ldsC, m->P, R.g, si, (const double *)psi, (const double *)w->psim, w->rho,
+            MLAUNCH(k_mx_C, pg, kPT, R.ldsC, m->P, R.g, si, (const double *)psi, (const double *)w->psim, w->rho,
                     w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
             prof_end(w, "push_C");
             QP_TRY(b_solve(R, s, slot));
         }
     }
-    MLAUNCH(k_mx_out, R.gb, 256, 0, w->ex, w->ey, w->ez, (const double *)psi, (const double *)w->psi_b[1 - slot],
-            (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz, first ? 0 : 1, w->ctl, slot,
-            0);
+    MLAUNCH(k_mx_out, dim3(R.gb, R.nw), 256, 0, w->ex, w->ey, w->ez, (const double *)w->psi_b[0],
+            (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz,
+            first ? 0 : 1, w->ctl, slot, 0);
     return QP_OK;
 }
 
@@ -965,52 +1069,84 @@ int resort(MxRun &R, hipStream_t s) {
     MixedState *m = R.m;
     const int ntk = (w->p.nz + kTile - 1) / kTile, ntj = (w->p.ny + kTile - 1) / kTile;
     const int ntiles = ntk * ntj;
+    const dim3 pg(R.pb, 1);
     QP_HIP_CHECK(hipMemsetAsync(m->count, 0, sizeof(unsigned) * ntiles, s));
-    MLAUNCH(k_sort_hist, R.pb, kPT, 0, m->P, m->count, ntk);
-    MLAUNCH(k_sort_scan, 1, 1024, 0, m->count, m->cursor, ntiles);
-    MLAUNCH(k_sort_scatter, R.pb, kPT, 0, m->P, m->Q, (const unsigned *)m->count, m->cursor, (const int *)m->perm,
+    MLAUNCH(k_sort_hist, pg, kPT, 0, m->P, m->count, ntk);
+    MLAUNCH(k_sort_scan, dim3(1), 1024, 0, m->count, m->cursor, ntiles);
+    MLAUNCH(k_sort_scatter, pg, kPT, 0, m->P, m->Q, (const unsigned *)m->count, m->cursor, (const int *)m->perm,
             m->perm2, ntk);
-    MLAUNCH(k_sort_copyback, R.pb, kPT, 0, m->P, m->Q, m->perm, (const int *)m->perm2);
+    MLAUNCH(k_sort_copyback, pg, kPT, 0, m->P, m->Q, m->perm, (const int *)m->perm2);
     return QP_OK;
+}
+
+// Stage the planes of one member (offset go) and call the batch callback.
+int deliver_member(Wake *w, batch_cb cb, void *user, int member, int slice, int kind, const double *psi,
+                   const double *rho, const double *jx, const double *jy, const double *jz, const double *ex,
+                   const double *ey, const double *ez, const double *by, const double *bz, const double *chi) {
+    const size_t go = (size_t)member * w->nn();
+    auto off = [&](const double *a) -> const double * { return a ? a + go : nullptr; };
+    struct Fwd {
+        batch_cb cb;
+        void *user;
+        int member;
+    } f{cb, user, member};
+    auto tramp = [](const qp_slice_view *v, void *u) -> int {
+        const Fwd *ff = static_cast<const Fwd *>(u);
+        return ff->cb(ff->member, v, ff->user);
+    };
+    return deliver_slice(w, tramp, &f, slice, kind, off(psi), off(rho), off(jx), off(jy), off(jz), off(ex),
+                         off(ey), off(ez), off(by), off(bz), off(chi));
 }
 
 }  // namespace
 
-int wake_run_mixed(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
+int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     const qp_wake_params &p = w->p;
     hipStream_t s = w->stream();
     QP_TRY(wake_upload_mixed(w));
     MixedState *m = mstate(w);
+    const int nw = w->nw;
     MxRun R;
     R.w = w;
     R.m = m;
     R.nn = w->nn();
-    R.np = (int)w->np;
-    R.pb = pblocks(R.np);
+    R.nw = nw;
+    R.pb = m->pstride / kPT;
     R.gb = (int)((R.nn + 255) / 256);
+    R.sstride = (size_t)w->S.n1 * w->S.nh;
     R.g = MxGeom{p.ny, p.nz, p.dy, p.dz, p.ly, p.lz, p.dx, 1.0 / p.dy, 1.0 / p.dz};
     for (int sl = 0; sl < 2; ++sl) R.si[sl] = MxSliceIn{w->a_sqr3, w->asq2, w->rho_b3, w->jx_b3, sl};
     R.ldsC = sizeof(double) * 5 * kWinCap;
     R.ldsS = sizeof(double) * 3 * kWinCap;
 
-    // zero the fields (src/System_3d.cpp:265-316) and the control block
+    // members' lasers (a member without one gets amplitude 0)
+    std::vector<LaserGauss> hl(nw);
+    for (int i = 0; i < nw; ++i)
+        hl[i] = w->mem[i].laser_set ? w->mem[i].laser : LaserGauss{0.0, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0};
+    QP_HIP_CHECK(hipMemcpyAsync(w->d_laser, hl.data(), sizeof(LaserGauss) * nw, hipMemcpyHostToDevice, s));
+    // zero the fields (src/System_3d.cpp:265-316) and the control blocks
+    const size_t plane = sizeof(double) * R.nn * nw;
     double *zero_list[] = {w->psi_b[0], w->psi_b[1], w->psim, w->by, w->bz, w->rho_ion, w->src, w->rho, w->jx,
                            w->chi, w->ex, w->ey, w->ez, w->asq2};
-    for (double *a : zero_list) QP_HIP_CHECK(hipMemsetAsync(a, 0, sizeof(double) * R.nn, s));
+    for (double *a : zero_list) QP_HIP_CHECK(hipMemsetAsync(a, 0, plane, s));
     for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) QP_HIP_CHECK(hipMemsetAsync(m->jbuf[a][b], 0, sizeof(double) * R.nn, s));
-    Ctl hc0{};
-    hc0.slice_of[(p.nx - 1) & 1] = p.nx - 1;
-    hc0.slice_of[p.nx & 1] = p.nx;
-    QP_HIP_CHECK(hipMemcpyAsync(w->ctl, &hc0, sizeof(Ctl), hipMemcpyHostToDevice, s));
+        for (int b = 0; b < 2; ++b) QP_HIP_CHECK(hipMemsetAsync(m->jbuf[a][b], 0, plane, s));
+    std::vector<Ctl> hc0(nw);
+    for (Ctl &c : hc0) {
+        c = Ctl{};
+        c.slice_of[(p.nx - 1) & 1] = p.nx - 1;
+        c.slice_of[p.nx & 1] = p.nx;
+    }
+    QP_HIP_CHECK(hipMemcpyAsync(w->ctl, hc0.data(), sizeof(Ctl) * nw, hipMemcpyHostToDevice, s));
     QP_HIP_CHECK(hipEventRecord(w->ev0, s));
-    if (R.np) MLAUNCH(k_mx_ion, R.pb, kPT, 0, m->P, R.g, w->rho_ion);
+    MLAUNCH(k_mx_ion, dim3(R.pb, nw), kPT, 0, m->P, R.g, w->rho_ion);
 
     const bool graph_ok = p.use_graph && !cb && !w->prof_class;
-    int delivered_stop = 0;
+    std::vector<char> delivered_stop(nw, 0);
+    std::vector<Ctl> hc(nw);
     for (int i = p.nx - 1; i >= 0; --i) {
         const int slot = i & 1;
-        if (p.resort_every > 0 && i < p.nx - 1 && ((p.nx - 1 - i) % p.resort_every) == 0 && R.np)
+        if (p.resort_every > 0 && i < p.nx - 1 && ((p.nx - 1 - i) % p.resort_every) == 0 && m->P.np)
             QP_TRY(resort(R, s));
         const bool generic = (i < p.nx - 1) && (i > 0) && (p.K > 0);
         if (graph_ok && generic) {
@@ -1037,47 +1173,60 @@ int wake_run_mixed(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
         }
         if (cb && i > 0) {
             QP_HIP_CHECK(hipStreamSynchronize(s));
-            Ctl hc;
-            QP_HIP_CHECK(hipMemcpy(&hc, w->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-            if (hc.stop_acc) {
-                QP_TRY(deliver_slice(w, cb, user, hc.stop_slice, QP_SLICE_STOPPED, nullptr, nullptr, nullptr,
-                                     nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz, w->chi));
-                delivered_stop = 1;
-                break;
+            QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
+            int running = 0;
+            for (int mb = 0; mb < nw; ++mb) {
+                if (delivered_stop[mb]) continue;
+                if (hc[mb].stop_acc) {
+                    QP_TRY(deliver_member(w, cb, user, mb, hc[mb].stop_slice, QP_SLICE_STOPPED, nullptr, nullptr,
+                                          nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz,
+                                          w->chi));
+                    delivered_stop[mb] = 1;
+                    continue;
+                }
+                ++running;
+                QP_TRY(deliver_member(w, cb, user, mb, i, QP_SLICE_NORMAL, w->psi_b[slot], w->rho, w->jx,
+                                      m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], w->ex, w->ey, w->ez, w->by,
+                                      w->bz, w->chi));
             }
-            QP_TRY(deliver_slice(w, cb, user, i, QP_SLICE_NORMAL, w->psi_b[slot], w->rho, w->jx,
-                                 m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], w->ex, w->ey, w->ez, w->by, w->bz,
-                                 w->chi));
+            if (!running) break;
         }
     }
     QP_HIP_CHECK(hipStreamSynchronize(s));
-    Ctl hc;
-    QP_HIP_CHECK(hipMemcpy(&hc, w->ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-    // output_step(0) after the loop (src/System_3d.cpp:622)
-    const int cur = hc.stop_acc ? hc.stop_slice : 0;
-    const int cslot = cur & 1;
-    hipLaunchKernelGGL(k_mx_out, dim3(R.gb), dim3(256), 0, s, w->ex, w->ey, w->ez, (const double *)w->psi_b[cslot],
-                       (const double *)w->psi_b[1 - cslot], (const double *)w->by, (const double *)w->bz, p.ny, p.nz,
-                       p.dx, p.dy, p.dz, p.nx > 1 ? 1 : 0, w->ctl, 0, 1);
+    // output_step(0) after the loop (src/System_3d.cpp:622); the kernel picks each
+    // member's psi slot from its own stop slice
+    hipLaunchKernelGGL(k_mx_out, dim3(R.gb, nw), dim3(256), 0, s, w->ex, w->ey, w->ez, (const double *)w->psi_b[0],
+                       (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx,
+                       p.dy, p.dz, p.nx > 1 ? 1 : 0, w->ctl, 0, 1);
     QP_HIP_CHECK(hipGetLastError());
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
+    QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
     float ms = 0.f;
     QP_HIP_CHECK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
     w->last_ms = ms;
-    if (hc.stop_acc && cb && !delivered_stop) {
-        QP_TRY(deliver_slice(w, cb, user, hc.stop_slice, QP_SLICE_STOPPED, nullptr, nullptr, nullptr, nullptr,
-                             nullptr, nullptr, nullptr, nullptr, w->by, w->bz, w->chi));
+    int any_stop = 0;
+    for (int mb = 0; mb < nw; ++mb) {
+        const bool stopped = hc[mb].stop_acc != 0;
+        any_stop |= stopped ? 1 : 0;
+        if (stopped && cb && !delivered_stop[mb]) {
+            QP_TRY(deliver_member(w, cb, user, mb, hc[mb].stop_slice, QP_SLICE_STOPPED, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz, w->chi));
+        }
+        if (cb) {
+            const int cslot = (stopped ? hc[mb].stop_slice : 0) & 1;
+            // a stopped member's B and chi planes hold the stopped slice: zero planes
+            // instead (zero2 has a zero plane for every member)
+            const double *by0 = stopped ? w->zero2 : w->by;
+            const double *bz0 = stopped ? w->zero2 : w->bz;
+            const double *chi0 = stopped ? w->zero2 : w->chi;
+            QP_TRY(deliver_member(w, cb, user, mb, 0, QP_SLICE_FINAL, w->psi_b[cslot], w->rho, w->jx,
+                                  m->jbuf[1 - cslot][0], m->jbuf[1 - cslot][1], w->ex, w->ey, w->ez, by0, bz0,
+                                  chi0));
+        }
+        if (stop_slices) stop_slices[mb] = stopped ? hc[mb].stop_slice : -1;
     }
-    if (cb) {
-        const double *by0 = hc.stop_acc ? w->zero2 : w->by;
-        const double *bz0 = hc.stop_acc ? w->zero2 : w->bz;
-        const double *chi0 = hc.stop_acc ? w->zero2 : w->chi;
-        QP_TRY(deliver_slice(w, cb, user, 0, QP_SLICE_FINAL, w->psi_b[cslot], w->rho, w->jx, m->jbuf[1 - cslot][0],
-                             m->jbuf[1 - cslot][1], w->ex, w->ey, w->ez, by0, bz0, chi0));
-    }
-    if (stop_slice) *stop_slice = hc.stop_acc ? hc.stop_slice : -1;
-    return hc.stop_acc ? QP_STOPPED : QP_OK;
+    return any_stop ? QP_STOPPED : QP_OK;
 }
 
 }  // namespace qp

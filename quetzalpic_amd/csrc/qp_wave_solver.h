@@ -4,8 +4,9 @@
 // Spectrum layout: column-major half spectrum spec[l * n1 + j] (l < nh =
 // n2/2 + 1 along the row axis, j < n1), so the column pass reads and writes
 // contiguous columns and the row passes move 2T consecutive rows per
-// frequency.  Same sources / sinks as the generic kernels of
-// qp_solver_kernels.h (functors with operator()(j, k) and begin()).
+// frequency.  Same sources / sinks and the same batch convention
+// (blockIdx.y = wake * nspec + which) as the generic kernels of
+// qp_solver_kernels.h.
 #pragma once
 
 #include "qp_solver_kernels.h"
@@ -53,14 +54,15 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c) {
 
 // Rows forward: rows (2p, 2p+1) packed as z = x_a + i x_b, complex FFT along
 // n2, X_a[k] = (Z[k] + conj Z[-k])/2, X_b[k] = (Z[k] - conj Z[-k])/(2i).
-// blockIdx.y selects spec0 / spec1.
 template <int N2, int E, int TB, class Src>
 __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, double2 *spec1, int n1,
-                                                  const double2 *tw2, const Ctl *ctl, int skip_mode) {
+                                                  const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
+                                                  size_t sstride) {
     using W = WfShape<N2, E, TB>;
-    if (solver_skip(ctl, skip_mode)) return;
+    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
     __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N2];
-    double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+    double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     const int tid = threadIdx.x;
     for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
     const int tt = tid / W::L, j = tid % W::L;
@@ -70,8 +72,8 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
     for (int r = 0; r < E; ++r) {
         const int k = j + W::L * r;
         double xa = 0.0, xb = 0.0;
-        if (ra < n1) xa = src(ra, k);
-        if (rb < n1) xb = src(rb, k);
+        if (ra < n1) xa = src(wk, which, ra, k);
+        if (rb < n1) xb = src(wk, which, rb, k);
         v[r] = make_double2(xa, xb);
     }
     wave_fft<N2, E, -1>(v, sb0 + tt * W::PADN, sb1 + tt * W::PADN, twl, j);
@@ -106,15 +108,19 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
 //   sum_k x[0][k]      = n2 Re G[0][0]
 // (H = divided spectrum, G = its inverse column transform, c_l = 1 for l = 0
 // and l = n2/2, else 2); per block: bpart[2b] = first sum, bpart[2b+1] =
-// second, and block 0 writes Re G[0][0] to bpart[2 gridDim.x].
+// second, and block 0 writes Re G[0][0] to bpart[2 gridDim.x] (each wake of
+// a batch at its own offset bstride).
 template <int N1, int E, int TB>
 __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, int nh, int n2, const double2 *tw1,
                                               const double *lam1, const double *lam2, double D, double *bpart,
-                                              const Ctl *ctl, int skip_mode) {
+                                              const Ctl *ctl, int skip_mode, int nspec, size_t sstride,
+                                              size_t bstride) {
     using W = WfShape<N1, E, TB>;
-    if (solver_skip(ctl, skip_mode)) return;
+    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
     __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N1];
-    double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
+    double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
+    if (bpart) bpart += wk * bstride;
     const int tid = threadIdx.x;
     for (int t = tid; t < N1; t += TB) twl[t] = tw1[t];
     const int tt = tid / W::L, j = tid % W::L;
@@ -168,12 +174,14 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
 // semantics).  Re z -> row 2p, Im z -> row 2p+1, handed to the sink.
 template <int N2, int E, int TB, class Sink>
 __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const double2 *spec1, Sink sink, int n1,
-                                                  const double2 *tw2, const Ctl *ctl, int skip_mode) {
+                                                  const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
+                                                  size_t sstride) {
     using W = WfShape<N2, E, TB>;
-    if (solver_skip(ctl, skip_mode)) return;
+    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
     __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N2];
-    const double2 *spec = blockIdx.y == 0 ? spec0 : spec1;
-    sink.begin();
+    const double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
+    sink.begin(wk);
     const int tid = threadIdx.x;
     for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
     const int tt = tid / W::L, j = tid % W::L;
@@ -202,8 +210,8 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int k = j + W::L * m;
-        sink(ra, k, v[m].x);
-        if (rb < n1) sink(rb, k, v[m].y);
+        sink(wk, which, ra, k, v[m].x);
+        if (rb < n1) sink(wk, which, rb, k, v[m].y);
     }
 }
 
@@ -221,27 +229,31 @@ inline int wcols_blocks(int n1, int nh) {
     return nb;
 }
 
+// nw wakes of one batch: spectra of wake w at spec0/spec1 + w * n1 * nh, control
+// blocks at ctl + w.
 template <class Src>
 int wlaunch_rows_fwd(const Solver2d &S, const Src &src, double2 *spec0, double2 *spec1, int nspec, hipStream_t s,
-                     const Ctl *ctl, int skip_mode) {
+                     const Ctl *ctl, int skip_mode, int nw = 1) {
+    const size_t sstride = (size_t)S.n1 * S.nh;
     wf_dispatch(S.n2, [&](auto NC) {
         constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
-        const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec);
+        const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec * nw);
         hipLaunchKernelGGL((kw_rows_fwd<N, E, TB, Src>), grid, dim3(TB), 0, s, src, spec0, spec1, S.n1,
-                           (const double2 *)S.ax2.dev.tw, ctl, skip_mode);
+                           (const double2 *)S.ax2.dev.tw, ctl, skip_mode, nspec, sstride);
     });
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
 }
 
 inline int wlaunch_cols(const Solver2d &S, double D, int nspec, double *bpart, hipStream_t s, const Ctl *ctl,
-                        int skip_mode) {
+                        int skip_mode, int nw = 1, size_t bstride = 0) {
+    const size_t sstride = (size_t)S.n1 * S.nh;
     wf_dispatch(S.n1, [&](auto NC) {
         constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
-        const dim3 grid(wcols_blocks_t(S.nh, WfShape<N, E, TB>::T), nspec);
+        const dim3 grid(wcols_blocks_t(S.nh, WfShape<N, E, TB>::T), nspec * nw);
         hipLaunchKernelGGL((kw_cols<N, E, TB>), grid, dim3(TB), 0, s, S.spec[0], S.spec[1], S.nh, S.n2,
                            (const double2 *)S.ax1.dev.tw, (const double *)S.d_lam1, (const double *)S.d_lam2, D,
-                           bpart, ctl, skip_mode);
+                           bpart, ctl, skip_mode, nspec, sstride, bstride);
     });
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
@@ -249,12 +261,13 @@ inline int wlaunch_cols(const Solver2d &S, double D, int nspec, double *bpart, h
 
 template <class Sink>
 int wlaunch_rows_inv(const Solver2d &S, const double2 *spec0, const double2 *spec1, const Sink &sink, int nspec,
-                     hipStream_t s, const Ctl *ctl, int skip_mode) {
+                     hipStream_t s, const Ctl *ctl, int skip_mode, int nw = 1) {
+    const size_t sstride = (size_t)S.n1 * S.nh;
     wf_dispatch(S.n2, [&](auto NC) {
         constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
-        const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec);
+        const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec * nw);
         hipLaunchKernelGGL((kw_rows_inv<N, E, TB, Sink>), grid, dim3(TB), 0, s, spec0, spec1, sink, S.n1,
-                           (const double2 *)S.ax2.dev.tw, ctl, skip_mode);
+                           (const double2 *)S.ax2.dev.tw, ctl, skip_mode, nspec, sstride);
     });
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
