@@ -137,6 +137,121 @@ def cpu_baseline(wl, budget_s, gpu_fields):
     return res, errs
 
 
+def bench_c4(args, capi, workloads):
+    """C4: 64 independent wakes.  The members of one grid size form a batch
+    (qp_wake_create_batch: every launch of a batch carries all its members); the
+    batches run concurrently, each on its own context (stream) from its own host
+    thread.  value = all members' particle-slice pushes of a pass / wall time."""
+    import threading
+    members = workloads.c4_members()
+    classes = {}
+    for m in members:
+        classes.setdefault(m.ny, []).append(m)
+    sizes = sorted(classes)
+    batches = {}
+    for n in sizes:
+        ms = classes[n]
+        m0 = ms[0]
+        ctx = capi.Context(0)
+        b = capi.WakeBatch(ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K,
+                           use_graph=0 if args.no_graph else 1)
+        for i, m in enumerate(ms):
+            y, z, w, py, pz = m.particles()
+            b.set_particles(i, y, z, w, py, pz)
+            b.set_laser_gaussian(i, *m.laser)
+        batches[n] = (ctx, b, sum(m.n_particles for m in ms))
+        log(f"[bench] C4 class {n}x{n}: {len(ms)} members, {batches[n][2]} particles")
+    pushes = sum(m.n_particles * m.nx for m in members)
+
+    def run_all():
+        errs = []
+
+        def one(b):
+            try:
+                b.run()
+            except Exception as e:  # surfaced below
+                errs.append(e)
+
+        ths = [threading.Thread(target=one, args=(b,)) for (_, b, _) in batches.values()]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        if errs:
+            raise errs[0]
+
+    for _ in range(args.warmup):
+        run_all()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_all()
+    t1 = time.perf_counter()
+    ms_per_step = 1e3 * (t1 - t0) / args.steps
+    value = pushes / (ms_per_step * 1e-3)
+    log(f"[bench] C4: {ms_per_step:.1f} ms per pass of 64 wakes = {value:.3e} pushes/s")
+    # each class alone (device time of one pass)
+    alone = {}
+    for n in sizes:
+        ctx, b, npart = batches[n]
+        b.run()
+        alone[f"{n}x{n}"] = {"members": len(classes[n]), "particles": npart, "ms": b.last_time_ms(),
+                             "pushes_per_s": npart * classes[n][0].nx / (b.last_time_ms() * 1e-3)}
+    # roofline of the dominant push kernel in the largest class
+    ctx, b, npart = batches[sizes[-1]]
+    kc = {}
+    for cls in ("push_A", "push_B", "push_C", "solve"):
+        nl, us = b.profile_kernel(cls)
+        kc[cls] = {"launches": nl, "mean_us": us, "total_ms": nl * us * 1e-3}
+    dom = max(("push_A", "push_B", "push_C"), key=lambda c: kc[c]["total_ms"])
+    dom_bytes = bytes_per_particle(dom, members[0].K) * npart
+    achieved = dom_bytes / (kc[dom]["mean_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_mx_{dom[-1]}",
+                "algorithmic_bytes_per_launch": dom_bytes, "class": f"{sizes[-1]}x{sizes[-1]}"}
+    # parity: the first member of the smallest class against the FP64 oracle, whole pass
+    err = None
+    cpu = None
+    if not args.no_check:
+        from oracle import oracle_ctypes as oc
+        m = classes[sizes[0]][0]
+        ctx, b, _ = batches[sizes[0]]
+        outs, _, _ = b.run(fields=("psi", "ex"))
+        g = oc.grid_from_box(m.lx, m.L, m.L, m.dx0, m.L / m.n0, m.L / m.n0, K=m.K)
+        y, z, w, py, pz = m.particles()
+        a = oc.gaussian_a_sqr(g, *m.laser)
+        t0 = time.perf_counter()
+        want, _, _ = oc.wake_run(g, y, z, w, a, py=py, pz=pz, fields=("psi", "ex"), variant="omp")
+        tc = time.perf_counter() - t0
+        err = {}
+        for f in ("psi", "ex"):
+            d = outs[0][f] - want[f]
+            err[f] = [float(np.abs(d).max() / np.abs(want[f]).max()),
+                      float(np.linalg.norm(d) / np.linalg.norm(want[f]))]
+        lib = oc.load("omp")
+        cpu = {"value": len(y) * g.nx / tc, "unit": "particle-slice pushes/s", "cores": int(lib.qpo_num_threads()),
+               "kind": "port", "sample": f"one whole C4 member ({m.ny}x{m.nz}, {len(y)} particles, {g.nx} slices), "
+                                         f"FP64, OpenMP atomics, -Ofast; {tc:.1f} s"}
+        log(f"[bench] C4 member vs oracle: {err}")
+    for (ctx, b, _) in batches.values():
+        b.close()
+        ctx.close()
+    line = {
+        "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, <=1e-5 rel err in psi and Ez vs FP64)",
+        "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "C4: 64 wakes over [-12,12]^2, grids from {64,96,128,192,256} (one batch per size, "
+                               "batches on concurrent streams), 4 ppc, 800 slices dxi=0.025, a0~U[0.5,3], "
+                               "w0~U[2,5], sigma_xi=2, transverse density up-ramp slope~U[0,0.1]",
+                   "members": {f"{n}x{n}": len(classes[n]) for n in sizes}, "particles": sum(
+                       m.n_particles for m in members), "slices": members[0].nx},
+        "classes_alone": alone, "kernel_classes_largest": kc, "roofline": roofline,
+        "err_vs_oracle_member": err, "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +276,9 @@ def main():
     from quetzalpic_amd import capi, workloads
     if args.calib_copy_mib:
         calib_copy(args.calib_copy_mib)
+    if args.workload == "C4":
+        bench_c4(args, capi, workloads)
+        return
     wl = workloads.get(args.workload)
     log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles}")
     ctx = capi.Context(0)

@@ -45,6 +45,9 @@ class Workload:
             base, curv = self.profile[1], self.profile[2]
             r2 = (y - self.L / 2) ** 2 + (z - self.L / 2) ** 2
             return -(base + curv * r2)
+        if kind == "ramp":  # -max(0, 1 + slope (y - L/2)): linear up-ramp across the box (extension)
+            slope = self.profile[1]
+            return -np.maximum(0.0, 1.0 + slope * (y - self.L / 2)) + 0.0 * z
         raise ValueError(kind)
 
     def particles(self):
@@ -68,6 +71,8 @@ class Workload:
 
     @property
     def n_particles(self):
+        if self.profile[0] == "ramp":
+            return len(self.particles()[0])
         return (self.ppc * self.ny) * (self.ppc * self.nz)
 
     def describe(self):
@@ -77,6 +82,26 @@ class Workload:
             "a0": self.a0, "w0": self.w0, "sigma_xi": self.sigma, "profile": list(self.profile),
             "note": self.note,
         }
+
+
+C4_SIZES = (64, 96, 128, 192, 256)
+
+
+def c4_members(count=64, seed=2024):
+    """C4: `count` independent wakes over [-12,12]^2 with grids drawn uniformly
+    from C4_SIZES, 4 ppc, 800 slices dxi = 0.025, a0 ~ U[0.5, 3], w0 ~ U[2, 5],
+    sigma_xi = 2, a linear density up-ramp across the box with slope ~ U[0, 0.1],
+    one generator seeded per instance."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng(seed + i)
+        n = int(rng.choice(C4_SIZES))
+        a0 = float(rng.uniform(0.5, 3.0))
+        w0 = float(rng.uniform(2.0, 5.0))
+        slope = float(rng.uniform(0.0, 0.1))
+        out.append(Workload(f"C4 member {i}", lx=20.0, dx=0.025, L=24.0, n=n, a0=a0, w0=w0, sigma=2.0,
+                            profile=("ramp", slope), seed=seed + i))
+    return out
 
 
 def get(name):

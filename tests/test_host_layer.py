@@ -22,3 +22,19 @@ def test_cli_needs_an_input():
     assert os.path.exists(exe)
     r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
     assert "Provide a path to the input .toml file" in r.stdout
+
+
+def test_c4_members_are_seeded_and_cover_every_grid_size():
+    """The C4 workload: 64 members, grids drawn from {64, 96, 128, 192, 256},
+    the same members on every call (seeded per instance), ramp densities >= 0."""
+    from quetzalpic_amd import workloads
+    a = workloads.c4_members()
+    b = workloads.c4_members()
+    assert len(a) == 64
+    assert {m.ny for m in a} == set(workloads.C4_SIZES)
+    for x, y in zip(a, b):
+        assert (x.ny, x.a0, x.w0, x.profile) == (y.ny, y.a0, y.w0, y.profile)
+        assert 0.5 <= x.a0 <= 3.0 and 2.0 <= x.w0 <= 5.0 and 0.0 <= x.profile[1] <= 0.1
+        assert x.nx == 800 and abs(x.dx - 0.025) < 1e-12
+    yy, zz, w, _, _ = a[3].particles()
+    assert (w <= 0).all() and len(w) == a[3].n_particles
