@@ -144,6 +144,8 @@ def bench_c4(args, capi, workloads):
     thread.  value = all members' particle-slice pushes of a pass / wall time."""
     import threading
     members = workloads.c4_members()
+    if args.c4_class:  # one grid-size class only (profiling runs)
+        members = [m for m in members if m.ny == args.c4_class]
     classes = {}
     for m in members:
         classes.setdefault(m.ny, []).append(m)
@@ -191,7 +193,7 @@ def bench_c4(args, capi, workloads):
     log(f"[bench] C4: {ms_per_step:.1f} ms per pass of 64 wakes = {value:.3e} pushes/s")
     # each class alone (device time of one pass)
     alone = {}
-    for n in sizes:
+    for n in (sizes if not args.no_profile else []):
         ctx, b, npart = batches[n]
         b.run()
         alone[f"{n}x{n}"] = {"members": len(classes[n]), "particles": npart, "ms": b.last_time_ms(),
@@ -199,15 +201,17 @@ def bench_c4(args, capi, workloads):
     # roofline of the dominant push kernel in the largest class
     ctx, b, npart = batches[sizes[-1]]
     kc = {}
-    for cls in ("push_A", "push_B", "push_C", "solve"):
+    for cls in (("push_A", "push_B", "push_C", "solve") if not args.no_profile else ()):
         nl, us = b.profile_kernel(cls)
         kc[cls] = {"launches": nl, "mean_us": us, "total_ms": nl * us * 1e-3}
-    dom = max(("push_A", "push_B", "push_C"), key=lambda c: kc[c]["total_ms"])
-    dom_bytes = bytes_per_particle(dom, members[0].K) * npart
-    achieved = dom_bytes / (kc[dom]["mean_us"] * 1e-6) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_mx_{dom[-1]}",
-                "algorithmic_bytes_per_launch": dom_bytes, "class": f"{sizes[-1]}x{sizes[-1]}"}
+    roofline = None
+    if kc:
+        dom = max(("push_A", "push_B", "push_C"), key=lambda c: kc[c]["total_ms"])
+        dom_bytes = bytes_per_particle(dom, members[0].K) * npart
+        achieved = dom_bytes / (kc[dom]["mean_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_mx_{dom[-1]}",
+                    "algorithmic_bytes_per_launch": dom_bytes, "class": f"{sizes[-1]}x{sizes[-1]}"}
     # parity: the first member of the smallest class against the FP64 oracle, whole pass
     err = None
     cpu = None
@@ -265,6 +269,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the slice step directly instead of replaying hipGraphs (for tracers)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel-class event pass")
+    ap.add_argument("--c4-class", type=int, default=0,
+                    help="C4 only: run the members of this grid size alone (profiling runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()

@@ -96,11 +96,42 @@ __device__ __forceinline__ void split_coord(double Y, int n, int &j, uint32_t &o
 }
 
 // Periodic wrap in cell units (normalize_coordinates, src/System_3d.cpp:668-684).
+// A step moves a particle by less than one box width, so fmod reduces to one
+// subtraction (exact: Y - n with n <= Y < 2n is representable) or one
+// addition (the reference's y += l); fmod stays for anything further out.
 __device__ __forceinline__ double wrap_cells(double Y, int n) {
-    Y = fmod(Y, (double)n);
-    if (Y < 0) Y += n;
-    if (Y >= n) Y = 0.0;
+    const double dn = (double)n;
+    if (Y >= 0.0 && Y < dn) return Y;
+    if (Y >= dn && Y < 2.0 * dn) {
+        Y -= dn;
+    } else if (Y < 0.0 && Y >= -dn) {
+        Y += dn;
+    } else {
+        Y = fmod(Y, dn);
+        if (Y < 0) Y += dn;
+    }
+    if (Y >= dn) Y = 0.0;
     return Y;
+}
+
+// Element i of a grid through a 32-bit byte offset from the (uniform) base:
+// the compiler then uses the scalar-base + 32-bit-offset load form instead of
+// 64-bit address arithmetic per lane.  Grids hold at most 1024 x 1024 nodes
+// per member, and member bases are applied to the pointer.
+__device__ __forceinline__ double ld_at(const double *a, unsigned i) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(a) + (size_t)(i << 3));
+}
+__device__ __forceinline__ double *at(double *a, unsigned i) {
+    return reinterpret_cast<double *>(reinterpret_cast<char *>(a) + (size_t)(i << 3));
+}
+// Node index j * n2 + k with a 24-bit multiply (j, n2 <= 1024).
+__device__ __forceinline__ unsigned nidx(int j, int k, int n2) { return __umul24((unsigned)j, (unsigned)n2) + (unsigned)k; }
+
+__device__ __forceinline__ double gather4u(const double *a, int n2, int j1, int j2, int k1, int k2, double yf,
+                                           double zf) {
+    const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2);
+    return ld_at(a, r1 + k1) * (1 - yf) * (1 - zf) + ld_at(a, r2 + k1) * yf * (1 - zf) +
+           ld_at(a, r1 + k2) * (1 - yf) * zf + ld_at(a, r2 + k2) * yf * zf;
 }
 
 __device__ __forceinline__ Pos pos_from_cells(double Y, double Z, int n1, int n2) {
@@ -139,7 +170,7 @@ __device__ __forceinline__ double mx_gather(const double *a, const Pos &q, int n
     double yf, zf;
     ax_node(q.j, q.fy, n1, j1, j2, yf);
     ax_node(q.k, q.fz, n2, k1, k2, zf);
-    return gather4(a, n2, j1, j2, k1, k2, yf, zf);
+    return gather4u(a, n2, j1, j2, k1, k2, yf, zf);
 }
 // Gathers on the staggered B grids: B_y has origin (0, dz/2), B_z (dy/2, 0).
 __device__ __forceinline__ double mx_gather_hz(const double *a, const Pos &q, int n1, int n2) {
@@ -147,14 +178,14 @@ __device__ __forceinline__ double mx_gather_hz(const double *a, const Pos &q, in
     double yf, zf;
     ax_node(q.j, q.fy, n1, j1, j2, yf);
     ax_half(q.k, q.fz, n2, k1, k2, zf);
-    return gather4(a, n2, j1, j2, k1, k2, yf, zf);
+    return gather4u(a, n2, j1, j2, k1, k2, yf, zf);
 }
 __device__ __forceinline__ double mx_gather_hy(const double *a, const Pos &q, int n1, int n2) {
     int j1, j2, k1, k2;
     double yf, zf;
     ax_half(q.j, q.fy, n1, j1, j2, yf);
     ax_node(q.k, q.fz, n2, k1, k2, zf);
-    return gather4(a, n2, j1, j2, k1, k2, yf, zf);
+    return gather4u(a, n2, j1, j2, k1, k2, yf, zf);
 }
 // Staggered-derivative gathers (src/array_utils.cpp:240-288).
 // The half-node differences are interpolated first and scaled once by 1/d.
@@ -164,10 +195,13 @@ __device__ __forceinline__ double mx_yder(const double *a, const Pos &q, int n1,
     ax_half(q.j, q.fy, n1, j1, j2, yf);
     const int j3 = j2 + 1 == n1 ? 0 : j2 + 1;
     ax_node(q.k, q.fz, n2, k1, k2, zf);
-    const double g11 = a[(size_t)n2 * j2 + k1] - a[(size_t)n2 * j1 + k1];
-    const double g21 = a[(size_t)n2 * j3 + k1] - a[(size_t)n2 * j2 + k1];
-    const double g12 = a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j1 + k2];
-    const double g22 = a[(size_t)n2 * j3 + k2] - a[(size_t)n2 * j2 + k2];
+    const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2),
+                   r3 = __umul24((unsigned)j3, (unsigned)n2);
+    const double a21 = ld_at(a, r2 + k1), a22 = ld_at(a, r2 + k2);
+    const double g11 = a21 - ld_at(a, r1 + k1);
+    const double g21 = ld_at(a, r3 + k1) - a21;
+    const double g12 = a22 - ld_at(a, r1 + k2);
+    const double g22 = ld_at(a, r3 + k2) - a22;
     return (g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf) * inv_d1;
 }
 __device__ __forceinline__ double mx_zder(const double *a, const Pos &q, int n1, int n2, double inv_d2) {
@@ -176,10 +210,12 @@ __device__ __forceinline__ double mx_zder(const double *a, const Pos &q, int n1,
     ax_node(q.j, q.fy, n1, j1, j2, yf);
     ax_half(q.k, q.fz, n2, k1, k2, zf);
     const int k3 = k2 + 1 == n2 ? 0 : k2 + 1;
-    const double g11 = a[(size_t)n2 * j1 + k2] - a[(size_t)n2 * j1 + k1];
-    const double g21 = a[(size_t)n2 * j2 + k2] - a[(size_t)n2 * j2 + k1];
-    const double g12 = a[(size_t)n2 * j1 + k3] - a[(size_t)n2 * j1 + k2];
-    const double g22 = a[(size_t)n2 * j2 + k3] - a[(size_t)n2 * j2 + k2];
+    const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2);
+    const double a12 = ld_at(a, r1 + k2), a22 = ld_at(a, r2 + k2);
+    const double g11 = a12 - ld_at(a, r1 + k1);
+    const double g21 = a22 - ld_at(a, r2 + k1);
+    const double g12 = ld_at(a, r1 + k3) - a12;
+    const double g22 = ld_at(a, r2 + k3) - a22;
     return (g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf) * inv_d2;
 }
 
@@ -346,10 +382,10 @@ __device__ __forceinline__ void dep4w(double *lds, const Win &w, double *glob, i
             return;
         }
     }
-    unsafeAtomicAdd(&glob[(size_t)n2 * j1 + k1], a11);
-    unsafeAtomicAdd(&glob[(size_t)n2 * j2 + k1], a21);
-    unsafeAtomicAdd(&glob[(size_t)n2 * j1 + k2], a12);
-    unsafeAtomicAdd(&glob[(size_t)n2 * j2 + k2], a22);
+    unsafeAtomicAdd(at(glob, nidx(j1, k1, n2)), a11);
+    unsafeAtomicAdd(at(glob, nidx(j2, k1, n2)), a21);
+    unsafeAtomicAdd(at(glob, nidx(j1, k2, n2)), a12);
+    unsafeAtomicAdd(at(glob, nidx(j2, k2, n2)), a22);
 }
 
 // Fold-free deposit of value v (CIC weights computed here).
@@ -376,9 +412,10 @@ __device__ inline void win_flush(const double *lds, double *const *glob, int ngr
         if (j >= n1) j -= n1;
         int k = w.k0 + dk;
         if (k >= n2) k -= n2;
+        const unsigned t = nidx(j, k, n2);
         for (int g = 0; g < ngrid; ++g) {
             const double v = lds[g * kWinCap + i];
-            if (v != 0.0) unsafeAtomicAdd(&glob[g][(size_t)n2 * j + k], v);
+            if (v != 0.0) unsafeAtomicAdd(at(glob[g], t), v);
         }
     }
 }

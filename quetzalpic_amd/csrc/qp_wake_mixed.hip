@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
     }
     const int lp = blockIdx.x * kPT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
-    const int p = wk * P.pstride + lp;
+    const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos q{0, 0, 0.0, 0.0};
     double w = 0.0;
     if (valid) {
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     const double *asq = mx_asq(ctl, si, nn, go);
     const int lp = blockIdx.x * kPT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
-    const int p = wk * P.pstride + lp;
+    const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos qm{0, 0, 0.0, 0.0};
     double w = 0.0;
     if (valid) {
@@ -248,7 +248,11 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
 
 // ------------------------------------------------------------------ C phase
 // ngr = 5 grids: jy*, jz*, rho, jx, chi.
-__global__ __launch_bounds__(kPT) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, const double *psi,
+// Second launch bound (waves per SIMD): C fits 80 VGPRs at 6 waves without
+// spills (5 otherwise), which hides more of its gather -> deposit -> flush chain
+// in batches (256^2 x 10 members: 69.8 -> 65.9 us).  The same bound on B
+// (56 VGPRs, 8 waves) made it slower (77.9 -> 83.2 us) and is not used.
+__global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, const double *psi,
                                               const double *psim, double *rho, double *jx, double *chi,
                                               double *jyn, double *jzn, Ctl *ctl) {
     const int wk = blockIdx.y;
@@ -269,7 +273,7 @@ __global__ __launch_bounds__(kPT) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, 
     const double *asq = mx_asq(ctl, si, nn, go);
     const int lp = blockIdx.x * kPT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
-    const int p = wk * P.pstride + lp;
+    const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos q{0, 0, 0.0, 0.0}, qm{0, 0, 0.0, 0.0};
     double vjy = 0, vjz = 0, vjx = 0, vrho = 0, vchi = 0;
     if (valid) {
@@ -364,7 +368,7 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
     const double *asq = mx_asq(ctl, si, nn, go);
     const int lp = blockIdx.x * kPT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
-    const int p = wk * P.pstride + lp;
+    const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos q{0, 0, 0.0, 0.0};
     double vjx = 0, vrho = 0, vchi = 0;
     if (valid) {
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(kPT) void k_mx_ion(MxPart P, MxGeom g, double *rho_
     const int wk = blockIdx.y;
     const int lp = blockIdx.x * kPT + threadIdx.x;
     if (lp >= mx_np(P, wk)) return;
-    const int p = wk * P.pstride + lp;
+    const unsigned p = (unsigned)(wk * P.pstride + lp);
     rho_ion += (size_t)wk * g.n1 * g.n2;
     const Pos q = load_pos(P, p);
     int j1, j2, k1, k2;
