@@ -117,3 +117,35 @@ def get(name):
                         "laser a0=4 w0=3.5 sigma=1.5, thermal p N(0,1e-3)", lx=20.0, dx=0.01, L=30.0, n=512,
                         a0=4.0, w0=3.5, sigma=1.5, thermal=1e-3)
     raise KeyError(name)
+
+
+def write_members(members, path):
+    """Member list for the native batch runner (quetzalpic_amd/bin/qp_batch_run):
+    one geometry line, then one line per member (laser in the reference's keys,
+    ppc per axis, profile kind and parameters).  All members share the first
+    member's geometry."""
+    m0 = members[0]
+    with open(path, "w") as f:
+        f.write("# qp batch members v1\n")
+        f.write(f"geometry {m0.nx} {m0.ny} {m0.nz} {m0.dx!r} {m0.dy!r} {m0.dz!r} {m0.L!r} {m0.L!r} {m0.K}\n")
+        for m in members:
+            assert (m.nx, m.ny, m.nz) == (m0.nx, m0.ny, m0.nz)
+            kind = m.profile[0]
+            p1 = -m.profile[1] if kind == "constant" else m.profile[1]
+            p2 = m.profile[2] if len(m.profile) > 2 else 0.0
+            f.write("member " + " ".join(repr(float(v)) for v in m.laser) + f" {m.ppc} {kind} {p1!r} {p2!r}\n")
+
+
+if __name__ == "__main__":
+    import sys
+    # python -m quetzalpic_amd.workloads members <C2|C4:<n>> <out-file>
+    if len(sys.argv) != 4 or sys.argv[1] != "members":
+        sys.exit("usage: python -m quetzalpic_amd.workloads members <C2|C4:<grid size>> <out-file>")
+    what = sys.argv[2]
+    if what.startswith("C4:"):
+        n = int(what[3:])
+        ms = [m for m in c4_members() if m.ny == n]
+    else:
+        ms = [get(what)]
+    write_members(ms, sys.argv[3])
+    print(f"{len(ms)} members -> {sys.argv[3]}")
