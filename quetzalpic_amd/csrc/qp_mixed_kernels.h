@@ -388,6 +388,39 @@ __device__ __forceinline__ void dep4w(double *lds, const Win &w, double *glob, i
     unsafeAtomicAdd(at(glob, nidx(j2, k2, n2)), a22);
 }
 
+// G grids sharing one stencil (weights a[4 g .. 4 g + 3] for grid g, LDS
+// windows G apart by kWinCap): one slot computation and one branch for all.
+template <int G>
+__device__ __forceinline__ void dep4w_multi(double *lds, const Win &w, double *const *glob, int n1, int n2, int j1,
+                                            int j2, int k1, int k2, const double *a) {
+    if (w.lds) {
+        const int s11 = win_slot(w, j1, k1, n1, n2), s21 = win_slot(w, j2, k1, n1, n2);
+        const int s12 = win_slot(w, j1, k2, n1, n2), s22 = win_slot(w, j2, k2, n1, n2);
+        if (s11 >= 0 && s21 >= 0 && s12 >= 0 && s22 >= 0) {
+            lds_f64 *l = (lds_f64 *)lds;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                __hip_atomic_fetch_add(l + g * kWinCap + s11, a[4 * g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(l + g * kWinCap + s21, a[4 * g + 1], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(l + g * kWinCap + s12, a[4 * g + 2], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(l + g * kWinCap + s22, a[4 * g + 3], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            return;
+        }
+    }
+    const unsigned t11 = nidx(j1, k1, n2), t21 = nidx(j2, k1, n2), t12 = nidx(j1, k2, n2), t22 = nidx(j2, k2, n2);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        unsafeAtomicAdd(at(glob[g], t11), a[4 * g]);
+        unsafeAtomicAdd(at(glob[g], t21), a[4 * g + 1]);
+        unsafeAtomicAdd(at(glob[g], t12), a[4 * g + 2]);
+        unsafeAtomicAdd(at(glob[g], t22), a[4 * g + 3]);
+    }
+}
+
 // Fold-free deposit of value v (CIC weights computed here).
 __device__ __forceinline__ void dep4(double *lds, const Win &w, double *glob, int n1, int n2, int j1, int j2, int k1,
                                      int k2, double yf, double zf, double v) {

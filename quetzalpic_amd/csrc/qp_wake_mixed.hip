@@ -295,12 +295,13 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         const double p2 = 0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs);
         const double gm = gamma_of(p2, a, ps, inv1ps);
         const double px = 0.5 * (1 + p2 + a - (1 + ps) * (1 + ps)) * inv1ps;
-        // P7 (:523-526): vx = px/gamma, 1/(1-vx) = gamma/(gamma-px) = gamma/(1+psi)
-        const double vx = px / gm;
-        const double inv1mvx = 1.0 / (1 - vx);
-        vjx = w * vx * inv1mvx;
-        vrho = w * inv1mvx;
-        vchi = -w * inv1mvx / gm;
+        // P7 (:523-526) with vx = px/gamma and 1/(1-vx) = gamma/(gamma-px) = gamma/(1+psi)
+        // (gamma - px = 1 + psi exactly, from the two closed forms): n vx/(1-vx) =
+        // n px/(1+psi), n/(1-vx) = n gamma/(1+psi), -n/((1-vx) gamma) = -n/(1+psi).
+        // No divisions, and no cancellation in gamma - px near psi -> -1.
+        vjx = w * px * inv1ps;
+        vrho = w * gm * inv1ps;
+        vchi = -w * inv1ps;
     }
     if (threadIdx.x == 0) {
         sh[4 * (kPT / 64)] = q.j;
@@ -332,6 +333,8 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         cic_w(vrho, ycf, zcf, ac);
         cic_w(vjx, ycf, zcf, ac + 4);
         cic_w(vchi, ycf, zcf, ac + 8);
+        // the quad fold also pays in VALU-bound batches: without it the same-address
+        // LDS conflicts cost more (256^2 x 10 members: 61.4 us folded, 86.8 us not)
         bool fa = valid, fb = valid, fc = valid;
         quad_fold<4>(ya1 * n2 + za1, fa, aa);
         quad_fold<4>(yb1 * n2 + zb1, fb, ab);
@@ -339,9 +342,8 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         if (fa) dep4w(lds, win, jyn, n1, n2, ya1, ya2, za1, za2, aa[0], aa[1], aa[2], aa[3]);
         if (fb) dep4w(lds + kWinCap, win, jzn, n1, n2, yb1, yb2, zb1, zb2, ab[0], ab[1], ab[2], ab[3]);
         if (fc) {
-            dep4w(lds + 2 * kWinCap, win, rho, n1, n2, yc1, yc2, zc1, zc2, ac[0], ac[1], ac[2], ac[3]);
-            dep4w(lds + 3 * kWinCap, win, jx, n1, n2, yc1, yc2, zc1, zc2, ac[4], ac[5], ac[6], ac[7]);
-            dep4w(lds + 4 * kWinCap, win, chi, n1, n2, yc1, yc2, zc1, zc2, ac[8], ac[9], ac[10], ac[11]);
+            double *const gc[3] = {rho, jx, chi};
+            dep4w_multi<3>(lds + 2 * kWinCap, win, gc, n1, n2, yc1, yc2, zc1, zc2, ac);
         }
     }
     __syncthreads();
@@ -380,11 +382,10 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
         const double inv1ps = 1.0 / (1 + ps);
         const double gm = gamma_of(py * py + pz * pz, a, ps, inv1ps);
         const double px = 0.5 * (1 + py * py + pz * pz + a - (1 + ps) * (1 + ps)) * inv1ps;
-        const double vx = px / gm;
-        const double inv1mvx = 1.0 / (1 - vx);
-        vjx = w * vx * inv1mvx;
-        vrho = w * inv1mvx;
-        vchi = -w * inv1mvx / gm;
+        // S5 (:390-397) with 1/(1-vx) = gamma/(1+psi), as in the C phase
+        vjx = w * px * inv1ps;
+        vrho = w * gm * inv1ps;
+        vchi = -w * inv1ps;
     }
     if (threadIdx.x == 0) {
         sh[4 * (kPT / 64)] = q.j;
@@ -409,9 +410,8 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
         bool act = valid;
         quad_fold<12>(y1 * n2 + z1, act, ac);
         if (act) {
-            dep4w(lds, win, rho, n1, n2, y1, y2, z1, z2, ac[0], ac[1], ac[2], ac[3]);
-            dep4w(lds + kWinCap, win, jx, n1, n2, y1, y2, z1, z2, ac[4], ac[5], ac[6], ac[7]);
-            dep4w(lds + 2 * kWinCap, win, chi, n1, n2, y1, y2, z1, z2, ac[8], ac[9], ac[10], ac[11]);
+            double *const gc[3] = {rho, jx, chi};
+            dep4w_multi<3>(lds, win, gc, n1, n2, y1, y2, z1, z2, ac);
         }
     }
     __syncthreads();
