@@ -82,6 +82,8 @@ typedef struct {
     int precision;           /* QP_PREC_FP64 | QP_PREC_MIXED */
     int use_graph;           /* mixed path: capture the slice step in a hipGraph (1) or not (0) */
     int resort_every;        /* mixed path: re-bin particles by cell tile every N slices (0 = never) */
+    int longitudinal_b;      /* mixed path, extension with no reference counterpart: 1 = also solve
+                                lap_perp B_x = dz j_y - dy j_z per slice (output plane bx) */
 } qp_wake_params;
 
 /* Fields of one slice handed to the per-slice callback (FP64, host memory,
@@ -98,6 +100,8 @@ typedef struct {
     int kind;
     const double *psi, *rho, *jx, *jy, *jz;   /* 2D arrays written by write_slice */
     const double *ex, *ey, *ez, *by, *bz, *chi;  /* slice i of the 3D arrays */
+    const double *bx;        /* longitudinal B at the cell corners (dy/2, dz/2), from the j_y, j_z
+                                of this view; NULL unless longitudinal_b */
 } qp_slice_view;
 typedef int (*qp_slice_cb)(const qp_slice_view *view, void *user);
 

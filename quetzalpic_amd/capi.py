@@ -33,11 +33,12 @@ class WakeParams(ctypes.Structure):
         ("K", ctypes.c_int), ("D", ctypes.c_double),
         ("psi_threshold", ctypes.c_double), ("b_threshold", ctypes.c_double),
         ("precision", ctypes.c_int), ("use_graph", ctypes.c_int), ("resort_every", ctypes.c_int),
+        ("longitudinal_b", ctypes.c_int),
     ]
 
 
 class SliceView(ctypes.Structure):
-    _fields_ = [("slice", ctypes.c_int), ("kind", ctypes.c_int)] + [(n, _dp) for n in FIELD_NAMES]
+    _fields_ = [("slice", ctypes.c_int), ("kind", ctypes.c_int)] + [(n, _dp) for n in FIELD_NAMES] + [("bx", _dp)]
 
 
 SLICE_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(SliceView), ctypes.c_void_p)
@@ -180,11 +181,11 @@ class Wake:
     """One wake on the device: System_3d::solve_wakefield behind the C-ABI."""
 
     def __init__(self, ctx, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
-                 b_threshold=100.0, precision=QP_PREC_FP64, use_graph=1, resort_every=0):
+                 b_threshold=100.0, precision=QP_PREC_FP64, use_graph=1, resort_every=0, longitudinal_b=0):
         self.ctx = ctx
         self.shape = (nx, ny, nz)
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
-                                 precision, use_graph, resort_every)
+                                 precision, use_graph, resort_every, longitudinal_b)
         self.h = ctypes.c_void_p()
         _check(load().qp_wake_create(ctx.h, ctypes.byref(self.params), ctypes.byref(self.h)), "qp_wake_create")
         self.np = 0
@@ -227,10 +228,12 @@ class Wake:
                 v = vp.contents
                 i = v.slice
                 for k in out:
-                    arr = np.ctypeslib.as_array(getattr(v, k), shape=(nn,)).reshape(ny, nz)
                     if v.kind == QP_SLICE_STOPPED and k not in ("by", "bz", "chi"):
                         continue
-                    out[k][i] = arr
+                    ptr = getattr(v, k)
+                    if not ptr:  # bx without longitudinal_b
+                        continue
+                    out[k][i] = np.ctypeslib.as_array(ptr, shape=(nn,)).reshape(ny, nz)
                 if callback:
                     return int(callback(v) or 0)
                 return 0
@@ -282,12 +285,12 @@ class WakeBatch:
     member has its own particles, Gaussian laser, fields and stop rule."""
 
     def __init__(self, ctx, nwakes, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
-                 b_threshold=100.0, use_graph=1):
+                 b_threshold=100.0, use_graph=1, longitudinal_b=0):
         self.ctx = ctx
         self.nwakes = nwakes
         self.shape = (nx, ny, nz)
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
-                                 QP_PREC_MIXED, use_graph, 0)
+                                 QP_PREC_MIXED, use_graph, 0, longitudinal_b)
         self.h = ctypes.c_void_p()
         _check(load().qp_wake_create_batch(ctx.h, ctypes.byref(self.params), nwakes, ctypes.byref(self.h)),
                "qp_wake_create_batch")
@@ -320,7 +323,10 @@ class WakeBatch:
                 for k in out:
                     if v.kind == QP_SLICE_STOPPED and k not in ("by", "bz", "chi"):
                         continue
-                    out[k][v.slice] = np.ctypeslib.as_array(getattr(v, k), shape=(nn,)).reshape(ny, nz)
+                    ptr = getattr(v, k)
+                    if not ptr:
+                        continue
+                    out[k][v.slice] = np.ctypeslib.as_array(ptr, shape=(nn,)).reshape(ny, nz)
                 if callback:
                     return int(callback(member, v) or 0)
                 return 0

@@ -178,6 +178,8 @@ int qp_wake_create_batch(qp_ctx ctx, const qp_wake_params *params, int nwakes, q
     ARG_CHECK(p.precision == QP_PREC_FP64 || p.precision == QP_PREC_MIXED, "qp_wake_create: unknown precision");
     ARG_CHECK(nwakes == 1 || p.precision == QP_PREC_MIXED, "qp_wake_create_batch: batches need QP_PREC_MIXED");
     ARG_CHECK(nwakes == 1 || p.resort_every == 0, "qp_wake_create_batch: re-binning is for single wakes only");
+    ARG_CHECK(p.longitudinal_b == 0 || p.precision == QP_PREC_MIXED,
+              "qp_wake_create: longitudinal_b is an extension of the mixed path");
     QP_HIP_CHECK(hipSetDevice(ctx->device));
     auto *h = new (std::nothrow) qp_wake_s();
     if (!h) return QP_ERR_ALLOC;
@@ -410,19 +412,19 @@ int wake_alloc_common(Wake *w) {
     const size_t nn = w->nn();
     const size_t nw = (size_t)w->nw;
     QP_TRY(w->S.init(p.ny, p.nz, p.dy, p.dz, w->nw));
-    const int ngrids = 26;
+    const int ngrids = 27;
     QP_HIP_CHECK(hipMalloc(&w->g_base, sizeof(double) * nn * nw * ngrids));
     QP_HIP_CHECK(hipMemset(w->g_base, 0, sizeof(double) * nn * nw * ngrids));
     double **grids[] = {&w->src, &w->psi, &w->psi_prev, &w->psim, &w->rho, &w->jx, &w->jy, &w->jz, &w->jyn,
                         &w->jzn, &w->djy, &w->djz, &w->rho_ion, &w->by, &w->bz, &w->chi, &w->ex, &w->ey,
-                        &w->ez, &w->src2, &w->asq2, &w->psi_b[0], &w->psi_b[1], &w->zero2};
+                        &w->ez, &w->src2, &w->asq2, &w->psi_b[0], &w->psi_b[1], &w->zero2, &w->bx};
     int gi = 0;
     for (double **g : grids) *g = w->g_base + nn * nw * (gi++);
     QP_HIP_CHECK(hipMalloc(&w->ctl, sizeof(Ctl) * nw));
     QP_HIP_CHECK(hipMemset(w->ctl, 0, sizeof(Ctl) * nw));
     QP_HIP_CHECK(hipMalloc(&w->d_laser, sizeof(LaserGauss) * nw));
     QP_HIP_CHECK(hipMemset(w->d_laser, 0, sizeof(LaserGauss) * nw));
-    QP_HIP_CHECK(hipHostMalloc(&w->h_out, sizeof(double) * nn * 11, hipHostMallocDefault));
+    QP_HIP_CHECK(hipHostMalloc(&w->h_out, sizeof(double) * nn * 12, hipHostMallocDefault));
     QP_HIP_CHECK(hipEventCreate(&w->ev0));
     QP_HIP_CHECK(hipEventCreate(&w->ev1));
     if (p.precision == QP_PREC_MIXED) {
@@ -468,11 +470,12 @@ void prof_end(Wake *w, const char *cls) {
 
 int deliver_slice(Wake *w, qp_slice_cb cb, void *user, int slice, int kind, const double *psi, const double *rho,
                   const double *jx, const double *jy, const double *jz, const double *ex, const double *ey,
-                  const double *ez, const double *by, const double *bz, const double *chi) {
+                  const double *ez, const double *by, const double *bz, const double *chi, const double *bx) {
     const size_t nn = w->nn();
     hipStream_t s = w->stream();
-    const double *src[11] = {psi, rho, jx, jy, jz, ex, ey, ez, by, bz, chi};
-    for (int f = 0; f < 11; ++f)
+    const double *src[12] = {psi, rho, jx, jy, jz, ex, ey, ez, by, bz, chi, bx};
+    const int nf = bx ? 12 : 11;
+    for (int f = 0; f < nf; ++f)
         QP_HIP_CHECK(hipMemcpyAsync(w->h_out + f * nn, src[f] ? src[f] : w->zero2, sizeof(double) * nn,
                                     hipMemcpyDeviceToHost, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
@@ -491,6 +494,7 @@ int deliver_slice(Wake *w, qp_slice_cb cb, void *user, int slice, int kind, cons
     v.by = h + 8 * nn;
     v.bz = h + 9 * nn;
     v.chi = h + 10 * nn;
+    v.bx = bx ? h + 11 * nn : nullptr;
     int r = cb(&v, user);
     if (r != 0) {
         set_error("slice callback returned " + std::to_string(r));

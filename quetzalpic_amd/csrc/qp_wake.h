@@ -48,14 +48,14 @@ struct Wake {
            *jx = nullptr, *jy = nullptr, *jz = nullptr, *jyn = nullptr, *jzn = nullptr, *djy = nullptr,
            *djz = nullptr, *rho_ion = nullptr, *by = nullptr, *bz = nullptr, *chi = nullptr, *ex = nullptr,
            *ey = nullptr, *ez = nullptr, *src2 = nullptr, *asq2 = nullptr, *psi_b[2] = {nullptr, nullptr},
-           *zero2 = nullptr;
+           *zero2 = nullptr, *bx = nullptr;
     // driver inputs (the 3D arrays only for a single wake)
     double *a_sqr3 = nullptr;            // nx*ny*nz or null
     LaserGauss *d_laser = nullptr;       // nw members' lasers on the device
     double *rho_b3 = nullptr, *jx_b3 = nullptr;  // nx*ny*nz or null
 
     Ctl *ctl = nullptr;                  // nw control blocks
-    double *h_out = nullptr;             // pinned host staging, 11 planes
+    double *h_out = nullptr;             // pinned host staging, 12 planes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
 
@@ -99,10 +99,11 @@ const double *wake_jx_b(Wake *w, int i);
 void prof_begin(Wake *w, const char *cls);
 void prof_end(Wake *w, const char *cls);
 
-// Stage the 11 output planes of slice i to host and call cb.
+// Stage the output planes of slice i to host and call cb (bx may be null).
 int deliver_slice(Wake *w, qp_slice_cb cb, void *user, int slice, int kind, const double *psi,
                   const double *rho, const double *jx, const double *jy, const double *jz, const double *ex,
-                  const double *ey, const double *ez, const double *by, const double *bz, const double *chi);
+                  const double *ey, const double *ez, const double *by, const double *bz, const double *chi,
+                  const double *bx = nullptr);
 
 }  // namespace qp
 

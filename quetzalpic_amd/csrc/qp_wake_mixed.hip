@@ -558,6 +558,33 @@ struct SinkBPair {
     }
 };
 
+// Longitudinal B (extension, no reference counterpart): in the quasistatic
+// limit (dx^2 - dt^2 vanishes on functions of t - x) the x component of
+// curl curl B = curl j reads lap_perp B_x = dz j_y - dy j_z.  With j_y on
+// (dy/2, 0) and j_z on (0, dz/2) the source sits on the cell corners
+// (dy/2, dz/2): (j_y[j][k+1] - j_y[j][k])/dz - (j_z[j+1][k] - j_z[j][k])/dy,
+// periodic.  Solved with D = 0 (the (0,0) mode dropped: B_x has zero mean).
+struct SrcCurl {
+    const double *jy, *jz;
+    int n1, n2;
+    double dy, dz;
+    __device__ double operator()(int wk, int, int j, int k) const {
+        const size_t go = (size_t)wk * n1 * n2;
+        const int kp = k + 1 == n2 ? 0 : k + 1, jp = j + 1 == n1 ? 0 : j + 1;
+        const size_t t = go + (size_t)j * n2 + k;
+        return (jy[go + (size_t)j * n2 + kp] - jy[t]) / dz - (jz[go + (size_t)jp * n2 + k] - jz[t]) / dy;
+    }
+};
+
+struct SinkMember {
+    double *dst;
+    int n1, n2;
+    __device__ void begin(int) {}
+    __device__ void operator()(int wk, int, int j, int k, double v) const {
+        dst[(size_t)wk * n1 * n2 + (size_t)j * n2 + k] = v / n1 / n2;
+    }
+};
+
 // Column pass of the psi solves, with the partial sums the border mean needs:
 // after the inverse column transform G[j][l], the c2r row value at k = 0 is
 // sum_l c_l Re G[j][l] (c_0 = c_{n2/2} = 1, else 2) and sum_k x[0][k] = n2 Re G[0][0].
@@ -1024,6 +1051,31 @@ int b_solve(MxRun &R, hipStream_t s, int slot) {
     return QP_OK;
 }
 
+// B_x of the slice from the output currents j_y, j_z (buffer of parity 1 - slot).
+int bx_solve(MxRun &R, hipStream_t s, int slot) {
+    Wake *w = R.w;
+    const Solver2d &S = w->S;
+    MixedState *m = R.m;
+    const qp_wake_params &p = w->p;
+    const int nw = R.nw;
+    SrcCurl src{m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], S.n1, S.n2, p.dy, p.dz};
+    SinkMember snk{w->bx, S.n1, S.n2};
+    if (S.wave) {
+        QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[0], 1, s, (const Ctl *)w->ctl, 1, nw));
+        QP_TRY(wlaunch_cols(S, 0.0, 1, nullptr, s, (const Ctl *)w->ctl, 1, nw));
+        QP_TRY(wlaunch_rows_inv(S, (const double2 *)S.spec[0], (const double2 *)S.spec[0], snk, 1, s,
+                                (const Ctl *)w->ctl, 1, nw));
+        return QP_OK;
+    }
+    MLAUNCH(k_rows_fwd<SrcCurl>, dim3(rows_blocks(S), nw), kSolverThreads, rows_lds(S), src, S.spec[0], S.spec[0],
+            S.n1, S.n2, S.nh, S.ax2.dev, S.rows_per_block, (const Ctl *)w->ctl, 1, 1, R.sstride);
+    QP_TRY(solve_cols_batch(S, 0.0, 1, nw, s, w->ctl, 1));
+    MLAUNCH(k_rows_inv<SinkMember>, dim3(rows_blocks(S), nw), kSolverThreads, rows_lds(S),
+            (const double2 *)S.spec[0], (const double2 *)S.spec[0], snk, S.n1, S.n2, S.nh, S.ax2.dev,
+            S.rows_per_block, (const Ctl *)w->ctl, 1, 1, R.sstride);
+    return QP_OK;
+}
+
 // Enqueue the work of slice i (everything between two output steps) for every member.
 int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     Wake *w = R.w;
@@ -1046,6 +1098,7 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
         prof_begin(w, "push_S");
         MLAUNCH(k_mx_S, pg, kPT, R.ldsS, m->P, R.g, si, (const double *)psi, w->rho, w->jx, w->chi, w->ctl);
         prof_end(w, "push_S");
+        if (p.longitudinal_b) QP_TRY(bx_solve(R, s, slot));
         if (i == 0) return QP_OK;
     } else {
         for (int it = 0; it < p.K; ++it) {
@@ -1061,6 +1114,7 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
             prof_end(w, "push_C");
             QP_TRY(b_solve(R, s, slot));
         }
+        if (p.longitudinal_b) QP_TRY(bx_solve(R, s, slot));
     }
     MLAUNCH(k_mx_out, dim3(R.gb, R.nw), 256, 0, w->ex, w->ey, w->ez, (const double *)w->psi_b[0],
             (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz,
@@ -1086,7 +1140,8 @@ int resort(MxRun &R, hipStream_t s) {
 // Stage the planes of one member (offset go) and call the batch callback.
 int deliver_member(Wake *w, batch_cb cb, void *user, int member, int slice, int kind, const double *psi,
                    const double *rho, const double *jx, const double *jy, const double *jz, const double *ex,
-                   const double *ey, const double *ez, const double *by, const double *bz, const double *chi) {
+                   const double *ey, const double *ez, const double *by, const double *bz, const double *chi,
+                   const double *bx = nullptr) {
     const size_t go = (size_t)member * w->nn();
     auto off = [&](const double *a) -> const double * { return a ? a + go : nullptr; };
     struct Fwd {
@@ -1099,7 +1154,7 @@ int deliver_member(Wake *w, batch_cb cb, void *user, int member, int slice, int 
         return ff->cb(ff->member, v, ff->user);
     };
     return deliver_slice(w, tramp, &f, slice, kind, off(psi), off(rho), off(jx), off(jy), off(jz), off(ex),
-                         off(ey), off(ez), off(by), off(bz), off(chi));
+                         off(ey), off(ez), off(by), off(bz), off(chi), off(bx));
 }
 
 }  // namespace
@@ -1131,7 +1186,7 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     // zero the fields (src/System_3d.cpp:265-316) and the control blocks
     const size_t plane = sizeof(double) * R.nn * nw;
     double *zero_list[] = {w->psi_b[0], w->psi_b[1], w->psim, w->by, w->bz, w->rho_ion, w->src, w->rho, w->jx,
-                           w->chi, w->ex, w->ey, w->ez, w->asq2};
+                           w->chi, w->ex, w->ey, w->ez, w->asq2, w->bx};
     for (double *a : zero_list) QP_HIP_CHECK(hipMemsetAsync(a, 0, plane, s));
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) QP_HIP_CHECK(hipMemsetAsync(m->jbuf[a][b], 0, plane, s));
@@ -1191,7 +1246,7 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
                 ++running;
                 QP_TRY(deliver_member(w, cb, user, mb, i, QP_SLICE_NORMAL, w->psi_b[slot], w->rho, w->jx,
                                       m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], w->ex, w->ey, w->ez, w->by,
-                                      w->bz, w->chi));
+                                      w->bz, w->chi, p.longitudinal_b ? w->bx : nullptr));
             }
             if (!running) break;
         }
@@ -1224,9 +1279,10 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
             const double *by0 = stopped ? w->zero2 : w->by;
             const double *bz0 = stopped ? w->zero2 : w->bz;
             const double *chi0 = stopped ? w->zero2 : w->chi;
+            const double *bx0 = p.longitudinal_b ? (stopped ? w->zero2 : w->bx) : nullptr;
             QP_TRY(deliver_member(w, cb, user, mb, 0, QP_SLICE_FINAL, w->psi_b[cslot], w->rho, w->jx,
                                   m->jbuf[1 - cslot][0], m->jbuf[1 - cslot][1], w->ex, w->ey, w->ez, by0, bz0,
-                                  chi0));
+                                  chi0, bx0));
         }
         if (stop_slices) stop_slices[mb] = stopped ? hc[mb].stop_slice : -1;
     }
