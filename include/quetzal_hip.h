@@ -160,6 +160,46 @@ int qp_wake_run_batch(qp_wake w, qp_batch_cb cb, void *user, int *stop_slices);
 int qp_wake_get_particles_at(qp_wake w, int member, size_t np, double *y, double *z, double *py,
                              double *pz);
 
+/* ---- Laser envelope (no reference counterpart) ------------------------------
+ * The reference keeps <a^2> fixed for the whole run (src/System_3d.cpp:805-816
+ * is its only writer) and deposits chi (src/System_3d.cpp:396,526) without
+ * reading it.  A qp_envelope holds the complex envelope a of the laser vector
+ * potential Re[a exp(i k0 (x - t))] on an nx x ny x nz grid (slice i at
+ * x = i dx, x toward the front; arrays [i][j][k] like a_sqr), k0 in units of
+ * the plasma wavenumber, and advances it in the time tau of the moving window:
+ *     lap_perp a + 2 d_tau (d_x + i k0) a = chi a
+ * (Crank-Nicolson in tau, periodic 5-point lap_perp, d_x differenced toward
+ * the front, slices swept from the front; chi minus its midrange on the slice
+ * couples the transverse modes in the implicit half and is solved by
+ * chi_iterations fixed-point corrections per slice, which contract for any
+ * dtau; 0 = that part of chi a at the old time only).
+ * Attached to a wake, the wake reads <a^2> = |a|^2/2 in place of its a_sqr or
+ * laser and stores the chi of every slice it computes into the envelope
+ * (slices after a stop keep their previous chi).  Single wakes only; the
+ * envelope must share the wake's context and grid, and outlive the attachment. */
+typedef struct qp_envelope_s *qp_envelope;
+int qp_envelope_create(qp_ctx ctx, int nx, int ny, int nz, double dx, double dy, double dz, double k0,
+                       qp_envelope *out);
+/* Fails with QP_ERR_STATE while a wake is attached. */
+int qp_envelope_destroy(qp_envelope e);
+/* Host arrays nx*ny*nz; im may be NULL (real envelope). */
+int qp_envelope_set(qp_envelope e, const double *re, const double *im);
+/* a = a0 exp(-4 sum (x-x0)^2/sigma^2): <a^2> equals qp_wake_set_laser_gaussian's. */
+int qp_envelope_set_gaussian(qp_envelope e, double a0, double xsigma, double ysigma, double zsigma,
+                             double x0, double y0, double z0);
+int qp_envelope_get(qp_envelope e, double *re, double *im);
+/* chi of every slice, nx*ny*nz (NULL = zero). */
+int qp_envelope_set_chi(qp_envelope e, const double *chi);
+int qp_envelope_get_chi(qp_envelope e, double *chi);
+/* <a^2> = |a|^2/2 as the attached wake reads it. */
+int qp_envelope_get_a_sqr(qp_envelope e, double *a_sqr);
+/* steps steps of dtau (one sweep each, captured in a hipGraph when use_graph),
+ * chi_iterations in [0, 16]; ms (may be NULL) gets the device time of the call. */
+int qp_envelope_advance(qp_envelope e, double dtau, int steps, int chi_iterations, int use_graph,
+                        double *ms);
+/* Attach e to the wake (NULL detaches). */
+int qp_wake_attach_envelope(qp_wake w, qp_envelope e);
+
 #ifdef __cplusplus
 }
 #endif

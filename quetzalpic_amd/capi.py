@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libquetzal_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "quetzal_hip.h")
 
 QP_OK, QP_STOPPED = 0, 1
+QP_ERR_ARG, QP_ERR_HIP, QP_ERR_STATE, QP_ERR_ALLOC = -1, -2, -3, -4
 QP_PREC_FP64, QP_PREC_MIXED = 0, 1
 QP_SLICE_NORMAL, QP_SLICE_STOPPED, QP_SLICE_FINAL = 0, 1, 2
 FIELD_NAMES = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
@@ -85,6 +86,17 @@ def load():
         "qp_wake_set_laser_gaussian_at": ([vp, ctypes.c_int] + [ctypes.c_double] * 7, ctypes.c_int),
         "qp_wake_run_batch": ([vp, BATCH_CB, vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "qp_wake_get_particles_at": ([vp, ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "qp_envelope_create": ([vp] + [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.POINTER(vp)],
+                               ctypes.c_int),
+        "qp_envelope_destroy": ([vp], ctypes.c_int),
+        "qp_envelope_set": ([vp, _dp, _dp], ctypes.c_int),
+        "qp_envelope_set_gaussian": ([vp] + [ctypes.c_double] * 7, ctypes.c_int),
+        "qp_envelope_get": ([vp, _dp, _dp], ctypes.c_int),
+        "qp_envelope_set_chi": ([vp, _dp], ctypes.c_int),
+        "qp_envelope_get_chi": ([vp, _dp], ctypes.c_int),
+        "qp_envelope_get_a_sqr": ([vp, _dp], ctypes.c_int),
+        "qp_envelope_advance": ([vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+        "qp_wake_attach_envelope": ([vp, vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -187,6 +199,7 @@ class Wake:
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
                                  precision, use_graph, resort_every, longitudinal_b)
         self.h = ctypes.c_void_p()
+        self._env = None
         _check(load().qp_wake_create(ctx.h, ctypes.byref(self.params), ctypes.byref(self.h)), "qp_wake_create")
         self.np = 0
         ctx._adopt(self)
@@ -267,9 +280,86 @@ class Wake:
                "qp_wake_get_particles")
         return y, z, py, pz
 
+    def attach_envelope(self, env):
+        """Read <a^2> = |a|^2/2 from `env` and store each slice's chi into it (None detaches)."""
+        _check(load().qp_wake_attach_envelope(self.h, env.h if env is not None else None),
+               "qp_wake_attach_envelope")
+        if self._env is not None:
+            self._env._wakes.discard(self)
+        self._env = env
+        if env is not None:
+            env._wakes.add(self)
+
     def close(self):
         if self.h:
             load().qp_wake_destroy(self.h)
+            self.h = ctypes.c_void_p()
+        if getattr(self, "_env", None) is not None:
+            self._env._wakes.discard(self)
+            self._env = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Envelope:
+    """qp_envelope: complex laser envelope a[i][j][k] on a wake's grid, advanced in tau
+    (no counterpart in the reference; the scheme is stated in include/quetzal_hip.h)."""
+
+    def __init__(self, ctx, nx, ny, nz, dx, dy, dz, k0):
+        self.h = ctypes.c_void_p()
+        self.shape = (nx, ny, nz)
+        self._wakes = weakref.WeakSet()
+        _check(load().qp_envelope_create(ctx.h, nx, ny, nz, dx, dy, dz, k0, ctypes.byref(self.h)),
+               "qp_envelope_create")
+        ctx._adopt(self)
+
+    def set(self, a):
+        a = np.asarray(a)
+        assert a.shape == self.shape
+        re = np.ascontiguousarray(a.real, dtype=np.float64)
+        im = np.ascontiguousarray(a.imag, dtype=np.float64) if np.iscomplexobj(a) else None
+        _check(load().qp_envelope_set(self.h, _ptr(re), _ptr(im)), "qp_envelope_set")
+
+    def set_gaussian(self, a0, xsigma, ysigma, zsigma, x0, y0, z0):
+        _check(load().qp_envelope_set_gaussian(self.h, a0, xsigma, ysigma, zsigma, x0, y0, z0),
+               "qp_envelope_set_gaussian")
+
+    def get(self):
+        re, im = np.zeros(self.shape), np.zeros(self.shape)
+        _check(load().qp_envelope_get(self.h, _ptr(re), _ptr(im)), "qp_envelope_get")
+        return re + 1j * im
+
+    def set_chi(self, chi):
+        c = None if chi is None else np.ascontiguousarray(chi, dtype=np.float64)
+        assert c is None or c.shape == self.shape
+        _check(load().qp_envelope_set_chi(self.h, _ptr(c)), "qp_envelope_set_chi")
+
+    def get_chi(self):
+        c = np.zeros(self.shape)
+        _check(load().qp_envelope_get_chi(self.h, _ptr(c)), "qp_envelope_get_chi")
+        return c
+
+    def get_a_sqr(self):
+        c = np.zeros(self.shape)
+        _check(load().qp_envelope_get_a_sqr(self.h, _ptr(c)), "qp_envelope_get_a_sqr")
+        return c
+
+    def advance(self, dtau, steps=1, chi_iterations=2, use_graph=1):
+        """Returns the device time of the call in ms."""
+        ms = ctypes.c_double()
+        _check(load().qp_envelope_advance(self.h, dtau, steps, chi_iterations, use_graph, ctypes.byref(ms)),
+               "qp_envelope_advance")
+        return ms.value
+
+    def close(self):
+        for w in list(self._wakes):
+            w.attach_envelope(None)
+        if self.h:
+            _check(load().qp_envelope_destroy(self.h), "qp_envelope_destroy")
             self.h = ctypes.c_void_p()
 
     def __del__(self):

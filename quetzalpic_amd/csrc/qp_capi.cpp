@@ -201,6 +201,7 @@ int qp_wake_destroy(qp_wake h) {
     if (!h) return QP_OK;
     Wake *w = &h->w;
     (void)hipStreamSynchronize(w->stream());
+    if (w->env) --w->env->attached;
     wake_free_particles(w);
     wake_destroy_mixed(w);
     w->S.destroy();
@@ -266,6 +267,7 @@ int qp_wake_set_a_sqr(qp_wake h, const double *a_sqr) {
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;
     ARG_CHECK(w->nw == 1, "qp_wake_set_a_sqr: 3D arrays are for single wakes (use the Gaussian laser)");
+    ++w->inputs_epoch;
     QP_TRY(upload3d(w, &w->a_sqr3, a_sqr));
     w->mem[0].laser_set = false;
     return QP_OK;
@@ -285,6 +287,7 @@ int qp_wake_set_laser_gaussian_at(qp_wake h, int member, double a0, double xsigm
     const double s8 = std::sqrt(8.0);
     w->mem[member].laser = LaserGauss{0.5 * a0 * a0, xsigma / s8, ysigma / s8, zsigma / s8, x0, y0, z0};
     w->mem[member].laser_set = true;
+    ++w->inputs_epoch;
     if (w->a_sqr3) (void)hipFree(w->a_sqr3);
     w->a_sqr3 = nullptr;
     return QP_OK;
@@ -294,8 +297,28 @@ int qp_wake_set_bunch_sources(qp_wake h, const double *rho_b, const double *jx_b
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;
     ARG_CHECK(w->nw == 1 || (!rho_b && !jx_b), "qp_wake_set_bunch_sources: bunch sources are for single wakes");
+    ++w->inputs_epoch;
     QP_TRY(upload3d(w, &w->rho_b3, rho_b));
     QP_TRY(upload3d(w, &w->jx_b3, jx_b));
+    return QP_OK;
+}
+
+int qp_wake_attach_envelope(qp_wake h, qp_envelope env) {
+    ARG_CHECK(h, "null wake");
+    Wake *w = &h->w;
+    if (env) {
+        const Envelope &e = env->e;
+        ARG_CHECK(w->nw == 1, "qp_wake_attach_envelope: envelopes are for single wakes");
+        ARG_CHECK(e.ctx == w->ctx, "qp_wake_attach_envelope: envelope and wake must share the context");
+        auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::fabs(b); };
+        ARG_CHECK(e.nx == w->p.nx && e.ny == w->p.ny && e.nz == w->p.nz && same(e.dx, w->p.dx) &&
+                      same(e.dy, w->p.dy) && same(e.dz, w->p.dz),
+                  "qp_wake_attach_envelope: envelope grid differs from the wake's");
+    }
+    if (w->env) --w->env->attached;
+    w->env = env ? &env->e : nullptr;
+    if (w->env) ++w->env->attached;
+    ++w->inputs_epoch;
     return QP_OK;
 }
 
@@ -383,7 +406,8 @@ int qp_wake_describe(qp_wake h, char *buf, size_t len) {
                     ", \"rows_per_block\": " + std::to_string(w->S.rows_per_block) +
                     ", \"cols_per_block\": " + std::to_string(w->S.cols_per_block) +
                     ", \"precision\": " + std::to_string(w->p.precision) +
-                    ", \"fft\": \"" + (w->S.wave ? "wave" : "generic") + "\"}";
+                    ", \"fft\": \"" + (w->S.wave ? "wave" : "generic") + "\"" +
+                    ", \"envelope\": " + (w->env ? "1" : "0") + "}";
     std::strncpy(buf, s.c_str(), len - 1);
     buf[len - 1] = 0;
     return QP_OK;

@@ -235,6 +235,10 @@ int solve_cols(const Solver2d &S, double D, int nspec, hipStream_t s, const Ctl 
 
 int solve_cols_forward_only(const Solver2d &S, hipStream_t s) { return cols_launch(S, 0.0, 1, s, nullptr, 0, 0); }
 
+int solve_cols_pass(const Solver2d &S, int nspec, int mode, hipStream_t s) {
+    return cols_launch(S, 0.0, nspec, s, nullptr, 0, mode);
+}
+
 int solve_cols_inverse_only(const Solver2d &S, hipStream_t s) { return cols_launch(S, 0.0, 1, s, nullptr, 0, 1); }
 
 int fourier_rows_inv(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s) {

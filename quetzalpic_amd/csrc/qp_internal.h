@@ -215,6 +215,8 @@ size_t cols_lds(const Solver2d &S);
 int rows_blocks(const Solver2d &S);
 int solve_cols_forward_only(const Solver2d &S, hipStream_t s);
 int solve_cols_inverse_only(const Solver2d &S, hipStream_t s);
+// Generic kernels only: column transforms of nspec spectra, mode 0 forward, 1 inverse.
+int solve_cols_pass(const Solver2d &S, int nspec, int mode, hipStream_t s);
 int solve_rows_inv_unscaled(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s);
 // Fourier2d transforms in FFTW's layout (generic kernels whatever S.wave says).
 int fourier_rows_fwd(const Solver2d &S, const double *src, double2 *spec, hipStream_t s);

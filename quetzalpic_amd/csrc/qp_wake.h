@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "qp_envelope.h"
 #include "qp_internal.h"
 
 namespace qp {
@@ -62,6 +63,13 @@ struct Wake {
     // mixed-path state (owned by qp_wake_mixed.hip)
     void *mixed = nullptr;
 
+    // Attached laser envelope (or null): <a^2> comes from its |a|^2/2 and the
+    // chi of every slice goes to its chi volume.
+    Envelope *env = nullptr;
+    // Bumped by every change of the driver inputs (a_sqr, laser, bunch sources,
+    // envelope): captured slice graphs hold their pointers and flags.
+    unsigned inputs_epoch = 0;
+
     // profiling hooks: when non-null, launches of `prof_class` are bracketed by events
     const char *prof_class = nullptr;
     std::vector<hipEvent_t> prof_ev;
@@ -75,6 +83,9 @@ struct Wake {
         return false;
     }
     hipStream_t stream() const { return ctx->stream; }
+    // 3D <a^2> the slice loop reads (null: Gaussian laser or none).
+    double *asq3() const { return env ? env->asq3 : a_sqr3; }
+    double *chi_store(int slice) const { return env ? env->chi3 + (size_t)slice * nn() : nullptr; }
 };
 
 int wake_alloc_common(Wake *w);

@@ -409,7 +409,10 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
         if (np) LAUNCH(k_dep_rho_jx_chi, pb, P, w->jx, w->rho, w->chi, np, g, ctl);
         prof_end(w, "push");
         // S6: by(i) = by(i+1) is the carried buffer
-        if (i == 0) break;
+        if (i == 0) {
+            if (w->env) LAUNCH(k_copy, gb, w->chi_store(i), (const double *)w->chi, nn, ctl);
+            break;
+        }
         for (int it = 0; it < p.K; ++it) {
             prof_begin(w, "push");
             if (np) LAUNCH(k_predictor, pb, P, asq, w->psi, w->by, w->bz, np, g, ctl);
@@ -439,6 +442,7 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
             LAUNCH(k_src_bz, gb, w->src, w->bz, w->jx, w->djy, n1, n2, p.D, p.dy, ctl);
             QP_TRY(solve_plain(w, w->src, w->bz, p.D, true, i));
         }
+        if (w->env) LAUNCH(k_copy, gb, w->chi_store(i), (const double *)w->chi, nn, ctl);
         prof_begin(w, "push");
         if (np) LAUNCH(k_commit, pb, P, np, ctl);
         if (np) LAUNCH(k_advance, pb, P, w->psim, np, g, ctl);
@@ -492,8 +496,8 @@ namespace qp {
 
 int wake_asq_slice(Wake *w, int i, const double **asq) {
     const qp_wake_params &p = w->p;
-    if (w->a_sqr3) {
-        *asq = w->a_sqr3 + (size_t)i * w->nn();
+    if (w->asq3()) {
+        *asq = w->asq3() + (size_t)i * w->nn();
         return QP_OK;
     }
     if (w->mem[0].laser_set) {

@@ -472,6 +472,14 @@ __global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi_b
     ez[t] = -(psi[tk] - psi[t]) / dz - (by ? by[t] : 0.0);
 }
 
+// chi of the slice into the attached envelope (single wake; the slice index
+// comes from the control block, so the kernel sits in the captured step).
+__global__ void k_mx_chi_store(const double *chi, double *chi3, Ctl *ctl, int slot, size_t nn) {
+    MX_SKIP(ctl);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nn) chi3[(size_t)ctl->slice_of[slot] * nn + t] = chi[t];
+}
+
 // ------------------------------------------------------- solver functors
 // All take (member, which, j, k): member planes sit at + member * nn.
 // psi source: jx_bunch(i) - rho_bunch(i) - rho_ion + deposits; clears the accumulator.
@@ -736,6 +744,7 @@ struct MixedState {
     double *jbuf[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][y/z], nw planes each
     std::vector<int> h_perm;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    unsigned graph_epoch = 0;  // Wake::inputs_epoch the graphs were captured at
     int ncb = 0;
     size_t cap = 0;
 };
@@ -1086,7 +1095,7 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     const bool s_phase = (i == 0) || (p.K == 0);
     const MxSliceIn &si = R.si[slot];
     double *psi = w->psi_b[slot];
-    const int laser = (w->any_laser() && !w->a_sqr3) ? 1 : 0;
+    const int laser = (w->any_laser() && !w->asq3()) ? 1 : 0;
     const dim3 pg(R.pb, R.nw);
     prof_begin(w, "push_A");
     MLAUNCH(k_mx_A, pg, kPT, 0, m->P, R.g, si, (const double *)w->psim, w->src, w->asq2,
@@ -1099,6 +1108,8 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
         MLAUNCH(k_mx_S, pg, kPT, R.ldsS, m->P, R.g, si, (const double *)psi, w->rho, w->jx, w->chi, w->ctl);
         prof_end(w, "push_S");
         if (p.longitudinal_b) QP_TRY(bx_solve(R, s, slot));
+        if (w->env)
+            MLAUNCH(k_mx_chi_store, dim3(R.gb), 256, 0, (const double *)w->chi, w->env->chi3, w->ctl, slot, R.nn);
         if (i == 0) return QP_OK;
     } else {
         for (int it = 0; it < p.K; ++it) {
@@ -1115,6 +1126,8 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
             QP_TRY(b_solve(R, s, slot));
         }
         if (p.longitudinal_b) QP_TRY(bx_solve(R, s, slot));
+        if (w->env)
+            MLAUNCH(k_mx_chi_store, dim3(R.gb), 256, 0, (const double *)w->chi, w->env->chi3, w->ctl, slot, R.nn);
     }
     MLAUNCH(k_mx_out, dim3(R.gb, R.nw), 256, 0, w->ex, w->ey, w->ez, (const double *)w->psi_b[0],
             (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz,
@@ -1174,7 +1187,14 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     R.gb = (int)((R.nn + 255) / 256);
     R.sstride = (size_t)w->S.n1 * w->S.nh;
     R.g = MxGeom{p.ny, p.nz, p.dy, p.dz, p.ly, p.lz, p.dx, 1.0 / p.dy, 1.0 / p.dz};
-    for (int sl = 0; sl < 2; ++sl) R.si[sl] = MxSliceIn{w->a_sqr3, w->asq2, w->rho_b3, w->jx_b3, sl};
+    for (int sl = 0; sl < 2; ++sl) R.si[sl] = MxSliceIn{w->asq3(), w->asq2, w->rho_b3, w->jx_b3, sl};
+    // captured slice steps hold the driver pointers and flags of their inputs
+    if (m->graph_epoch != w->inputs_epoch) {
+        for (auto &g : m->gexec)
+            if (g) (void)hipGraphExecDestroy(g);
+        m->gexec[0] = m->gexec[1] = nullptr;
+        m->graph_epoch = w->inputs_epoch;
+    }
     R.ldsC = sizeof(double) * 5 * kWinCap;
     R.ldsS = sizeof(double) * 3 * kWinCap;
 
