@@ -76,6 +76,48 @@ def read_traffic(kernel_cls):
                      vals["WRITE_KIB"] * vals.get("WRITE_FACTOR", 1.0))
 
 
+# Laser wavenumber of the envelope leg in units of k_p (not a reference input:
+# the reference has no envelope advance), its step and chi corrections.
+ENV_K0, ENV_DTAU, ENV_CHI_IT = 20.0, 2.0, 1
+
+
+def envelope_bytes_per_point(m):
+    """Algorithmic HBM bytes per transverse node and step of the envelope
+    advance with m chi corrections per slice.  One transform: row pass reads
+    a (re, im) 16 + chi 8 + d_{i+1} 16 (+ previous d_i 16 after the first) and
+    writes two half spectra 16; column pass reads and writes them 32; last row
+    pass reads them 16 and writes d_i 16.  The last transform of a slice also
+    reads and writes a (32); the <a^2> pass reads a 16 and writes 8."""
+    first = 16 + 8 + 16 + 16 + 32 + 16 + 16
+    return first + m * (first + 16) + 32 + 24
+
+
+def bench_envelope(capi, ctx, wl, steps, use_graph=1):
+    """Envelope advance on the workload's grid: chi from one mixed wake pass with
+    the attached Gaussian envelope, then `steps` steps (graph-replayed sweeps
+    unless use_graph = 0)."""
+    env = capi.Envelope(ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, ENV_K0)
+    env.set_gaussian(*wl.laser)
+    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=use_graph)
+    wk.attach_envelope(env)
+    wk.run()
+    wk.close()
+    env.advance(ENV_DTAU, steps=1, chi_iterations=ENV_CHI_IT, use_graph=use_graph)   # graph capture
+    ms = env.advance(ENV_DTAU, steps=steps, chi_iterations=ENV_CHI_IT, use_graph=use_graph) / steps
+    env.close()
+    pts = wl.nx * wl.ny * wl.nz
+    byts = envelope_bytes_per_point(ENV_CHI_IT) * pts
+    launches = wl.nx * 3 * (ENV_CHI_IT + 1) + 1
+    achieved = byts / (ms * 1e-3) / 1e9
+    res = {"k0": ENV_K0, "dtau": ENV_DTAU, "chi_iterations": ENV_CHI_IT, "graph": use_graph, "ms_per_step": ms,
+           "slices_per_s": wl.nx / (ms * 1e-3), "launches_per_step": launches,
+           "us_per_launch": 1e3 * ms / launches, "algorithmic_bytes_per_step": byts,
+           "achieved_GBs": achieved, "frac_hbm": achieved / HBM_PEAK_GBS}
+    log(f"[bench] envelope: {ms:.2f} ms/step ({launches} launches, {1e3 * ms / launches:.2f} us each), "
+        f"{achieved:.0f} GB/s algorithmic")
+    return res
+
+
 def calib_copy(mib):
     """Device-to-device copy of a known size through the HIP runtime (its copy
     kernel), so FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -252,6 +294,7 @@ def bench_c4(args, capi, workloads):
         "classes_alone": alone, "kernel_classes_largest": kc, "roofline": roofline,
         "err_vs_oracle_member": err, "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "envelope_advance": envelope,
     }
     print(json.dumps(line), flush=True)
 
@@ -271,6 +314,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel-class event pass")
     ap.add_argument("--c4-class", type=int, default=0,
                     help="C4 only: run the members of this grid size alone (profiling runs)")
+    ap.add_argument("--envelope-only", action="store_true",
+                    help="time only the laser-envelope advance on the workload's grid (profiling runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (counter calibration)")
     args = ap.parse_args()
@@ -288,6 +333,11 @@ def main():
     wl = workloads.get(args.workload)
     log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles}")
     ctx = capi.Context(0)
+    if args.envelope_only:
+        res = bench_envelope(capi, ctx, wl, args.steps, use_graph=0 if args.no_graph else 1)
+        print(json.dumps({"envelope": res, "config": wl.describe()}), flush=True)
+        ctx.close()
+        return
 
     # ---- timed region: the mixed fused path, hipGraph slice step
     wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=0 if args.no_graph else 1)
@@ -358,6 +408,9 @@ def main():
                             float(np.linalg.norm(d) / np.linalg.norm(fields64[f]))]
             log(f"[bench] mixed vs fp64 GPU (max, rms rel): {err64}")
     wk.close()
+
+    # ---- laser-envelope advance on the same grid (outside the timed region)
+    envelope = bench_envelope(capi, ctx, wl, max(args.steps, 3))
 
     cpu = None
     err_oracle = None
