@@ -200,6 +200,41 @@ int qp_envelope_advance(qp_envelope e, double dtau, int steps, int chi_iteration
 /* Attach e to the wake (NULL detaches). */
 int qp_wake_attach_envelope(qp_wake w, qp_envelope e);
 
+/* ---- Driver bunch on the device (System_3d's bunch, src/System_3d.cpp:843-946) --
+ * Replaces the host-side bunch of run() (src/System_3d.cpp:199-236):
+ * deposit_bunch_densities (:926-946), deposit_wakefield_to_bunch (:843-860)
+ * and advance_bunch_particles (:862-924) without its QED photon emission.
+ * Particles are FP64 SoA: position, momentum, gamma, q = species charge * n,
+ * cmr = species charge-to-mass ratio, photon = species is_photon.  Attached to
+ * a single wake, qp_wake_run streams the bunch: slice i's rho_b, jx_b are
+ * deposited into two planes right before the slice (in place of the 3D
+ * arrays of qp_wake_set_bunch_sources) and, once slice i's fields are final,
+ * the particles whose x-cell is i gather them with those of slice i + 1
+ * (slice 0 after output_step(0)).  Particles whose x-cell is outside
+ * [0, nx - 1) neither deposit nor gather (the reference's x cut,
+ * src/array_utils.cpp:299-301).  Getters return particles in upload order. */
+typedef struct qp_bunch_s *qp_bunch;
+/* field_schwinger: the Schwinger field in plasma units (the chi normalisation). */
+int qp_bunch_create(qp_ctx ctx, int nx, int ny, int nz, double dx, double dy, double dz,
+                    double field_schwinger, qp_bunch *out);
+/* Fails with QP_ERR_STATE while a wake is attached. */
+int qp_bunch_destroy(qp_bunch b);
+int qp_bunch_set_particles(qp_bunch b, size_t np, const double *x, const double *y, const double *z,
+                           const double *px, const double *py, const double *pz, const double *gamma,
+                           const double *q, const double *cmr, const int *photon);
+int qp_bunch_size(qp_bunch b, size_t *np);
+int qp_bunch_get_particles(qp_bunch b, size_t np, double *x, double *y, double *z, double *px,
+                           double *py, double *pz, double *gamma);
+/* Fields and chi gathered by the last qp_wake_run (zero where not gathered). */
+int qp_bunch_get_fields(qp_bunch b, size_t np, double *ex, double *ey, double *ez, double *by,
+                        double *bz, double *chi);
+/* advance_bunch_particles without QED emission, then re-bucketing by x-cell. */
+int qp_bunch_push(qp_bunch b, double dt);
+/* deposit_bunch_densities into nx*ny*nz host arrays (the 3D output). */
+int qp_bunch_deposit(qp_bunch b, double *rho_b, double *jx_b);
+/* Attach b to the wake (NULL detaches). */
+int qp_wake_attach_bunch(qp_wake w, qp_bunch b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,14 @@ def load(variant="det"):
     lib.qpo_wake_run.restype = ctypes.c_int
     lib.qpo_wake_run_partial.argtypes = run_args + [ctypes.c_int]
     lib.qpo_wake_run_partial.restype = ctypes.c_int
+    lib.qpo_deposit3d.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 4 + [_dp]
+    lib.qpo_gather3d.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 3 + [_dp]
+    lib.qpo_gather3d.restype = ctypes.c_double
+    lib.qpo_bunch_deposit.argtypes = [ctypes.POINTER(QpoGrid), ctypes.c_size_t] + [_dp] * 8
+    lib.qpo_bunch_fields.argtypes = ([ctypes.POINTER(QpoGrid), ctypes.c_size_t] + [_dp] * 12 +
+                                     [ctypes.c_double] + [_dp] * 6)
+    _ip = ctypes.POINTER(ctypes.c_int)
+    lib.qpo_bunch_push.argtypes = [ctypes.c_size_t] + [_dp] * 8 + [_ip] + [_dp] * 5 + [ctypes.c_double]
     lib.qpo_num_threads.restype = ctypes.c_int
     lib.qpo_splitmix64.argtypes = [ctypes.c_ulonglong] * 3
     lib.qpo_splitmix64.restype = ctypes.c_ulonglong
@@ -179,3 +187,55 @@ def wake_run(g, y, z, w, a_sqr, py=None, pz=None, rho_b=None, jx_b=None, n_slice
                                   _ptr(a_sqr), _ptr(rho_b), _ptr(jx_b), ctypes.byref(f), ctypes.byref(stop),
                                   int(n_slices))
     return out, st, stop.value
+
+
+# ------------------------------------------------------------ driver bunch
+BUNCH_STATE = ("x", "y", "z", "px", "py", "pz", "gamma")
+BUNCH_FIELDS = ("ex", "ey", "ez", "by", "bz", "chi")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def deposit3d(g, x, y, z, v, a3):
+    load().qpo_deposit3d(ctypes.byref(g), x, y, z, v, _ptr(a3))
+
+
+def gather3d(g, x, y, z, a3):
+    return load().qpo_gather3d(ctypes.byref(g), x, y, z, _ptr(_f64(a3)))
+
+
+def bunch_deposit(g, b):
+    """deposit_bunch_densities (src/System_3d.cpp:926-946) of the bunch dict b
+    (x, y, z, px, gamma, q arrays); returns (rho_b, jx_b) [nx, ny, nz]."""
+    rho = np.zeros((g.nx, g.ny, g.nz))
+    jx = np.zeros((g.nx, g.ny, g.nz))
+    a = [_f64(b[k]) for k in ("x", "y", "z", "px", "gamma", "q")]
+    load().qpo_bunch_deposit(ctypes.byref(g), len(a[0]), *[_ptr(v) for v in a], _ptr(rho), _ptr(jx))
+    return rho, jx
+
+
+def bunch_fields(g, b, f3, field_schwinger):
+    """deposit_wakefield_to_bunch (src/System_3d.cpp:843-860): dict of the five
+    gathered fields and chi, from the 3D arrays f3['ex'..'bz']."""
+    a = [_f64(b[k]) for k in BUNCH_STATE]
+    fl = [_f64(f3[k]) for k in ("ex", "ey", "ez", "by", "bz")]
+    n = len(a[0])
+    out = {k: np.zeros(n) for k in BUNCH_FIELDS}
+    load().qpo_bunch_fields(ctypes.byref(g), n, *[_ptr(v) for v in a], *[_ptr(v) for v in fl],
+                            field_schwinger, *[_ptr(out[k]) for k in BUNCH_FIELDS])
+    return out
+
+
+def bunch_push(b, fields, dt):
+    """advance_bunch_particles without QED (src/System_3d.cpp:862-890); returns
+    a new dict with the advanced state (cmr, photon, q carried over)."""
+    nb = {k: _f64(b[k]).copy() for k in BUNCH_STATE}
+    ph = np.ascontiguousarray(b["photon"], dtype=np.int32)
+    fl = [_f64(fields[k]) for k in ("ex", "ey", "ez", "by", "bz")]
+    load().qpo_bunch_push(len(nb["x"]), *[_ptr(nb[k]) for k in BUNCH_STATE], _ptr(_f64(b["cmr"])),
+                          ph.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), *[_ptr(v) for v in fl], dt)
+    for k in ("q", "cmr", "photon"):
+        nb[k] = b[k]
+    return nb

@@ -839,3 +839,121 @@ int qpo_wake_run(const qpo_grid *g, size_t np, const double *y, const double *z,
                  qpo_fields *out, int *stop_slice) {
     return qpo_wake_run_partial(g, np, y, z, py, pz, w, a_sqr, rho_b, jx_b, out, stop_slice, 0);
 }
+
+
+/* ------------------------------------------------------------------------
+ * Driver bunch: 3D CIC on the nodes, field gather, explicit push.
+ * ---------------------------------------------------------------------- */
+static int wrap_index(int j, int n) {
+    j %= n;
+    return j < 0 ? j + n : j;
+}
+
+/* src/array_utils.cpp:290-338 */
+void qpo_deposit3d(const qpo_grid *g, double x, double y, double z, double v, double *a3) {
+    const int i1 = (int)floor(x / g->dx), i2 = i1 + 1;
+    const double x_frac = x / g->dx - i1;
+    if (i1 < 0 || i2 >= g->nx) return;
+    const int n1 = g->ny, n2 = g->nz;
+    int j1 = (int)floor(y / g->dy);
+    const double y_frac = y / g->dy - j1;
+    const int j2 = wrap_index(j1 + 1, n1);
+    j1 = wrap_index(j1, n1);
+    int k1 = (int)floor(z / g->dz);
+    const double z_frac = z / g->dz - k1;
+    const int k2 = wrap_index(k1 + 1, n2);
+    k1 = wrap_index(k1, n2);
+    const size_t nn = (size_t)n1 * n2;
+    double *a = a3 + (size_t)i1 * nn, *b = a3 + (size_t)i2 * nn;
+    a[(size_t)j1 * n2 + k1] += v * (1 - x_frac) * (1 - y_frac) * (1 - z_frac);
+    a[(size_t)j2 * n2 + k1] += v * (1 - x_frac) * y_frac * (1 - z_frac);
+    a[(size_t)j1 * n2 + k2] += v * (1 - x_frac) * (1 - y_frac) * z_frac;
+    a[(size_t)j2 * n2 + k2] += v * (1 - x_frac) * y_frac * z_frac;
+    b[(size_t)j1 * n2 + k1] += v * x_frac * (1 - y_frac) * (1 - z_frac);
+    b[(size_t)j2 * n2 + k1] += v * x_frac * y_frac * (1 - z_frac);
+    b[(size_t)j1 * n2 + k2] += v * x_frac * (1 - y_frac) * z_frac;
+    b[(size_t)j2 * n2 + k2] += v * x_frac * y_frac * z_frac;
+}
+
+/* src/array_utils.cpp:340-374 */
+double qpo_gather3d(const qpo_grid *g, double x, double y, double z, const double *a3) {
+    const int i1 = (int)floor(x / g->dx), i2 = i1 + 1;
+    const double x_frac = x / g->dx - i1;
+    if (i1 < 0 || i2 >= g->nx) return 0.0;
+    const int n1 = g->ny, n2 = g->nz;
+    int j1 = (int)floor(y / g->dy);
+    const double y_frac = y / g->dy - j1;
+    const int j2 = wrap_index(j1 + 1, n1);
+    j1 = wrap_index(j1, n1);
+    int k1 = (int)floor(z / g->dz);
+    const double z_frac = z / g->dz - k1;
+    const int k2 = wrap_index(k1 + 1, n2);
+    k1 = wrap_index(k1, n2);
+    const size_t nn = (size_t)n1 * n2;
+    const double *a = a3 + (size_t)i1 * nn, *b = a3 + (size_t)i2 * nn;
+    return a[(size_t)j1 * n2 + k1] * (1 - x_frac) * (1 - y_frac) * (1 - z_frac) +
+           a[(size_t)j2 * n2 + k1] * (1 - x_frac) * y_frac * (1 - z_frac) +
+           a[(size_t)j1 * n2 + k2] * (1 - x_frac) * (1 - y_frac) * z_frac +
+           a[(size_t)j2 * n2 + k2] * (1 - x_frac) * y_frac * z_frac +
+           b[(size_t)j1 * n2 + k1] * x_frac * (1 - y_frac) * (1 - z_frac) +
+           b[(size_t)j2 * n2 + k1] * x_frac * y_frac * (1 - z_frac) +
+           b[(size_t)j1 * n2 + k2] * x_frac * (1 - y_frac) * z_frac +
+           b[(size_t)j2 * n2 + k2] * x_frac * y_frac * z_frac;
+}
+
+/* src/System_3d.cpp:926-946 */
+void qpo_bunch_deposit(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
+                       const double *px, const double *gamma, const double *q, double *rho_b, double *jx_b) {
+    const size_t vol = (size_t)g->nx * g->ny * g->nz;
+    memset(rho_b, 0, sizeof(double) * vol);
+    memset(jx_b, 0, sizeof(double) * vol);
+    for (size_t p = 0; p < np; ++p) {
+        qpo_deposit3d(g, x[p], y[p], z[p], q[p], rho_b);
+        qpo_deposit3d(g, x[p], y[p], z[p], q[p] * px[p] / gamma[p], jx_b);
+    }
+}
+
+/* src/System_3d.cpp:843-860 */
+void qpo_bunch_fields(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
+                      const double *px, const double *py, const double *pz, const double *gamma,
+                      const double *ex3, const double *ey3, const double *ez3, const double *by3,
+                      const double *bz3, double field_schwinger, double *ex, double *ey, double *ez,
+                      double *by, double *bz, double *chi) {
+    for (size_t p = 0; p < np; ++p) {
+        ex[p] = qpo_gather3d(g, x[p], y[p], z[p], ex3);
+        ey[p] = qpo_gather3d(g, x[p], y[p], z[p], ey3);
+        ez[p] = qpo_gather3d(g, x[p], y[p], z[p], ez3);
+        by[p] = qpo_gather3d(g, x[p], y[p], z[p], by3);
+        bz[p] = qpo_gather3d(g, x[p], y[p], z[p], bz3);
+        const double fx = gamma[p] * ex[p] + py[p] * bz[p] - pz[p] * by[p];
+        const double fy = gamma[p] * ey[p] - px[p] * bz[p];
+        const double fz = gamma[p] * ez[p] + px[p] * by[p];
+        const double fl = px[p] * ex[p] + py[p] * ey[p] + pz[p] * ez[p];
+        chi[p] = sqrt(fx * fx + fy * fy + fz * fz - fl * fl) / field_schwinger;
+    }
+}
+
+/* src/System_3d.cpp:862-890 (the QED branch :892-921 is not restated) */
+void qpo_bunch_push(size_t np, double *x, double *y, double *z, double *px, double *py, double *pz,
+                    double *gamma, const double *cmr, const int *photon, const double *ex,
+                    const double *ey, const double *ez, const double *by, const double *bz, double dt) {
+    for (size_t p = 0; p < np; ++p) {
+        if (photon[p]) {
+            x[p] += dt * (px[p] / gamma[p] - 1);
+            y[p] += dt * py[p] / gamma[p];
+            z[p] += dt * pz[p] / gamma[p];
+            gamma[p] = sqrt(px[p] * px[p] + py[p] * py[p] + pz[p] * pz[p]);
+            continue;
+        }
+        const double fx = cmr[p] * (ex[p] + (py[p] * bz[p] - pz[p] * by[p]) / gamma[p]);
+        const double fy = cmr[p] * (ey[p] - px[p] * bz[p] / gamma[p]);
+        const double fz = cmr[p] * (ez[p] + px[p] * by[p] / gamma[p]);
+        x[p] += dt * (px[p] / gamma[p] - 1);
+        y[p] += dt * py[p] / gamma[p];
+        z[p] += dt * pz[p] / gamma[p];
+        px[p] += dt * fx;
+        py[p] += dt * fy;
+        pz[p] += dt * fz;
+        gamma[p] = sqrt(1 + px[p] * px[p] + py[p] * py[p] + pz[p] * pz[p]);
+    }
+}

@@ -134,6 +134,33 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np, const double *y, const do
                          const double *a_sqr, const double *rho_b, const double *jx_b,
                          qpo_fields *out, int *stop_slice, int n_slices);
 
+/* ---- Driver bunch (src/System_3d.cpp:843-946) ----------------------------
+ * 3D arrays are nx*ny*nz row-major [i][j][k] on the nodes (origin 0).
+ * Index wrap: the reference computes j2 = (j1+1) % n before wrapping a
+ * negative j1 once (src/array_utils.cpp:303-314); the oracle wraps both
+ * indices fully, which is the same wherever the reference stays in bounds. */
+/* deposit(x,y,z,v,array3d&), src/array_utils.cpp:290-338: nothing when the
+ * x-cell i1 = floor(x/dx) is outside [0, nx-1). */
+void qpo_deposit3d(const qpo_grid *g, double x, double y, double z, double v, double *a3);
+/* array_to_particle(x,y,z,array3d&), src/array_utils.cpp:340-374 (0 outside). */
+double qpo_gather3d(const qpo_grid *g, double x, double y, double z, const double *a3);
+/* deposit_bunch_densities, src/System_3d.cpp:926-946: rho_b = sum q S,
+ * jx_b = sum q px/gamma S (both zeroed first); q = species charge * n. */
+void qpo_bunch_deposit(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
+                       const double *px, const double *gamma, const double *q, double *rho_b, double *jx_b);
+/* deposit_wakefield_to_bunch, src/System_3d.cpp:843-860: the five fields at
+ * each particle and the QED parameter chi (field_schwinger in plasma units). */
+void qpo_bunch_fields(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
+                      const double *px, const double *py, const double *pz, const double *gamma,
+                      const double *ex3, const double *ey3, const double *ez3, const double *by3,
+                      const double *bz3, double field_schwinger, double *ex, double *ey, double *ez,
+                      double *by, double *bz, double *chi);
+/* advance_bunch_particles without QED emission, src/System_3d.cpp:862-890:
+ * photons stream (gamma = |p|), charges take the explicit Lorentz step. */
+void qpo_bunch_push(size_t np, double *x, double *y, double *z, double *px, double *py, double *pz,
+                    double *gamma, const double *cmr, const int *photon, const double *ex,
+                    const double *ey, const double *ez, const double *by, const double *bz, double dt);
+
 /* Number of OpenMP threads the build uses (1 without OpenMP). */
 int qpo_num_threads(void);
 

@@ -97,6 +97,15 @@ def load():
         "qp_envelope_get_a_sqr": ([vp, _dp], ctypes.c_int),
         "qp_envelope_advance": ([vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
         "qp_wake_attach_envelope": ([vp, vp], ctypes.c_int),
+        "qp_bunch_create": ([vp] + [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.POINTER(vp)], ctypes.c_int),
+        "qp_bunch_destroy": ([vp], ctypes.c_int),
+        "qp_bunch_set_particles": ([vp, ctypes.c_size_t] + [_dp] * 9 + [ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "qp_bunch_size": ([vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+        "qp_bunch_get_particles": ([vp, ctypes.c_size_t] + [_dp] * 7, ctypes.c_int),
+        "qp_bunch_get_fields": ([vp, ctypes.c_size_t] + [_dp] * 6, ctypes.c_int),
+        "qp_bunch_push": ([vp, ctypes.c_double], ctypes.c_int),
+        "qp_bunch_deposit": ([vp, _dp, _dp], ctypes.c_int),
+        "qp_wake_attach_bunch": ([vp, vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -200,6 +209,7 @@ class Wake:
                                  precision, use_graph, resort_every, longitudinal_b)
         self.h = ctypes.c_void_p()
         self._env = None
+        self._bunch = None
         _check(load().qp_wake_create(ctx.h, ctypes.byref(self.params), ctypes.byref(self.h)), "qp_wake_create")
         self.np = 0
         ctx._adopt(self)
@@ -290,6 +300,15 @@ class Wake:
         if env is not None:
             env._wakes.add(self)
 
+    def attach_bunch(self, bunch):
+        """Stream `bunch` through the slice loop: per-slice sources and field gathers (None detaches)."""
+        _check(load().qp_wake_attach_bunch(self.h, bunch.h if bunch is not None else None), "qp_wake_attach_bunch")
+        if self._bunch is not None:
+            self._bunch._wakes.discard(self)
+        self._bunch = bunch
+        if bunch is not None:
+            bunch._wakes.add(self)
+
     def close(self):
         if self.h:
             load().qp_wake_destroy(self.h)
@@ -297,6 +316,9 @@ class Wake:
         if getattr(self, "_env", None) is not None:
             self._env._wakes.discard(self)
             self._env = None
+        if getattr(self, "_bunch", None) is not None:
+            self._bunch._wakes.discard(self)
+            self._bunch = None
 
     def __del__(self):
         try:
@@ -360,6 +382,66 @@ class Envelope:
             w.attach_envelope(None)
         if self.h:
             _check(load().qp_envelope_destroy(self.h), "qp_envelope_destroy")
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Bunch:
+    """qp_bunch: driver-bunch particles on the device (deposit, field gather and
+    push of System_3d's bunch, src/System_3d.cpp:843-946, without QED emission)."""
+
+    STATE = ("x", "y", "z", "px", "py", "pz", "gamma")
+    FIELDS = ("ex", "ey", "ez", "by", "bz", "chi")
+
+    def __init__(self, ctx, nx, ny, nz, dx, dy, dz, field_schwinger=1.0):
+        self.h = ctypes.c_void_p()
+        self.shape = (nx, ny, nz)
+        self.np = 0
+        self._wakes = weakref.WeakSet()
+        _check(load().qp_bunch_create(ctx.h, nx, ny, nz, dx, dy, dz, field_schwinger, ctypes.byref(self.h)),
+               "qp_bunch_create")
+        ctx._adopt(self)
+
+    def set_particles(self, b):
+        """b: dict with x, y, z, px, py, pz, gamma, q, cmr (float) and photon (int) arrays."""
+        a = [np.ascontiguousarray(b[k], dtype=np.float64) for k in self.STATE + ("q", "cmr")]
+        ph = np.ascontiguousarray(b["photon"], dtype=np.int32)
+        n = len(a[0])
+        _check(load().qp_bunch_set_particles(self.h, n, *[_ptr(v) for v in a],
+                                             ph.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+               "qp_bunch_set_particles")
+        self.np = n
+
+    def get_particles(self):
+        out = {k: np.zeros(self.np) for k in self.STATE}
+        _check(load().qp_bunch_get_particles(self.h, self.np, *[_ptr(out[k]) for k in self.STATE]),
+               "qp_bunch_get_particles")
+        return out
+
+    def get_fields(self):
+        out = {k: np.zeros(self.np) for k in self.FIELDS}
+        _check(load().qp_bunch_get_fields(self.h, self.np, *[_ptr(out[k]) for k in self.FIELDS]),
+               "qp_bunch_get_fields")
+        return out
+
+    def push(self, dt):
+        _check(load().qp_bunch_push(self.h, dt), "qp_bunch_push")
+
+    def deposit(self):
+        rho, jx = np.zeros(self.shape), np.zeros(self.shape)
+        _check(load().qp_bunch_deposit(self.h, _ptr(rho), _ptr(jx)), "qp_bunch_deposit")
+        return rho, jx
+
+    def close(self):
+        for w in list(self._wakes):
+            w.attach_bunch(None)
+        if self.h:
+            _check(load().qp_bunch_destroy(self.h), "qp_bunch_destroy")
             self.h = ctypes.c_void_p()
 
     def __del__(self):

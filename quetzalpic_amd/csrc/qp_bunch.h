@@ -1,0 +1,58 @@
+// qp_bunch.h -- device state of a driver bunch (host side) and the per-slice
+// launches the wake loops make for an attached bunch.
+//
+// The reference deposits the whole bunch into 3D arrays before the pass
+// (deposit_bunch_densities, src/System_3d.cpp:926-946) and gathers the 3D
+// field arrays after it (deposit_wakefield_to_bunch, :843-860).  Here the
+// particles are kept sorted by x-cell i1 = floor(x/dx) (bucket nx-1 holds the
+// particles whose cell is outside [0, nx-1)), and the wake streams both: at
+// the start of slice i the buckets i and i-1 deposit rho_b(i), jx_b(i) into
+// two planes; at its end bucket i gathers from the fields of slice i and the
+// saved fields of slice i+1.  No 3D array is needed.
+#pragma once
+
+#include "qp_internal.h"
+
+namespace qp {
+
+// What the kernels see (by value).
+struct BunchDev {
+    double *x, *y, *z, *px, *py, *pz, *gamma, *q, *cmr;  // q = species charge * n
+    int *photon, *id;                                      // id = index at upload
+    double *ex, *ey, *ez, *by, *bz, *chi;                  // gathered by the last pass
+    const int *start;  // nx + 1 entries: bucket b holds [start[b], start[b+1])
+    int np, nx, n1, n2;
+    double dx, dy, dz, fs;  // fs: Schwinger field in plasma units
+};
+
+struct Bunch {
+    qp_ctx_s *ctx = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    double dx = 0, dy = 0, dz = 0, fs = 1;
+    size_t np = 0, cap = 0;
+    BunchDev cur{}, tmp{};       // sorted arrays and the re-bucketing scratch
+    int *start = nullptr, *count = nullptr, *cursor = nullptr, *key = nullptr;
+    double *planes = nullptr;    // rho_b, jx_b planes, then 2 slots x 5 saved field planes
+    int attached = 0;            // wakes streaming this bunch
+    unsigned epoch = 0;          // bumped when the particle arrays move (captured slice steps hold them)
+
+    size_t nn() const { return (size_t)ny * nz; }
+    double *rho_plane() const { return planes; }
+    double *jx_plane() const { return planes + nn(); }
+    double *saved() const { return planes + 2 * nn(); }
+    hipStream_t stream() const { return ctx->stream; }
+};
+
+// Launches for an attached bunch, enqueued on s.  slot >= 0: the slice index
+// comes from ctl->slice_of[slot] (captured slice steps); otherwise `slice`.
+// par: parity of the slice (selects the saved-field slot).
+int bunch_clear_fields(Bunch *b, hipStream_t s);
+int bunch_slice_sources(Bunch *b, const Ctl *ctl, int slot, int slice, hipStream_t s);
+int bunch_slice_gather(Bunch *b, const double *ex, const double *ey, const double *ez, const double *by,
+                       const double *bz, const Ctl *ctl, int slot, int slice, int par, hipStream_t s);
+
+}  // namespace qp
+
+struct qp_bunch_s {
+    qp::Bunch b;
+};

@@ -1,0 +1,578 @@
+// qp_bunch.hip -- the driver bunch on the device: streamed per-slice sources
+// and field gathers for the wake loops, the explicit push and re-bucketing
+// (src/System_3d.cpp:843-946, CIC of src/array_utils.cpp:290-374).
+//
+// Arithmetic follows the reference's operation order and this file is built
+// without FMA contraction, like the plain FP64 path, so that deposits (up to
+// the order of the atomic sums), gathers and the push agree with the oracle at
+// rounding level.  Index wrap: the reference computes j2 = (j1+1) % n before
+// wrapping a negative j1 once (src/array_utils.cpp:303-314); both indices are
+// wrapped fully here, which is the same wherever the reference stays in
+// bounds.  Not restated on the device: QED photon emission (:892-921; random,
+// from the unseeded rand()).
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "qp_bunch.h"
+
+namespace qp {
+
+namespace {
+
+constexpr int kBT = 256;
+constexpr int kSliceBlocks = 128;  // grid of the per-slice launches (bucket sizes vary per slice)
+
+inline int bblk(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + kBT - 1) / kBT, 65535 * 4)); }
+
+__device__ __forceinline__ int wrap_idx(int j, int n) {
+    j %= n;
+    return j < 0 ? j + n : j;
+}
+
+__device__ __forceinline__ int slice_index(const Ctl *ctl, int slot, int slice) {
+    return slot >= 0 ? ctl->slice_of[slot] : slice;
+}
+
+// Transverse CIC indices and fractions on the nodes (origin 0).
+struct Tw {
+    int j1, j2, k1, k2;
+    double yf, zf;
+};
+__device__ __forceinline__ Tw tw_of(double y, double z, const BunchDev &B) {
+    Tw t;
+    const int j = (int)floor(y / B.dy);
+    t.yf = y / B.dy - j;
+    t.j2 = wrap_idx(j + 1, B.n1);
+    t.j1 = wrap_idx(j, B.n1);
+    const int k = (int)floor(z / B.dz);
+    t.zf = z / B.dz - k;
+    t.k2 = wrap_idx(k + 1, B.n2);
+    t.k1 = wrap_idx(k, B.n2);
+    return t;
+}
+
+// Bucket of a particle: its x-cell i1 = floor(x/dx) when the reference
+// deposits / gathers it (0 <= i1 and i1 + 1 < nx), else nx - 1.
+__device__ __forceinline__ int bucket_of(double x, const BunchDev &B) {
+    const double t = floor(x / B.dx);
+    return (t >= 0.0 && t < (double)(B.nx - 1)) ? (int)t : B.nx - 1;
+}
+
+__global__ void k_bunch_zero(double *planes, size_t n, const Ctl *ctl) {
+    if (ctl && ctl->stop_acc) return;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        planes[t] = 0.0;
+}
+
+// rho_b(i), jx_b(i) from bucket i (slice i is their i1: weight 1 - x_frac)
+// and bucket i-1 (slice i is their i2: weight x_frac).
+__global__ __launch_bounds__(kBT) void k_bunch_src(BunchDev B, double *rho, double *jx, const Ctl *ctl, int slot,
+                                                   int slice) {
+    if (ctl && ctl->stop_acc) return;
+    const int i = slice_index(ctl, slot, slice);
+    if (i < 0 || i >= B.nx) return;
+    int a0 = 0, na = 0, b0 = 0, nb = 0;
+    if (i <= B.nx - 2) {
+        a0 = B.start[i];
+        na = B.start[i + 1] - a0;
+    }
+    if (i >= 1) {
+        b0 = B.start[i - 1];
+        nb = B.start[i] - b0;
+    }
+    const int n2 = B.n2;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na + nb; t += gridDim.x * blockDim.x) {
+        const bool front = t < na;
+        const int p = front ? a0 + t : b0 + (t - na);
+        const double x = B.x[p];
+        const int i1 = (int)floor(x / B.dx);
+        const double x_frac = x / B.dx - i1;
+        const Tw w = tw_of(B.y[p], B.z[p], B);
+        const double v = B.q[p];
+        const double vj = v * B.px[p] / B.gamma[p];
+        const double wx = front ? (1 - x_frac) : x_frac;
+        const size_t c11 = (size_t)w.j1 * n2 + w.k1, c21 = (size_t)w.j2 * n2 + w.k1;
+        const size_t c12 = (size_t)w.j1 * n2 + w.k2, c22 = (size_t)w.j2 * n2 + w.k2;
+        unsafeAtomicAdd(&rho[c11], v * wx * (1 - w.yf) * (1 - w.zf));
+        unsafeAtomicAdd(&rho[c21], v * wx * w.yf * (1 - w.zf));
+        unsafeAtomicAdd(&rho[c12], v * wx * (1 - w.yf) * w.zf);
+        unsafeAtomicAdd(&rho[c22], v * wx * w.yf * w.zf);
+        unsafeAtomicAdd(&jx[c11], vj * wx * (1 - w.yf) * (1 - w.zf));
+        unsafeAtomicAdd(&jx[c21], vj * wx * w.yf * (1 - w.zf));
+        unsafeAtomicAdd(&jx[c12], vj * wx * (1 - w.yf) * w.zf);
+        unsafeAtomicAdd(&jx[c22], vj * wx * w.yf * w.zf);
+    }
+}
+
+// array_to_particle (src/array_utils.cpp:340-374) with slice i1 in a, i2 in b.
+__device__ __forceinline__ double gather8(const double *a, const double *b, const Tw &w, int n2, double x_frac) {
+    const size_t c11 = (size_t)w.j1 * n2 + w.k1, c21 = (size_t)w.j2 * n2 + w.k1;
+    const size_t c12 = (size_t)w.j1 * n2 + w.k2, c22 = (size_t)w.j2 * n2 + w.k2;
+    return a[c11] * (1 - x_frac) * (1 - w.yf) * (1 - w.zf) + a[c21] * (1 - x_frac) * w.yf * (1 - w.zf) +
+           a[c12] * (1 - x_frac) * (1 - w.yf) * w.zf + a[c22] * (1 - x_frac) * w.yf * w.zf +
+           b[c11] * x_frac * (1 - w.yf) * (1 - w.zf) + b[c21] * x_frac * w.yf * (1 - w.zf) +
+           b[c12] * x_frac * (1 - w.yf) * w.zf + b[c22] * x_frac * w.yf * w.zf;
+}
+
+struct Planes5 {
+    const double *f[5];  // ex, ey, ez, by, bz of the slice
+};
+
+// Bucket i gathers from slice i (the current planes) and slice i+1 (saved slot
+// 1 - par), and derives chi (src/System_3d.cpp:843-860); then the current
+// planes are saved into slot par for slice i-1.
+__global__ __launch_bounds__(kBT) void k_bunch_gather(BunchDev B, Planes5 cur, double *saved, size_t nn,
+                                                      const Ctl *ctl, int slot, int slice, int par) {
+    if (ctl && ctl->stop_acc) return;
+    const int i = slice_index(ctl, slot, slice);
+    if (i < 0 || i >= B.nx) return;
+    const double *prev = saved + (size_t)(1 - par) * 5 * nn;
+    double *keep = saved + (size_t)par * 5 * nn;
+    const int stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= B.nx - 2) {
+        const int a0 = B.start[i], na = B.start[i + 1] - a0;
+        const int n2 = B.n2;
+        for (int t = t0; t < na; t += stride) {
+            const int p = a0 + t;
+            const double x = B.x[p];
+            const int i1 = (int)floor(x / B.dx);
+            const double x_frac = x / B.dx - i1;
+            const Tw w = tw_of(B.y[p], B.z[p], B);
+            const double ex = gather8(cur.f[0], prev, w, n2, x_frac);
+            const double ey = gather8(cur.f[1], prev + nn, w, n2, x_frac);
+            const double ez = gather8(cur.f[2], prev + 2 * nn, w, n2, x_frac);
+            const double by = gather8(cur.f[3], prev + 3 * nn, w, n2, x_frac);
+            const double bz = gather8(cur.f[4], prev + 4 * nn, w, n2, x_frac);
+            const double px = B.px[p], py = B.py[p], pz = B.pz[p], g = B.gamma[p];
+            const double fx = g * ex + py * bz - pz * by;
+            const double fy = g * ey - px * bz;
+            const double fz = g * ez + px * by;
+            const double fl = px * ex + py * ey + pz * ez;
+            B.ex[p] = ex;
+            B.ey[p] = ey;
+            B.ez[p] = ez;
+            B.by[p] = by;
+            B.bz[p] = bz;
+            B.chi[p] = sqrt(fx * fx + fy * fy + fz * fz - fl * fl) / B.fs;
+        }
+    }
+    for (int f = 0; f < 5; ++f)
+        for (size_t t = (size_t)t0; t < nn; t += (size_t)stride) keep[(size_t)f * nn + t] = cur.f[f][t];
+}
+
+__global__ void k_bunch_clear(BunchDev B) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= B.np) return;
+    B.ex[p] = B.ey[p] = B.ez[p] = B.by[p] = B.bz[p] = B.chi[p] = 0.0;
+}
+
+// advance_bunch_particles without QED emission (src/System_3d.cpp:862-890):
+// reads S, writes the advanced particle to D (same slot) with its bucket.
+__global__ void k_bunch_push(BunchDev S, BunchDev D, double dt, int *key, unsigned *count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S.np) return;
+    double x = S.x[p], y = S.y[p], z = S.z[p], px = S.px[p], py = S.py[p], pz = S.pz[p], g = S.gamma[p];
+    const double ex = S.ex[p], ey = S.ey[p], ez = S.ez[p], by = S.by[p], bz = S.bz[p];
+    if (S.photon[p]) {
+        x += dt * (px / g - 1);
+        y += dt * py / g;
+        z += dt * pz / g;
+        g = sqrt(px * px + py * py + pz * pz);
+    } else {
+        const double cmr = S.cmr[p];
+        const double fx = cmr * (ex + (py * bz - pz * by) / g);
+        const double fy = cmr * (ey - px * bz / g);
+        const double fz = cmr * (ez + px * by / g);
+        x += dt * (px / g - 1);
+        y += dt * py / g;
+        z += dt * pz / g;
+        px += dt * fx;
+        py += dt * fy;
+        pz += dt * fz;
+        g = sqrt(1 + px * px + py * py + pz * pz);
+    }
+    D.x[p] = x;
+    D.y[p] = y;
+    D.z[p] = z;
+    D.px[p] = px;
+    D.py[p] = py;
+    D.pz[p] = pz;
+    D.gamma[p] = g;
+    D.q[p] = S.q[p];
+    D.cmr[p] = S.cmr[p];
+    D.photon[p] = S.photon[p];
+    D.id[p] = S.id[p];
+    D.ex[p] = ex;
+    D.ey[p] = ey;
+    D.ez[p] = ez;
+    D.by[p] = by;
+    D.bz[p] = bz;
+    D.chi[p] = S.chi[p];
+    const int b = bucket_of(x, D);
+    key[p] = b;
+    atomicAdd(&count[b], 1u);
+}
+
+__global__ void k_bunch_key(BunchDev S, int *key, unsigned *count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S.np) return;
+    const int b = bucket_of(S.x[p], S);
+    key[p] = b;
+    atomicAdd(&count[b], 1u);
+}
+
+// Exclusive scan of nb bucket counts into start[0..nb] (one block of 1024).
+__global__ void k_bunch_scan(const unsigned *count, int *start, unsigned *cursor, int nb) {
+    __shared__ unsigned part[1024];
+    const int per = (nb + blockDim.x - 1) / blockDim.x;
+    const int b0 = threadIdx.x * per;
+    unsigned s = 0;
+    for (int i = b0; i < b0 + per && i < nb; ++i) s += count[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned run = 0;
+        for (int i = 0; i < (int)blockDim.x; ++i) {
+            const unsigned v = part[i];
+            part[i] = run;
+            run += v;
+        }
+        start[nb] = (int)run;
+    }
+    __syncthreads();
+    unsigned run = part[threadIdx.x];
+    for (int i = b0; i < b0 + per && i < nb; ++i) {
+        start[i] = (int)run;
+        cursor[i] = 0;
+        run += count[i];
+    }
+}
+
+__global__ void k_bunch_scatter(BunchDev S, BunchDev D, const int *key, unsigned *cursor) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S.np) return;
+    const int b = key[p];
+    const unsigned d = (unsigned)D.start[b] + atomicAdd(&cursor[b], 1u);
+    if (d >= (unsigned)S.np) return;
+    D.x[d] = S.x[p];
+    D.y[d] = S.y[p];
+    D.z[d] = S.z[p];
+    D.px[d] = S.px[p];
+    D.py[d] = S.py[p];
+    D.pz[d] = S.pz[p];
+    D.gamma[d] = S.gamma[p];
+    D.q[d] = S.q[p];
+    D.cmr[d] = S.cmr[p];
+    D.photon[d] = S.photon[p];
+    D.id[d] = S.id[p];
+    D.ex[d] = S.ex[p];
+    D.ey[d] = S.ey[p];
+    D.ez[d] = S.ez[p];
+    D.by[d] = S.by[p];
+    D.bz[d] = S.bz[p];
+    D.chi[d] = S.chi[p];
+}
+
+// Full 3D deposit (deposit_bunch_densities, src/System_3d.cpp:926-946) into
+// nx*ny*nz arrays: the reference's 3D output of rho_bunch and jx_bunch.
+__global__ void k_bunch_dep3d(BunchDev B, double *rho3, double *jx3) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= B.np) return;
+    const double x = B.x[p];
+    if (bucket_of(x, B) == B.nx - 1) return;
+    const int i1 = (int)floor(x / B.dx), i2 = i1 + 1;
+    const double x_frac = x / B.dx - i1;
+    const Tw w = tw_of(B.y[p], B.z[p], B);
+    const int n2 = B.n2;
+    const size_t nn = (size_t)B.n1 * n2;
+    const size_t c11 = (size_t)w.j1 * n2 + w.k1, c21 = (size_t)w.j2 * n2 + w.k1;
+    const size_t c12 = (size_t)w.j1 * n2 + w.k2, c22 = (size_t)w.j2 * n2 + w.k2;
+    const double vals[2] = {B.q[p], B.q[p] * B.px[p] / B.gamma[p]};
+    double *arr[2] = {rho3, jx3};
+    for (int a = 0; a < 2; ++a) {
+        const double v = vals[a];
+        double *lo = arr[a] + (size_t)i1 * nn, *hi = arr[a] + (size_t)i2 * nn;
+        unsafeAtomicAdd(&lo[c11], v * (1 - x_frac) * (1 - w.yf) * (1 - w.zf));
+        unsafeAtomicAdd(&lo[c21], v * (1 - x_frac) * w.yf * (1 - w.zf));
+        unsafeAtomicAdd(&lo[c12], v * (1 - x_frac) * (1 - w.yf) * w.zf);
+        unsafeAtomicAdd(&lo[c22], v * (1 - x_frac) * w.yf * w.zf);
+        unsafeAtomicAdd(&hi[c11], v * x_frac * (1 - w.yf) * (1 - w.zf));
+        unsafeAtomicAdd(&hi[c21], v * x_frac * w.yf * (1 - w.zf));
+        unsafeAtomicAdd(&hi[c12], v * x_frac * (1 - w.yf) * w.zf);
+        unsafeAtomicAdd(&hi[c22], v * x_frac * w.yf * w.zf);
+    }
+}
+
+#define BLAUNCH(kern, grid, block, ...)                                       \
+    do {                                                                      \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__); \
+        QP_HIP_CHECK(hipGetLastError());                                      \
+    } while (0)
+
+void free_set(BunchDev &d) {
+    void *arrs[] = {d.x, d.y, d.z, d.px, d.py, d.pz, d.gamma, d.q, d.cmr, d.photon, d.id,
+                    d.ex, d.ey, d.ez, d.by, d.bz, d.chi};
+    for (void *a : arrs)
+        if (a) (void)hipFree(a);
+    const BunchDev keep = d;
+    d = BunchDev{};
+    d.start = keep.start;
+    d.nx = keep.nx;
+    d.n1 = keep.n1;
+    d.n2 = keep.n2;
+    d.dx = keep.dx;
+    d.dy = keep.dy;
+    d.dz = keep.dz;
+    d.fs = keep.fs;
+}
+
+int alloc_set(BunchDev &d, size_t n) {
+    double **dbl[] = {&d.x, &d.y, &d.z, &d.px, &d.py, &d.pz, &d.gamma, &d.q, &d.cmr,
+                      &d.ex, &d.ey, &d.ez, &d.by, &d.bz, &d.chi};
+    for (double **a : dbl) QP_HIP_CHECK(hipMalloc(a, sizeof(double) * n));
+    QP_HIP_CHECK(hipMalloc(&d.photon, sizeof(int) * n));
+    QP_HIP_CHECK(hipMalloc(&d.id, sizeof(int) * n));
+    return QP_OK;
+}
+
+// Sort set S (np particles) into b->cur by bucket; key and count already
+// filled by the caller's kernel on S.
+int bucket_sort(Bunch *b, BunchDev &S, hipStream_t s) {
+    BLAUNCH(k_bunch_scan, 1, 1024, (const unsigned *)b->count, b->start, (unsigned *)b->cursor, b->nx);
+    BLAUNCH(k_bunch_scatter, bblk(b->np), kBT, S, b->cur, (const int *)b->key, (unsigned *)b->cursor);
+    return QP_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ wake hooks
+int bunch_clear_fields(Bunch *b, hipStream_t s) {
+    if (b->np) BLAUNCH(k_bunch_clear, bblk(b->np), kBT, b->cur);
+    return QP_OK;
+}
+
+int bunch_slice_sources(Bunch *b, const Ctl *ctl, int slot, int slice, hipStream_t s) {
+    BLAUNCH(k_bunch_zero, bblk(2 * b->nn()), kBT, b->planes, 2 * b->nn(), ctl);
+    BLAUNCH(k_bunch_src, kSliceBlocks, kBT, b->cur, b->rho_plane(), b->jx_plane(), ctl, slot, slice);
+    return QP_OK;
+}
+
+int bunch_slice_gather(Bunch *b, const double *ex, const double *ey, const double *ez, const double *by,
+                       const double *bz, const Ctl *ctl, int slot, int slice, int par, hipStream_t s) {
+    Planes5 cur{{ex, ey, ez, by, bz}};
+    BLAUNCH(k_bunch_gather, kSliceBlocks, kBT, b->cur, cur, b->saved(), b->nn(), ctl, slot, slice, par);
+    return QP_OK;
+}
+
+}  // namespace qp
+
+using namespace qp;
+
+#define BARG_CHECK(cond, msg)        \
+    do {                             \
+        if (!(cond)) {               \
+            set_error(msg);          \
+            return QP_ERR_ARG;       \
+        }                            \
+    } while (0)
+
+namespace {
+
+// Host copy of a device array in upload order (the device keeps the bunch sorted).
+template <class T>
+int unsort(const Bunch *b, const T *dev, T *host, const std::vector<int> &id) {
+    if (!host) return QP_OK;
+    std::vector<T> tmp(b->np);
+    QP_HIP_CHECK(hipMemcpy(tmp.data(), dev, sizeof(T) * b->np, hipMemcpyDeviceToHost));
+    for (size_t p = 0; p < b->np; ++p) host[id[p]] = tmp[p];
+    return QP_OK;
+}
+
+int host_ids(const Bunch *b, std::vector<int> &id) {
+    id.resize(b->np);
+    QP_HIP_CHECK(hipStreamSynchronize(b->stream()));
+    if (b->np) QP_HIP_CHECK(hipMemcpy(id.data(), b->cur.id, sizeof(int) * b->np, hipMemcpyDeviceToHost));
+    return QP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qp_bunch_create(qp_ctx ctx, int nx, int ny, int nz, double dx, double dy, double dz, double field_schwinger,
+                    qp_bunch *out) {
+    BARG_CHECK(ctx && out, "qp_bunch_create: null argument");
+    BARG_CHECK(nx >= 2 && ny >= 2 && nz >= 2, "qp_bunch_create: need nx, ny, nz >= 2");
+    BARG_CHECK(dx > 0 && dy > 0 && dz > 0 && field_schwinger > 0, "qp_bunch_create: steps and field must be > 0");
+    *out = nullptr;
+    qp_bunch h = new qp_bunch_s();
+    Bunch *b = &h->b;
+    b->ctx = ctx;
+    b->nx = nx;
+    b->ny = ny;
+    b->nz = nz;
+    b->dx = dx;
+    b->dy = dy;
+    b->dz = dz;
+    b->fs = field_schwinger;
+    for (BunchDev *d : {&b->cur, &b->tmp}) {
+        d->nx = nx;
+        d->n1 = ny;
+        d->n2 = nz;
+        d->dx = dx;
+        d->dy = dy;
+        d->dz = dz;
+        d->fs = field_schwinger;
+    }
+    auto fail = [&](int st) {
+        qp_bunch_destroy(h);
+        return st;
+    };
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(QP_ERR_HIP);
+    if (hipMalloc(&b->start, sizeof(int) * (nx + 1)) != hipSuccess ||
+        hipMalloc(&b->count, sizeof(unsigned) * nx) != hipSuccess ||
+        hipMalloc(&b->cursor, sizeof(unsigned) * nx) != hipSuccess ||
+        hipMalloc(&b->planes, sizeof(double) * 12 * b->nn()) != hipSuccess) {
+        set_error("qp_bunch_create: device allocation failed");
+        return fail(QP_ERR_ALLOC);
+    }
+    hipStream_t s = b->stream();
+    if (hipMemsetAsync(b->start, 0, sizeof(int) * (nx + 1), s) != hipSuccess ||
+        hipMemsetAsync(b->planes, 0, sizeof(double) * 12 * b->nn(), s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return fail(QP_ERR_HIP);
+    b->cur.start = b->start;
+    b->tmp.start = b->start;
+    *out = h;
+    return QP_OK;
+}
+
+int qp_bunch_destroy(qp_bunch h) {
+    if (!h) return QP_OK;
+    Bunch *b = &h->b;
+    if (b->attached) {
+        set_error("qp_bunch_destroy: a wake still streams this bunch (detach or destroy it first)");
+        return QP_ERR_STATE;
+    }
+    if (b->ctx) (void)hipStreamSynchronize(b->stream());
+    free_set(b->cur);
+    free_set(b->tmp);
+    for (void *p : {(void *)b->start, (void *)b->count, (void *)b->cursor, (void *)b->key, (void *)b->planes})
+        if (p) (void)hipFree(p);
+    delete h;
+    return QP_OK;
+}
+
+int qp_bunch_set_particles(qp_bunch h, size_t np, const double *x, const double *y, const double *z,
+                           const double *px, const double *py, const double *pz, const double *gamma,
+                           const double *q, const double *cmr, const int *photon) {
+    BARG_CHECK(h, "null bunch");
+    BARG_CHECK(np == 0 || (x && y && z && px && py && pz && gamma && q && cmr && photon),
+               "qp_bunch_set_particles: every array is required");
+    BARG_CHECK(np < (size_t)INT_MAX / 2, "qp_bunch_set_particles: too many particles");
+    Bunch *b = &h->b;
+    hipStream_t s = b->stream();
+    QP_HIP_CHECK(hipStreamSynchronize(s));
+    if (np > b->cap) {
+        free_set(b->cur);
+        free_set(b->tmp);
+        if (b->key) (void)hipFree(b->key);
+        b->key = nullptr;
+        b->cap = 0;
+        QP_TRY(alloc_set(b->cur, np));
+        QP_TRY(alloc_set(b->tmp, np));
+        QP_HIP_CHECK(hipMalloc(&b->key, sizeof(int) * np));
+        b->cap = np;
+        ++b->epoch;
+    }
+    b->np = np;
+    b->cur.np = b->tmp.np = (int)np;
+    if (np == 0) {
+        QP_HIP_CHECK(hipMemsetAsync(b->start, 0, sizeof(int) * (b->nx + 1), s));
+        QP_HIP_CHECK(hipStreamSynchronize(s));
+        return QP_OK;
+    }
+    BunchDev &T = b->tmp;
+    const double *src[] = {x, y, z, px, py, pz, gamma, q, cmr};
+    double *dst[] = {T.x, T.y, T.z, T.px, T.py, T.pz, T.gamma, T.q, T.cmr};
+    for (int a = 0; a < 9; ++a) QP_HIP_CHECK(hipMemcpy(dst[a], src[a], sizeof(double) * np, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(T.photon, photon, sizeof(int) * np, hipMemcpyHostToDevice));
+    std::vector<int> ids(np);
+    for (size_t p = 0; p < np; ++p) ids[p] = (int)p;
+    QP_HIP_CHECK(hipMemcpy(T.id, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    for (double *f : {T.ex, T.ey, T.ez, T.by, T.bz, T.chi}) QP_HIP_CHECK(hipMemsetAsync(f, 0, sizeof(double) * np, s));
+    QP_HIP_CHECK(hipMemsetAsync(b->count, 0, sizeof(unsigned) * b->nx, s));
+    BLAUNCH(k_bunch_key, bblk(np), kBT, T, b->key, (unsigned *)b->count);
+    QP_TRY(bucket_sort(b, T, s));
+    QP_HIP_CHECK(hipStreamSynchronize(s));
+    return QP_OK;
+}
+
+int qp_bunch_get_particles(qp_bunch h, size_t np, double *x, double *y, double *z, double *px, double *py,
+                           double *pz, double *gamma) {
+    BARG_CHECK(h, "null bunch");
+    Bunch *b = &h->b;
+    BARG_CHECK(np == b->np, "qp_bunch_get_particles: count mismatch");
+    std::vector<int> id;
+    QP_TRY(host_ids(b, id));
+    const double *src[] = {b->cur.x, b->cur.y, b->cur.z, b->cur.px, b->cur.py, b->cur.pz, b->cur.gamma};
+    double *dst[] = {x, y, z, px, py, pz, gamma};
+    for (int a = 0; a < 7; ++a) QP_TRY(unsort(b, src[a], dst[a], id));
+    return QP_OK;
+}
+
+int qp_bunch_get_fields(qp_bunch h, size_t np, double *ex, double *ey, double *ez, double *by, double *bz,
+                        double *chi) {
+    BARG_CHECK(h, "null bunch");
+    Bunch *b = &h->b;
+    BARG_CHECK(np == b->np, "qp_bunch_get_fields: count mismatch");
+    std::vector<int> id;
+    QP_TRY(host_ids(b, id));
+    const double *src[] = {b->cur.ex, b->cur.ey, b->cur.ez, b->cur.by, b->cur.bz, b->cur.chi};
+    double *dst[] = {ex, ey, ez, by, bz, chi};
+    for (int a = 0; a < 6; ++a) QP_TRY(unsort(b, src[a], dst[a], id));
+    return QP_OK;
+}
+
+int qp_bunch_push(qp_bunch h, double dt) {
+    BARG_CHECK(h, "null bunch");
+    BARG_CHECK(std::isfinite(dt), "qp_bunch_push: dt must be finite");
+    Bunch *b = &h->b;
+    if (!b->np) return QP_OK;
+    hipStream_t s = b->stream();
+    QP_HIP_CHECK(hipMemsetAsync(b->count, 0, sizeof(unsigned) * b->nx, s));
+    BLAUNCH(k_bunch_push, bblk(b->np), kBT, b->cur, b->tmp, dt, b->key, (unsigned *)b->count);
+    QP_TRY(bucket_sort(b, b->tmp, s));
+    QP_HIP_CHECK(hipStreamSynchronize(s));
+    return QP_OK;
+}
+
+int qp_bunch_deposit(qp_bunch h, double *rho_b, double *jx_b) {
+    BARG_CHECK(h && rho_b && jx_b, "qp_bunch_deposit: null argument");
+    Bunch *b = &h->b;
+    hipStream_t s = b->stream();
+    const size_t vol = (size_t)b->nx * b->nn();
+    double *d3 = nullptr;
+    QP_HIP_CHECK(hipMalloc(&d3, sizeof(double) * 2 * vol));
+    struct Free {
+        double *p;
+        ~Free() { (void)hipFree(p); }
+    } guard{d3};
+    QP_HIP_CHECK(hipMemsetAsync(d3, 0, sizeof(double) * 2 * vol, s));
+    if (b->np) BLAUNCH(k_bunch_dep3d, bblk(b->np), kBT, b->cur, d3, d3 + vol);
+    QP_HIP_CHECK(hipStreamSynchronize(s));
+    QP_HIP_CHECK(hipMemcpy(rho_b, d3, sizeof(double) * vol, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(jx_b, d3 + vol, sizeof(double) * vol, hipMemcpyDeviceToHost));
+    return QP_OK;
+}
+
+int qp_bunch_size(qp_bunch h, size_t *np) {
+    BARG_CHECK(h && np, "null argument");
+    *np = h->b.np;
+    return QP_OK;
+}
+
+}  // extern "C"

@@ -202,6 +202,7 @@ int qp_wake_destroy(qp_wake h) {
     Wake *w = &h->w;
     (void)hipStreamSynchronize(w->stream());
     if (w->env) --w->env->attached;
+    if (w->bunch) --w->bunch->attached;
     wake_free_particles(w);
     wake_destroy_mixed(w);
     w->S.destroy();
@@ -300,6 +301,28 @@ int qp_wake_set_bunch_sources(qp_wake h, const double *rho_b, const double *jx_b
     ++w->inputs_epoch;
     QP_TRY(upload3d(w, &w->rho_b3, rho_b));
     QP_TRY(upload3d(w, &w->jx_b3, jx_b));
+    return QP_OK;
+}
+
+int qp_wake_attach_bunch(qp_wake h, qp_bunch bunch) {
+    ARG_CHECK(h, "null wake");
+    Wake *w = &h->w;
+    if (bunch) {
+        const Bunch &b = bunch->b;
+        ARG_CHECK(w->nw == 1, "qp_wake_attach_bunch: bunches are for single wakes");
+        ARG_CHECK(b.ctx == w->ctx, "qp_wake_attach_bunch: bunch and wake must share the context");
+        auto same = [](double a, double c) { return std::fabs(a - c) <= 1e-12 * std::fabs(c); };
+        ARG_CHECK(b.nx == w->p.nx && b.ny == w->p.ny && b.nz == w->p.nz && same(b.dx, w->p.dx) &&
+                      same(b.dy, w->p.dy) && same(b.dz, w->p.dz),
+                  "qp_wake_attach_bunch: bunch grid differs from the wake's");
+    }
+    if (w->bunch) --w->bunch->attached;
+    w->bunch = bunch ? &bunch->b : nullptr;
+    if (w->bunch) {
+        ++w->bunch->attached;
+        w->bunch_epoch = w->bunch->epoch;
+    }
+    ++w->inputs_epoch;
     return QP_OK;
 }
 
@@ -407,7 +430,7 @@ int qp_wake_describe(qp_wake h, char *buf, size_t len) {
                     ", \"cols_per_block\": " + std::to_string(w->S.cols_per_block) +
                     ", \"precision\": " + std::to_string(w->p.precision) +
                     ", \"fft\": \"" + (w->S.wave ? "wave" : "generic") + "\"" +
-                    ", \"envelope\": " + (w->env ? "1" : "0") + "}";
+                    ", \"envelope\": " + (w->env ? "1" : "0") + ", \"bunch\": " + (w->bunch ? "1" : "0") + "}";
     std::strncpy(buf, s.c_str(), len - 1);
     buf[len - 1] = 0;
     return QP_OK;
@@ -470,8 +493,14 @@ void wake_free_particles(Wake *w) {
     }
 }
 
-const double *wake_rho_b(Wake *w, int i) { return w->rho_b3 ? w->rho_b3 + (size_t)i * w->nn() : nullptr; }
-const double *wake_jx_b(Wake *w, int i) { return w->jx_b3 ? w->jx_b3 + (size_t)i * w->nn() : nullptr; }
+const double *wake_rho_b(Wake *w, int i) {
+    const double *b = w->rho_b_src();
+    return b ? b + (size_t)i * w->bsrc_stride() : nullptr;
+}
+const double *wake_jx_b(Wake *w, int i) {
+    const double *b = w->jx_b_src();
+    return b ? b + (size_t)i * w->bsrc_stride() : nullptr;
+}
 
 void prof_begin(Wake *w, const char *cls) {
     if (!w->prof_class || std::strcmp(cls, w->prof_class) != 0) return;

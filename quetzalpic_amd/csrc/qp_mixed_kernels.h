@@ -52,7 +52,8 @@ __device__ __forceinline__ int mx_np(const MxPart &P, int wk) { return P.npw ? P
 struct MxSliceIn {
     const double *a_sqr3;   // nx*n1*n2 or null
     const double *asq2;     // laser plane buffer (used when a_sqr3 is null)
-    const double *rho_b3, *jx_b3;
+    const double *rho_b3, *jx_b3;  // bunch sources: slice i at + i * bstride (0: one streamed plane)
+    size_t bstride;
     int slot;               // which ctl->slice_of[] entry holds this step's slice
 };
 

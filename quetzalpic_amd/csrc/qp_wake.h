@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "qp_bunch.h"
 #include "qp_envelope.h"
 #include "qp_internal.h"
 
@@ -66,6 +67,10 @@ struct Wake {
     // Attached laser envelope (or null): <a^2> comes from its |a|^2/2 and the
     // chi of every slice goes to its chi volume.
     Envelope *env = nullptr;
+    // Attached driver bunch (or null): its sources are deposited per slice into
+    // two planes (in place of rho_b3/jx_b3) and it gathers the fields per slice.
+    Bunch *bunch = nullptr;
+    unsigned bunch_epoch = 0;  // Bunch::epoch the captured slice steps saw
     // Bumped by every change of the driver inputs (a_sqr, laser, bunch sources,
     // envelope): captured slice graphs hold their pointers and flags.
     unsigned inputs_epoch = 0;
@@ -86,6 +91,11 @@ struct Wake {
     // 3D <a^2> the slice loop reads (null: Gaussian laser or none).
     double *asq3() const { return env ? env->asq3 : a_sqr3; }
     double *chi_store(int slice) const { return env ? env->chi3 + (size_t)slice * nn() : nullptr; }
+    // Bunch sources the slice loop reads: slice i at base + i * stride (stride 0
+    // for the per-slice planes of an attached bunch).
+    const double *rho_b_src() const { return bunch ? bunch->rho_plane() : rho_b3; }
+    const double *jx_b_src() const { return bunch ? bunch->jx_plane() : jx_b3; }
+    size_t bsrc_stride() const { return bunch ? 0 : nn(); }
 };
 
 int wake_alloc_common(Wake *w);

@@ -385,11 +385,13 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
     for (double *a : zero_list) QP_HIP_CHECK(hipMemsetAsync(a, 0, sizeof(double) * nn, s));
     QP_HIP_CHECK(hipEventRecord(w->ev0, s));
     if (np) LAUNCH(k_dep_ion, pb, P.y, P.z, P.n, w->rho_ion, np, g);
+    if (w->bunch) QP_TRY(bunch_clear_fields(w->bunch, s));
 
     int delivered_stop = 0;
     for (int i = p.nx - 1; i >= 0; --i) {
         const double *asq;
         QP_TRY(wake_asq_slice(w, i, &asq));
+        if (w->bunch) QP_TRY(bunch_slice_sources(w->bunch, ctl, -1, i, s));
         const double *rb = wake_rho_b(w, i), *jb = wake_jx_b(w, i);
         // S1-S3
         LAUNCH(k_copy, gb, w->psi_prev, w->psi, nn, ctl);
@@ -449,6 +451,7 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
         prof_end(w, "push");
         LAUNCH(k_output, gb, w->ex, w->ey, w->ez, w->psi, w->psi_prev, w->by, w->bz, n1, n2, p.dx, p.dy, p.dz,
                i < p.nx - 1 ? 1 : 0, ctl, 0);
+        if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, i, i & 1, s));
         if (cb) {
             QP_HIP_CHECK(hipStreamSynchronize(s));
             Ctl hc;
@@ -468,6 +471,8 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
     // output_step(0) after the loop (src/System_3d.cpp:622)
     LAUNCH(k_output, gb, w->ex, w->ey, w->ez, w->psi, w->psi_prev, w->by, w->bz, n1, n2, p.dx, p.dy, p.dz,
            p.nx > 1 ? 1 : 0, ctl, 1);
+    // bunch bucket 0: slice 0's fields are final only after output_step(0)
+    if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, 0, 0, s));
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;

@@ -63,8 +63,8 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
     const LaserGauss L = Ls[wk];
     const int slice = mx_slice(ctl, si);
     if (laser || init_rho) {
-        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * nn : nullptr;
-        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * nn : nullptr;
+        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
+        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
         for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
             if (laser) {
                 const int j = (int)(t / n2), k = (int)(t % n2);
@@ -166,8 +166,8 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     jzn += go;
     const int slice = mx_slice(ctl, si);
     {  // rho = rho_bunch(i), jx = jx_bunch(i), chi = 0, jy* = jz* = 0 before the C phase (:473-478, :511-518)
-        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * nn : nullptr;
-        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * nn : nullptr;
+        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
+        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
         for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
             rho[t] = rb ? rb[t] : 0.0;
             jx[t] = jb ? jb[t] : 0.0;
@@ -485,15 +485,16 @@ __global__ void k_mx_chi_store(const double *chi, double *chi3, Ctl *ctl, int sl
 // psi source: jx_bunch(i) - rho_bunch(i) - rho_ion + deposits; clears the accumulator.
 struct SrcPsi {
     double *acc;
-    const double *rho_ion, *rho_b3, *jx_b3;
+    const double *rho_ion, *rho_b3, *jx_b3;  // bunch sources: slice i at + i * bstride
+    size_t bstride;
     const Ctl *ctl;
     int slot, n2;
     size_t nn;
     __device__ double operator()(int wk, int, int j, int k) const {
         const size_t t = (size_t)j * n2 + k, tw = (size_t)wk * nn + t;
         const int slice = ctl[wk].slice_of[slot];
-        const double jb = jx_b3 ? jx_b3[(size_t)slice * nn + t] : 0.0;
-        const double rb = rho_b3 ? rho_b3[(size_t)slice * nn + t] : 0.0;
+        const double jb = jx_b3 ? jx_b3[(size_t)slice * bstride + t] : 0.0;
+        const double rb = rho_b3 ? rho_b3[(size_t)slice * bstride + t] : 0.0;
         const double v = acc[tw];
         acc[tw] = 0.0;
         return (jb - rb - rho_ion[tw]) + v;
@@ -990,7 +991,7 @@ int psi_solve(MxRun &R, hipStream_t s, int slot, double *dst) {
     const Solver2d &S = w->S;
     MixedState *m = R.m;
     const int nw = R.nw;
-    SrcPsi src{w->src, w->rho_ion, w->rho_b3, w->jx_b3, w->ctl, slot, S.n2, R.nn};
+    SrcPsi src{w->src, w->rho_ion, w->rho_b_src(), w->jx_b_src(), w->bsrc_stride(), w->ctl, slot, S.n2, R.nn};
     SinkPsiFinal snk{dst, m->bpart, m->ncb, S.n1, S.n2, w->p.psi_threshold - 1, 0.0, m->bstride};
     if (S.wave) {
         prof_begin(w, "solve");
@@ -1097,6 +1098,7 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     double *psi = w->psi_b[slot];
     const int laser = (w->any_laser() && !w->asq3()) ? 1 : 0;
     const dim3 pg(R.pb, R.nw);
+    if (w->bunch) QP_TRY(bunch_slice_sources(w->bunch, (const Ctl *)w->ctl, slot, 0, s));
     prof_begin(w, "push_A");
     MLAUNCH(k_mx_A, pg, kPT, 0, m->P, R.g, si, (const double *)w->psim, w->src, w->asq2,
             (const LaserGauss *)w->d_laser, laser, w->rho, w->jx, w->chi, s_phase ? 1 : 0,
@@ -1132,6 +1134,9 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     MLAUNCH(k_mx_out, dim3(R.gb, R.nw), 256, 0, w->ex, w->ey, w->ez, (const double *)w->psi_b[0],
             (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz,
             first ? 0 : 1, w->ctl, slot, 0);
+    if (w->bunch)
+        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, slot, 0, slot,
+                                  s));
     return QP_OK;
 }
 
@@ -1187,7 +1192,12 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     R.gb = (int)((R.nn + 255) / 256);
     R.sstride = (size_t)w->S.n1 * w->S.nh;
     R.g = MxGeom{p.ny, p.nz, p.dy, p.dz, p.ly, p.lz, p.dx, 1.0 / p.dy, 1.0 / p.dz};
-    for (int sl = 0; sl < 2; ++sl) R.si[sl] = MxSliceIn{w->asq3(), w->asq2, w->rho_b3, w->jx_b3, sl};
+    if (w->bunch && w->bunch->epoch != w->bunch_epoch) {  // the bunch's particle arrays moved
+        w->bunch_epoch = w->bunch->epoch;
+        ++w->inputs_epoch;
+    }
+    for (int sl = 0; sl < 2; ++sl)
+        R.si[sl] = MxSliceIn{w->asq3(), w->asq2, w->rho_b_src(), w->jx_b_src(), w->bsrc_stride(), sl};
     // captured slice steps hold the driver pointers and flags of their inputs
     if (m->graph_epoch != w->inputs_epoch) {
         for (auto &g : m->gexec)
@@ -1219,6 +1229,7 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     QP_HIP_CHECK(hipMemcpyAsync(w->ctl, hc0.data(), sizeof(Ctl) * nw, hipMemcpyHostToDevice, s));
     QP_HIP_CHECK(hipEventRecord(w->ev0, s));
     MLAUNCH(k_mx_ion, dim3(R.pb, nw), kPT, 0, m->P, R.g, w->rho_ion);
+    if (w->bunch) QP_TRY(bunch_clear_fields(w->bunch, s));
 
     const bool graph_ok = p.use_graph && !cb && !w->prof_class;
     std::vector<char> delivered_stop(nw, 0);
@@ -1278,6 +1289,9 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
                        (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx,
                        p.dy, p.dz, p.nx > 1 ? 1 : 0, w->ctl, 0, 1);
     QP_HIP_CHECK(hipGetLastError());
+    // bunch bucket 0: slice 0's fields are final only after output_step(0)
+    if (w->bunch)
+        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, 0, s));
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
