@@ -118,6 +118,73 @@ def bench_envelope(capi, ctx, wl, steps, use_graph=1):
     return res
 
 
+# Schwinger field in plasma units at n0 = 1e18 cm^-3 (only scales the bunch's chi).
+FIELD_SCHWINGER = 1.4e7
+
+
+def bench_bunch_job(args, capi, workloads):
+    """C5: the whole beam-driven job, `steps` beam time steps of run()
+    (src/System_3d.cpp:199-236) without QED: the mixed wake pass with the
+    bunch streamed through it (per-slice sources and field gathers on the
+    device), then the bunch push.  value = plasma pushes of all passes / time."""
+    wl = workloads.get(args.workload)
+    bp = wl.bunch
+    steps = args.beam_steps or bp["steps"]
+    b0 = wl.bunch_particles()
+    nb = len(b0["x"])
+    log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles} bunch={nb}")
+    ctx = capi.Context(0)
+    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=0 if args.no_graph else 1)
+    db = capi.Bunch(ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, FIELD_SCHWINGER)
+    wk.attach_bunch(db)
+    db.set_particles(b0)
+    for _ in range(args.warmup):     # one beam step, then back to the initial bunch
+        wk.run()
+        db.push(bp["dt"])
+    db.set_particles(b0)
+    ctx.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for s_ in range(steps):
+        _, st, stop = wk.run()
+        dev_ms.append(wk.last_time_ms())
+        db.push(bp["dt"])
+        log(f"[bench]   beam step {s_}: pass {dev_ms[-1]:.1f} ms (status {st})")
+    ctx.synchronize()
+    t = time.perf_counter() - t0
+    pushes = wl.n_particles * (wl.nx - 1) * steps
+    value = pushes / t
+    final = db.get_particles()
+    log(f"[bench] C5: {steps} beam steps in {t:.2f} s -> {value:.3e} pushes/s; bunch <gamma> "
+        f"{float(np.mean(final['gamma'])):.2f}")
+    classes, roofline = {}, None
+    if not args.no_profile:
+        for cls in ("push_B", "push_C"):
+            n, us = wk.profile_kernel(cls)
+            classes[cls] = {"launches": n, "mean_us": us, "total_ms": n * us * 1e-3}
+            log(f"[bench]   {cls}: {n} launches x {us:.2f} us")
+        dom = max(classes, key=lambda c: classes[c]["total_ms"])
+        byts = bytes_per_particle(dom, wl.K) * wl.n_particles
+        ach = byts / (classes[dom]["mean_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_mx_{dom[-1]}",
+                    "algorithmic_bytes_per_launch": byts}
+    wk.close()
+    db.close()
+    ctx.close()
+    line = {
+        "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, beam-driven job)",
+        "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic", "config": wl.describe(),
+        "bunch_particles": nb, "device_ms_per_pass": float(np.median(dev_ms)),
+        "kernel_classes": classes, "roofline": roofline, "cpu_baseline": None,
+        "note": "3D arrays of the reference's layout (281 GiB at this size) are never formed: the bunch "
+                "sources and field gathers are streamed slice by slice",
+    }
+    print(json.dumps(line), flush=True)
+
+
 def calib_copy(mib):
     """Device-to-device copy of a known size through the HIP runtime (its copy
     kernel), so FETCH_SIZE / WRITE_SIZE can be scaled to bytes."""
@@ -314,6 +381,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel-class event pass")
     ap.add_argument("--c4-class", type=int, default=0,
                     help="C4 only: run the members of this grid size alone (profiling runs)")
+    ap.add_argument("--beam-steps", type=int, default=0,
+                    help="C5: beam time steps to time (default: the configuration's 10; fewer for profiling)")
     ap.add_argument("--envelope-only", action="store_true",
                     help="time only the laser-envelope advance on the workload's grid (profiling runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -329,6 +398,9 @@ def main():
         calib_copy(args.calib_copy_mib)
     if args.workload == "C4":
         bench_c4(args, capi, workloads)
+        return
+    if args.workload == "C5":
+        bench_bunch_job(args, capi, workloads)
         return
     wl = workloads.get(args.workload)
     log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles}")

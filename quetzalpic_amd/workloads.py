@@ -14,7 +14,7 @@ import numpy as np
 
 class Workload:
     def __init__(self, name, lx, dx, L, n, a0, w0, sigma, profile=("constant", -1.0), ppc=2, K=5,
-                 thermal=0.0, seed=1234, note=""):
+                 thermal=0.0, seed=1234, note="", laser_x0=None, bunch=None):
         self.name = name
         self.lx, self.dx0 = lx, dx
         self.L, self.n0 = L, n
@@ -25,6 +25,8 @@ class Workload:
         self.thermal = thermal
         self.seed = seed
         self.note = note
+        self.laser_x0 = laser_x0
+        self.bunch = bunch
         # grid rule (src/System_3d.cpp:87-95)
         self.nx = int(lx / dx)
         self.ny = self.nz = int(L / (L / n))
@@ -34,8 +36,44 @@ class Workload:
     @property
     def laser(self):
         """(a0, xsigma, ysigma, zsigma, x0, y0, z0) in the reference's keys."""
-        return (self.a0, math.sqrt(8.0) * self.sigma, 2 * self.w0, 2 * self.w0, self.lx - 3 * self.sigma,
-                self.L / 2, self.L / 2)
+        x0 = self.lx - 3 * self.sigma if self.laser_x0 is None else self.laser_x0
+        return (self.a0, math.sqrt(8.0) * self.sigma, 2 * self.w0, 2 * self.w0, x0, self.L / 2, self.L / 2)
+
+    def bunch_particles(self):
+        """Driver-bunch electrons of a Gaussian [bunch] table (src/Config_reader.cpp:146-160:
+        rho0 exp(-sum (x-x0)^2/w^2), w = full 1/e^2 width / sqrt(8)) on the lattice of
+        init_bunch_particles (src/System_3d.cpp:771-803: ((i+1/2) dx/ppcx, ...), i outer,
+        n = rho/ppc^3, gamma given, p_x = sqrt(gamma^2 - 1)).  The reference keeps every
+        cell with rho > 0, which for a Gaussian is every cell of the box; here cells
+        below cutoff * rho0 are dropped (extension).  Electrons: q = -n, cmr = -1."""
+        b = self.bunch
+        ppc = b["ppc"]
+        wx, wy = b["xsigma"] / math.sqrt(8.0), b["ysigma"] / math.sqrt(8.0)
+        x0, y0, z0 = b["x0"], self.L / 2, self.L / 2
+        reach = math.sqrt(math.log(1.0 / b["cutoff"]))
+
+        def axis(n, d, c, w):
+            i = np.arange(ppc * n)
+            v = (i + 0.5) * d / ppc
+            return v[np.abs(v - c) < reach * w]
+
+        xs, ys, zs = axis(self.nx, self.dx, x0, wx), axis(self.ny, self.dy, y0, wy), axis(self.nz, self.dz, z0, wy)
+        parts = {k: [] for k in ("x", "y", "z", "n")}
+        Y, Z = np.meshgrid(ys, zs, indexing="ij")
+        ey = (Y - y0) ** 2 / wy ** 2 + (Z - z0) ** 2 / wy ** 2
+        for x in xs:   # i outer, then j, k (row-major Y, Z)
+            rho = b["rho0"] * np.exp(-(x - x0) ** 2 / wx ** 2 - ey)
+            keep = rho > b["cutoff"] * b["rho0"]
+            parts["x"].append(np.full(int(keep.sum()), x))
+            parts["y"].append(Y[keep])
+            parts["z"].append(Z[keep])
+            parts["n"].append(rho[keep] / ppc ** 3)
+        out = {k: np.concatenate(v) for k, v in parts.items()}
+        n = len(out["x"])
+        g = b["gamma"]
+        out.update(px=np.full(n, math.sqrt(g * g - 1)), py=np.zeros(n), pz=np.zeros(n), gamma=np.full(n, g),
+                   q=-out["n"], cmr=np.full(n, -1.0), photon=np.zeros(n, np.int32))
+        return out
 
     def profile_value(self, y, z):
         kind = self.profile[0]
@@ -80,7 +118,7 @@ class Workload:
             "workload": self.name, "grid": f"{self.ny}x{self.nz}", "box": self.L, "slices": self.nx,
             "dxi": self.dx, "particles": self.n_particles, "ppc": self.ppc * self.ppc, "K": self.K,
             "a0": self.a0, "w0": self.w0, "sigma_xi": self.sigma, "profile": list(self.profile),
-            "note": self.note,
+            "note": self.note, **({"bunch": self.bunch} if self.bunch else {}),
         }
 
 
@@ -116,6 +154,13 @@ def get(name):
         return Workload("C3: 512x512 over [-15,15]^2, 4 ppc (1048576 particles), 2000 slices dxi=0.01, n0=1, "
                         "laser a0=4 w0=3.5 sigma=1.5, thermal p N(0,1e-3)", lx=20.0, dx=0.01, L=30.0, n=512,
                         a0=4.0, w0=3.5, sigma=1.5, thermal=1e-3)
+    if name == "C5":
+        return Workload("C5: 1024x1024 over [-16,16]^2, 4 ppc (4194304 particles), 4000 slices dxi=0.005, n0=1, "
+                        "Gaussian electron bunch nb/n0=2 sigma_r=0.5 sigma_xi=0.7 gamma=2000 (cells above "
+                        "1e-3 of the peak, 1 per cell), trailing laser a0=1.5 w0=4 sigma=1.5, 10 beam steps "
+                        "dt=20", lx=20.0, dx=0.005, L=32.0, n=1024, a0=1.5, w0=4.0, sigma=1.5, laser_x0=11.0,
+                        bunch={"rho0": 2.0, "xsigma": 2.8, "ysigma": 2.0, "gamma": 2000.0, "x0": 17.0,
+                               "ppc": 1, "cutoff": 1e-3, "dt": 20.0, "steps": 10})
     raise KeyError(name)
 
 
