@@ -7,8 +7,9 @@
 // particles are kept sorted by x-cell i1 = floor(x/dx) (bucket nx-1 holds the
 // particles whose cell is outside [0, nx-1)), and the wake streams both: at
 // the start of slice i the buckets i and i-1 deposit rho_b(i), jx_b(i) into
-// two planes; at its end bucket i gathers from the fields of slice i and the
-// saved fields of slice i+1.  No 3D array is needed.
+// two planes; once slice i's fields are final, the particles whose x-cell has
+// slice i as its upper or lower face take that half of their trilinear gather.
+// No 3D array and no plane of an earlier slice is kept.
 #pragma once
 
 #include "qp_internal.h"
@@ -32,24 +33,22 @@ struct Bunch {
     size_t np = 0, cap = 0;
     BunchDev cur{}, tmp{};       // sorted arrays and the re-bucketing scratch
     int *start = nullptr, *count = nullptr, *cursor = nullptr, *key = nullptr;
-    double *planes = nullptr;    // rho_b, jx_b planes, then 2 slots x 5 saved field planes
+    double *planes = nullptr;    // rho_b, jx_b planes of the current slice
     int attached = 0;            // wakes streaming this bunch
     unsigned epoch = 0;          // bumped when the particle arrays move (captured slice steps hold them)
 
     size_t nn() const { return (size_t)ny * nz; }
     double *rho_plane() const { return planes; }
     double *jx_plane() const { return planes + nn(); }
-    double *saved() const { return planes + 2 * nn(); }
     hipStream_t stream() const { return ctx->stream; }
 };
 
 // Launches for an attached bunch, enqueued on s.  slot >= 0: the slice index
 // comes from ctl->slice_of[slot] (captured slice steps); otherwise `slice`.
-// par: parity of the slice (selects the saved-field slot).
 int bunch_clear_fields(Bunch *b, hipStream_t s);
 int bunch_slice_sources(Bunch *b, const Ctl *ctl, int slot, int slice, hipStream_t s);
 int bunch_slice_gather(Bunch *b, const double *ex, const double *ey, const double *ez, const double *by,
-                       const double *bz, const Ctl *ctl, int slot, int slice, int par, hipStream_t s);
+                       const double *bz, const Ctl *ctl, int slot, int slice, hipStream_t s);
 
 }  // namespace qp
 

@@ -61,6 +61,32 @@ __device__ __forceinline__ int bucket_of(double x, const BunchDev &B) {
     return (t >= 0.0 && t < (double)(B.nx - 1)) ? (int)t : B.nx - 1;
 }
 
+// Slot of this lane in counter ctr[b], one atomic per distinct b in the
+// wavefront (the bunch is sorted by bucket, so a wavefront mostly holds one or
+// two buckets and per-particle atomics would all hit the same counter).  Every
+// lane of the wavefront calls it; inactive lanes take no slot.
+__device__ __forceinline__ unsigned wave_slot(unsigned *ctr, int b, bool active) {
+    const int lane = threadIdx.x & 63;
+    bool done = !active;
+    unsigned res = 0;
+    while (true) {
+        const unsigned long long pending = __ballot(!done);
+        if (!pending) break;
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lk = __shfl(b, leader);
+        const bool mine = !done && b == lk;
+        const unsigned long long m = __ballot(mine);
+        unsigned base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lk], (unsigned)__popcll(m));
+        base = (unsigned)__shfl((int)base, leader);
+        if (mine) {
+            res = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+            done = true;
+        }
+    }
+    return res;
+}
+
 __global__ void k_bunch_zero(double *planes, size_t n, const Ctl *ctl) {
     if (ctl && ctl->stop_acc) return;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
@@ -107,60 +133,73 @@ __global__ __launch_bounds__(kBT) void k_bunch_src(BunchDev B, double *rho, doub
     }
 }
 
-// array_to_particle (src/array_utils.cpp:340-374) with slice i1 in a, i2 in b.
-__device__ __forceinline__ double gather8(const double *a, const double *b, const Tw &w, int n2, double x_frac) {
-    const size_t c11 = (size_t)w.j1 * n2 + w.k1, c21 = (size_t)w.j2 * n2 + w.k1;
-    const size_t c12 = (size_t)w.j1 * n2 + w.k2, c22 = (size_t)w.j2 * n2 + w.k2;
-    return a[c11] * (1 - x_frac) * (1 - w.yf) * (1 - w.zf) + a[c21] * (1 - x_frac) * w.yf * (1 - w.zf) +
-           a[c12] * (1 - x_frac) * (1 - w.yf) * w.zf + a[c22] * (1 - x_frac) * w.yf * w.zf +
-           b[c11] * x_frac * (1 - w.yf) * (1 - w.zf) + b[c21] * x_frac * w.yf * (1 - w.zf) +
-           b[c12] * x_frac * (1 - w.yf) * w.zf + b[c22] * x_frac * w.yf * w.zf;
+// The two halves of array_to_particle (src/array_utils.cpp:340-374): the
+// four terms of the x-cell's lower slice i1 (weight 1 - x_frac) and of its
+// upper slice i2 = i1 + 1 (weight x_frac), from one 2D plane each.
+__device__ __forceinline__ double half_lo(const double *a, const Tw &w, int n2, double x_frac) {
+    return a[(size_t)w.j1 * n2 + w.k1] * (1 - x_frac) * (1 - w.yf) * (1 - w.zf) +
+           a[(size_t)w.j2 * n2 + w.k1] * (1 - x_frac) * w.yf * (1 - w.zf) +
+           a[(size_t)w.j1 * n2 + w.k2] * (1 - x_frac) * (1 - w.yf) * w.zf +
+           a[(size_t)w.j2 * n2 + w.k2] * (1 - x_frac) * w.yf * w.zf;
+}
+__device__ __forceinline__ double half_hi(const double *b, const Tw &w, int n2, double x_frac) {
+    return b[(size_t)w.j1 * n2 + w.k1] * x_frac * (1 - w.yf) * (1 - w.zf) +
+           b[(size_t)w.j2 * n2 + w.k1] * x_frac * w.yf * (1 - w.zf) +
+           b[(size_t)w.j1 * n2 + w.k2] * x_frac * (1 - w.yf) * w.zf +
+           b[(size_t)w.j2 * n2 + w.k2] * x_frac * w.yf * w.zf;
 }
 
 struct Planes5 {
     const double *f[5];  // ex, ey, ez, by, bz of the slice
 };
 
-// Bucket i gathers from slice i (the current planes) and slice i+1 (saved slot
-// 1 - par), and derives chi (src/System_3d.cpp:843-860); then the current
-// planes are saved into slot par for slice i-1.
-__global__ __launch_bounds__(kBT) void k_bunch_gather(BunchDev B, Planes5 cur, double *saved, size_t nn,
-                                                      const Ctl *ctl, int slot, int slice, int par) {
+// Once slice i's fields are final: bucket i - 1 (slice i is their i2) stores
+// its upper half in the field arrays; bucket i (slice i is their i1) adds its
+// lower half to the upper half stored at slice i + 1, which completes its
+// gather, and derives chi (src/System_3d.cpp:843-860).  No plane of an
+// earlier slice is kept.
+__global__ __launch_bounds__(kBT) void k_bunch_gather(BunchDev B, Planes5 cur, const Ctl *ctl, int slot,
+                                                      int slice) {
     if (ctl && ctl->stop_acc) return;
     const int i = slice_index(ctl, slot, slice);
     if (i < 0 || i >= B.nx) return;
-    const double *prev = saved + (size_t)(1 - par) * 5 * nn;
-    double *keep = saved + (size_t)par * 5 * nn;
-    const int stride = gridDim.x * blockDim.x, t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    int a0 = 0, na = 0, b0 = 0, nb = 0;
     if (i <= B.nx - 2) {
-        const int a0 = B.start[i], na = B.start[i + 1] - a0;
-        const int n2 = B.n2;
-        for (int t = t0; t < na; t += stride) {
-            const int p = a0 + t;
-            const double x = B.x[p];
-            const int i1 = (int)floor(x / B.dx);
-            const double x_frac = x / B.dx - i1;
-            const Tw w = tw_of(B.y[p], B.z[p], B);
-            const double ex = gather8(cur.f[0], prev, w, n2, x_frac);
-            const double ey = gather8(cur.f[1], prev + nn, w, n2, x_frac);
-            const double ez = gather8(cur.f[2], prev + 2 * nn, w, n2, x_frac);
-            const double by = gather8(cur.f[3], prev + 3 * nn, w, n2, x_frac);
-            const double bz = gather8(cur.f[4], prev + 4 * nn, w, n2, x_frac);
-            const double px = B.px[p], py = B.py[p], pz = B.pz[p], g = B.gamma[p];
-            const double fx = g * ex + py * bz - pz * by;
-            const double fy = g * ey - px * bz;
-            const double fz = g * ez + px * by;
-            const double fl = px * ex + py * ey + pz * ez;
-            B.ex[p] = ex;
-            B.ey[p] = ey;
-            B.ez[p] = ez;
-            B.by[p] = by;
-            B.bz[p] = bz;
-            B.chi[p] = sqrt(fx * fx + fy * fy + fz * fz - fl * fl) / B.fs;
-        }
+        a0 = B.start[i];
+        na = B.start[i + 1] - a0;
     }
-    for (int f = 0; f < 5; ++f)
-        for (size_t t = (size_t)t0; t < nn; t += (size_t)stride) keep[(size_t)f * nn + t] = cur.f[f][t];
+    if (i >= 1) {
+        b0 = B.start[i - 1];
+        nb = B.start[i] - b0;
+    }
+    const int n2 = B.n2;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na + nb; t += gridDim.x * blockDim.x) {
+        const bool low = t < na;
+        const int p = low ? a0 + t : b0 + (t - na);
+        const double x = B.x[p];
+        const int i1 = (int)floor(x / B.dx);
+        const double x_frac = x / B.dx - i1;
+        const Tw w = tw_of(B.y[p], B.z[p], B);
+        double *out[5] = {B.ex, B.ey, B.ez, B.by, B.bz};
+        if (!low) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) out[f][p] = half_hi(cur.f[f], w, n2, x_frac);
+            continue;
+        }
+        double v[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            v[f] = half_lo(cur.f[f], w, n2, x_frac) + out[f][p];
+            out[f][p] = v[f];
+        }
+        const double ex = v[0], ey = v[1], ez = v[2], by = v[3], bz = v[4];
+        const double px = B.px[p], py = B.py[p], pz = B.pz[p], g = B.gamma[p];
+        const double fx = g * ex + py * bz - pz * by;
+        const double fy = g * ey - px * bz;
+        const double fz = g * ez + px * by;
+        const double fl = px * ex + py * ey + pz * ez;
+        B.chi[p] = sqrt(fx * fx + fy * fy + fz * fz - fl * fl) / B.fs;
+    }
 }
 
 __global__ void k_bunch_clear(BunchDev B) {
@@ -171,9 +210,7 @@ __global__ void k_bunch_clear(BunchDev B) {
 
 // advance_bunch_particles without QED emission (src/System_3d.cpp:862-890):
 // reads S, writes the advanced particle to D (same slot) with its bucket.
-__global__ void k_bunch_push(BunchDev S, BunchDev D, double dt, int *key, unsigned *count) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= S.np) return;
+__device__ __forceinline__ int push_one(const BunchDev &S, const BunchDev &D, double dt, int p) {
     double x = S.x[p], y = S.y[p], z = S.z[p], px = S.px[p], py = S.py[p], pz = S.pz[p], g = S.gamma[p];
     const double ex = S.ex[p], ey = S.ey[p], ez = S.ez[p], by = S.by[p], bz = S.bz[p];
     if (S.photon[p]) {
@@ -211,17 +248,24 @@ __global__ void k_bunch_push(BunchDev S, BunchDev D, double dt, int *key, unsign
     D.by[p] = by;
     D.bz[p] = bz;
     D.chi[p] = S.chi[p];
-    const int b = bucket_of(x, D);
-    key[p] = b;
-    atomicAdd(&count[b], 1u);
+    return bucket_of(x, D);
+}
+
+__global__ void k_bunch_push(BunchDev S, BunchDev D, double dt, int *key, unsigned *count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = p < S.np;
+    int b = 0;
+    if (valid) b = push_one(S, D, dt, p);
+    (void)wave_slot(count, b, valid);
+    if (valid) key[p] = b;
 }
 
 __global__ void k_bunch_key(BunchDev S, int *key, unsigned *count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= S.np) return;
-    const int b = bucket_of(S.x[p], S);
-    key[p] = b;
-    atomicAdd(&count[b], 1u);
+    const bool valid = p < S.np;
+    const int b = valid ? bucket_of(S.x[p], S) : 0;
+    (void)wave_slot(count, b, valid);
+    if (valid) key[p] = b;
 }
 
 // Exclusive scan of nb bucket counts into start[0..nb] (one block of 1024).
@@ -253,9 +297,11 @@ __global__ void k_bunch_scan(const unsigned *count, int *start, unsigned *cursor
 
 __global__ void k_bunch_scatter(BunchDev S, BunchDev D, const int *key, unsigned *cursor) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= S.np) return;
-    const int b = key[p];
-    const unsigned d = (unsigned)D.start[b] + atomicAdd(&cursor[b], 1u);
+    const bool valid = p < S.np;
+    const int b = valid ? key[p] : 0;
+    const unsigned slot = wave_slot(cursor, b, valid);
+    if (!valid) return;
+    const unsigned d = (unsigned)D.start[b] + slot;
     if (d >= (unsigned)S.np) return;
     D.x[d] = S.x[p];
     D.y[d] = S.y[p];
@@ -361,9 +407,9 @@ int bunch_slice_sources(Bunch *b, const Ctl *ctl, int slot, int slice, hipStream
 }
 
 int bunch_slice_gather(Bunch *b, const double *ex, const double *ey, const double *ez, const double *by,
-                       const double *bz, const Ctl *ctl, int slot, int slice, int par, hipStream_t s) {
+                       const double *bz, const Ctl *ctl, int slot, int slice, hipStream_t s) {
     Planes5 cur{{ex, ey, ez, by, bz}};
-    BLAUNCH(k_bunch_gather, kSliceBlocks, kBT, b->cur, cur, b->saved(), b->nn(), ctl, slot, slice, par);
+    BLAUNCH(k_bunch_gather, kSliceBlocks, kBT, b->cur, cur, ctl, slot, slice);
     return QP_OK;
 }
 
@@ -435,13 +481,13 @@ int qp_bunch_create(qp_ctx ctx, int nx, int ny, int nz, double dx, double dy, do
     if (hipMalloc(&b->start, sizeof(int) * (nx + 1)) != hipSuccess ||
         hipMalloc(&b->count, sizeof(unsigned) * nx) != hipSuccess ||
         hipMalloc(&b->cursor, sizeof(unsigned) * nx) != hipSuccess ||
-        hipMalloc(&b->planes, sizeof(double) * 12 * b->nn()) != hipSuccess) {
+        hipMalloc(&b->planes, sizeof(double) * 2 * b->nn()) != hipSuccess) {
         set_error("qp_bunch_create: device allocation failed");
         return fail(QP_ERR_ALLOC);
     }
     hipStream_t s = b->stream();
     if (hipMemsetAsync(b->start, 0, sizeof(int) * (nx + 1), s) != hipSuccess ||
-        hipMemsetAsync(b->planes, 0, sizeof(double) * 12 * b->nn(), s) != hipSuccess ||
+        hipMemsetAsync(b->planes, 0, sizeof(double) * 2 * b->nn(), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return fail(QP_ERR_HIP);
     b->cur.start = b->start;

@@ -451,7 +451,7 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
         prof_end(w, "push");
         LAUNCH(k_output, gb, w->ex, w->ey, w->ez, w->psi, w->psi_prev, w->by, w->bz, n1, n2, p.dx, p.dy, p.dz,
                i < p.nx - 1 ? 1 : 0, ctl, 0);
-        if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, i, i & 1, s));
+        if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, i, s));
         if (cb) {
             QP_HIP_CHECK(hipStreamSynchronize(s));
             Ctl hc;
@@ -472,7 +472,7 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
     LAUNCH(k_output, gb, w->ex, w->ey, w->ez, w->psi, w->psi_prev, w->by, w->bz, n1, n2, p.dx, p.dy, p.dz,
            p.nx > 1 ? 1 : 0, ctl, 1);
     // bunch bucket 0: slice 0's fields are final only after output_step(0)
-    if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, 0, 0, s));
+    if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, 0, s));
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;

@@ -1135,8 +1135,7 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
             (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz,
             first ? 0 : 1, w->ctl, slot, 0);
     if (w->bunch)
-        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, slot, 0, slot,
-                                  s));
+        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, slot, 0, s));
     return QP_OK;
 }
 
@@ -1291,7 +1290,7 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     QP_HIP_CHECK(hipGetLastError());
     // bunch bucket 0: slice 0's fields are final only after output_step(0)
     if (w->bunch)
-        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, 0, s));
+        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, s));
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
