@@ -94,7 +94,7 @@ def load(variant="det"):
     lib.qpo_wake_run_partial.argtypes = run_args + [ctypes.c_int]
     lib.qpo_wake_run_partial.restype = ctypes.c_int
     lib.qpo_deposit3d.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 4 + [_dp]
-    lib.qpo_gather3d.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 3 + [_dp]
+    lib.qpo_gather3d.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 6 + [_dp]
     lib.qpo_gather3d.restype = ctypes.c_double
     lib.qpo_bunch_deposit.argtypes = [ctypes.POINTER(QpoGrid), ctypes.c_size_t] + [_dp] * 8
     lib.qpo_bunch_fields.argtypes = ([ctypes.POINTER(QpoGrid), ctypes.c_size_t] + [_dp] * 12 +
@@ -202,8 +202,14 @@ def deposit3d(g, x, y, z, v, a3):
     load().qpo_deposit3d(ctypes.byref(g), x, y, z, v, _ptr(a3))
 
 
-def gather3d(g, x, y, z, a3):
-    return load().qpo_gather3d(ctypes.byref(g), x, y, z, _ptr(_f64(a3)))
+def gather3d(g, x, y, z, a3, origin=(0.0, 0.0, 0.0)):
+    return load().qpo_gather3d(ctypes.byref(g), *origin, x, y, z, _ptr(_f64(a3)))
+
+
+def field_origins(g):
+    """Origins of the 3D field arrays (src/System_3d.cpp:179-183)."""
+    return {"ex": (-0.5 * g.dx, 0.0, 0.0), "ey": (0.0, 0.5 * g.dy, 0.0), "ez": (0.0, 0.0, 0.5 * g.dz),
+            "by": (0.0, 0.0, 0.5 * g.dz), "bz": (0.0, 0.5 * g.dy, 0.0)}
 
 
 def bunch_deposit(g, b):

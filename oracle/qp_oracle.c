@@ -876,17 +876,18 @@ void qpo_deposit3d(const qpo_grid *g, double x, double y, double z, double v, do
 }
 
 /* src/array_utils.cpp:340-374 */
-double qpo_gather3d(const qpo_grid *g, double x, double y, double z, const double *a3) {
-    const int i1 = (int)floor(x / g->dx), i2 = i1 + 1;
-    const double x_frac = x / g->dx - i1;
+double qpo_gather3d(const qpo_grid *g, double ox, double oy, double oz, double x, double y, double z,
+                    const double *a3) {
+    const int i1 = (int)floor((x - ox) / g->dx), i2 = i1 + 1;
+    const double x_frac = (x - ox) / g->dx - i1;
     if (i1 < 0 || i2 >= g->nx) return 0.0;
     const int n1 = g->ny, n2 = g->nz;
-    int j1 = (int)floor(y / g->dy);
-    const double y_frac = y / g->dy - j1;
+    int j1 = (int)floor((y - oy) / g->dy);
+    const double y_frac = (y - oy) / g->dy - j1;
     const int j2 = wrap_index(j1 + 1, n1);
     j1 = wrap_index(j1, n1);
-    int k1 = (int)floor(z / g->dz);
-    const double z_frac = z / g->dz - k1;
+    int k1 = (int)floor((z - oz) / g->dz);
+    const double z_frac = (z - oz) / g->dz - k1;
     const int k2 = wrap_index(k1 + 1, n2);
     k1 = wrap_index(k1, n2);
     const size_t nn = (size_t)n1 * n2;
@@ -920,11 +921,12 @@ void qpo_bunch_fields(const qpo_grid *g, size_t np, const double *x, const doubl
                       const double *bz3, double field_schwinger, double *ex, double *ey, double *ez,
                       double *by, double *bz, double *chi) {
     for (size_t p = 0; p < np; ++p) {
-        ex[p] = qpo_gather3d(g, x[p], y[p], z[p], ex3);
-        ey[p] = qpo_gather3d(g, x[p], y[p], z[p], ey3);
-        ez[p] = qpo_gather3d(g, x[p], y[p], z[p], ez3);
-        by[p] = qpo_gather3d(g, x[p], y[p], z[p], by3);
-        bz[p] = qpo_gather3d(g, x[p], y[p], z[p], bz3);
+        const double hx = 0.5 * g->dx, hy = 0.5 * g->dy, hz = 0.5 * g->dz;  /* :179-183 */
+        ex[p] = qpo_gather3d(g, -hx, 0, 0, x[p], y[p], z[p], ex3);
+        ey[p] = qpo_gather3d(g, 0, hy, 0, x[p], y[p], z[p], ey3);
+        ez[p] = qpo_gather3d(g, 0, 0, hz, x[p], y[p], z[p], ez3);
+        by[p] = qpo_gather3d(g, 0, 0, hz, x[p], y[p], z[p], by3);
+        bz[p] = qpo_gather3d(g, 0, hy, 0, x[p], y[p], z[p], bz3);
         const double fx = gamma[p] * ex[p] + py[p] * bz[p] - pz[p] * by[p];
         const double fy = gamma[p] * ey[p] - px[p] * bz[p];
         const double fz = gamma[p] * ez[p] + px[p] * by[p];

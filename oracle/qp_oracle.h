@@ -142,14 +142,19 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np, const double *y, const do
 /* deposit(x,y,z,v,array3d&), src/array_utils.cpp:290-338: nothing when the
  * x-cell i1 = floor(x/dx) is outside [0, nx-1). */
 void qpo_deposit3d(const qpo_grid *g, double x, double y, double z, double v, double *a3);
-/* array_to_particle(x,y,z,array3d&), src/array_utils.cpp:340-374 (0 outside). */
-double qpo_gather3d(const qpo_grid *g, double x, double y, double z, const double *a3);
+/* array_to_particle(x,y,z,array3d&), src/array_utils.cpp:340-374, for an
+ * array with origin (ox, oy, oz) (0 outside the x range). */
+double qpo_gather3d(const qpo_grid *g, double ox, double oy, double oz, double x, double y, double z,
+                    const double *a3);
 /* deposit_bunch_densities, src/System_3d.cpp:926-946: rho_b = sum q S,
  * jx_b = sum q px/gamma S (both zeroed first); q = species charge * n. */
 void qpo_bunch_deposit(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
                        const double *px, const double *gamma, const double *q, double *rho_b, double *jx_b);
 /* deposit_wakefield_to_bunch, src/System_3d.cpp:843-860: the five fields at
- * each particle and the QED parameter chi (field_schwinger in plasma units). */
+ * each particle and the QED parameter chi (field_schwinger in plasma units).
+ * The field arrays carry the reference's staggered origins
+ * (src/System_3d.cpp:179-183): ex (-dx/2, 0, 0), ey and bz (0, dy/2, 0),
+ * ez and by (0, 0, dz/2). */
 void qpo_bunch_fields(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
                       const double *px, const double *py, const double *pz, const double *gamma,
                       const double *ex3, const double *ey3, const double *ez3, const double *by3,

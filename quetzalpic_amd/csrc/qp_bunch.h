@@ -4,12 +4,12 @@
 // The reference deposits the whole bunch into 3D arrays before the pass
 // (deposit_bunch_densities, src/System_3d.cpp:926-946) and gathers the 3D
 // field arrays after it (deposit_wakefield_to_bunch, :843-860).  Here the
-// particles are kept sorted by x-cell i1 = floor(x/dx) (bucket nx-1 holds the
-// particles whose cell is outside [0, nx-1)), and the wake streams both: at
-// the start of slice i the buckets i and i-1 deposit rho_b(i), jx_b(i) into
-// two planes; once slice i's fields are final, the particles whose x-cell has
-// slice i as its upper or lower face take that half of their trilinear gather.
-// No 3D array and no plane of an earlier slice is kept.
+// particles are kept sorted by half x-cell floor(2x/dx), and the wake streams
+// both: at the start of slice i the particles whose x-cell has slice i as a
+// face deposit rho_b(i), jx_b(i) into two planes; once slice i's fields are
+// final, the particles for which slice i is a face of their x-cell in a field
+// array (e_x is staggered by -dx/2 in x) take that half of their trilinear
+// gather.  No 3D array and no plane of an earlier slice is kept.
 #pragma once
 
 #include "qp_internal.h"
@@ -21,7 +21,7 @@ struct BunchDev {
     double *x, *y, *z, *px, *py, *pz, *gamma, *q, *cmr;  // q = species charge * n
     int *photon, *id;                                      // id = index at upload
     double *ex, *ey, *ez, *by, *bz, *chi;                  // gathered by the last pass
-    const int *start;  // nx + 1 entries: bucket b holds [start[b], start[b+1])
+    const int *start;  // 2 nx + 1 entries: half-cell bucket b holds [start[b], start[b+1])
     int np, nx, n1, n2;
     double dx, dy, dz, fs;  // fs: Schwinger field in plasma units
 };

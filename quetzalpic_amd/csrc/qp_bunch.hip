@@ -1,6 +1,7 @@
 // qp_bunch.hip -- the driver bunch on the device: streamed per-slice sources
 // and field gathers for the wake loops, the explicit push and re-bucketing
-// (src/System_3d.cpp:843-946, CIC of src/array_utils.cpp:290-374).
+// (src/System_3d.cpp:843-946, CIC of src/array_utils.cpp:290-374, field
+// array origins of src/System_3d.cpp:179-183).
 //
 // Arithmetic follows the reference's operation order and this file is built
 // without FMA contraction, like the plain FP64 path, so that deposits (up to
@@ -41,24 +42,39 @@ struct Tw {
     int j1, j2, k1, k2;
     double yf, zf;
 };
-__device__ __forceinline__ Tw tw_of(double y, double z, const BunchDev &B) {
+__device__ __forceinline__ Tw tw_of(double y, double z, const BunchDev &B, double oy = 0.0, double oz = 0.0) {
     Tw t;
-    const int j = (int)floor(y / B.dy);
-    t.yf = y / B.dy - j;
+    const int j = (int)floor((y - oy) / B.dy);
+    t.yf = (y - oy) / B.dy - j;
     t.j2 = wrap_idx(j + 1, B.n1);
     t.j1 = wrap_idx(j, B.n1);
-    const int k = (int)floor(z / B.dz);
-    t.zf = z / B.dz - k;
+    const int k = (int)floor((z - oz) / B.dz);
+    t.zf = (z - oz) / B.dz - k;
     t.k2 = wrap_idx(k + 1, B.n2);
     t.k1 = wrap_idx(k, B.n2);
     return t;
 }
 
-// Bucket of a particle: its x-cell i1 = floor(x/dx) when the reference
-// deposits / gathers it (0 <= i1 and i1 + 1 < nx), else nx - 1.
+// Buckets are half x-cells: h = floor(2x/dx), bucket h + 1 for h in
+// [-1, 2nx - 3] (every particle some array of the reference deposits to or
+// gathers from: the sources and most fields sit on the nodes, e_x on nodes
+// shifted by -dx/2), bucket 2nx - 1 for the rest.  floor(h/2) is the node
+// x-cell floor(x/dx) exactly (2x/dx is exactly twice x/dx).
 __device__ __forceinline__ int bucket_of(double x, const BunchDev &B) {
-    const double t = floor(x / B.dx);
-    return (t >= 0.0 && t < (double)(B.nx - 1)) ? (int)t : B.nx - 1;
+    const double h = floor(2.0 * x / B.dx);
+    return (h >= -1.0 && h <= (double)(2 * B.nx - 3)) ? (int)h + 1 : 2 * B.nx - 1;
+}
+
+// Particle range of the buckets [b0, b1] (clipped to the valid buckets).
+__device__ __forceinline__ void bucket_range(const BunchDev &B, int b0, int b1, int &p0, int &np) {
+    b0 = b0 < 0 ? 0 : b0;
+    b1 = b1 > 2 * B.nx - 2 ? 2 * B.nx - 2 : b1;
+    if (b1 < b0) {
+        p0 = np = 0;
+        return;
+    }
+    p0 = B.start[b0];
+    np = B.start[b1 + 1] - p0;
 }
 
 // Slot of this lane in counter ctr[b], one atomic per distinct b in the
@@ -93,33 +109,27 @@ __global__ void k_bunch_zero(double *planes, size_t n, const Ctl *ctl) {
         planes[t] = 0.0;
 }
 
-// rho_b(i), jx_b(i) from bucket i (slice i is their i1: weight 1 - x_frac)
-// and bucket i-1 (slice i is their i2: weight x_frac).
+// rho_b(i), jx_b(i) from the particles whose x-cell i1 = floor(x/dx) is i
+// (weight 1 - x_frac) or i - 1 (weight x_frac), inside the reference's x cut
+// 0 <= i1, i1 + 1 < nx (src/array_utils.cpp:290-338): half-buckets 2i-1..2i+2.
 __global__ __launch_bounds__(kBT) void k_bunch_src(BunchDev B, double *rho, double *jx, const Ctl *ctl, int slot,
                                                    int slice) {
     if (ctl && ctl->stop_acc) return;
     const int i = slice_index(ctl, slot, slice);
     if (i < 0 || i >= B.nx) return;
-    int a0 = 0, na = 0, b0 = 0, nb = 0;
-    if (i <= B.nx - 2) {
-        a0 = B.start[i];
-        na = B.start[i + 1] - a0;
-    }
-    if (i >= 1) {
-        b0 = B.start[i - 1];
-        nb = B.start[i] - b0;
-    }
+    int p0, np;
+    bucket_range(B, 2 * i - 1, 2 * i + 2, p0, np);
     const int n2 = B.n2;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na + nb; t += gridDim.x * blockDim.x) {
-        const bool front = t < na;
-        const int p = front ? a0 + t : b0 + (t - na);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x) {
+        const int p = p0 + t;
         const double x = B.x[p];
         const int i1 = (int)floor(x / B.dx);
+        if (i1 < 0 || i1 + 1 >= B.nx || (i1 != i && i1 + 1 != i)) continue;
         const double x_frac = x / B.dx - i1;
         const Tw w = tw_of(B.y[p], B.z[p], B);
         const double v = B.q[p];
         const double vj = v * B.px[p] / B.gamma[p];
-        const double wx = front ? (1 - x_frac) : x_frac;
+        const double wx = i1 == i ? (1 - x_frac) : x_frac;
         const size_t c11 = (size_t)w.j1 * n2 + w.k1, c21 = (size_t)w.j2 * n2 + w.k1;
         const size_t c12 = (size_t)w.j1 * n2 + w.k2, c22 = (size_t)w.j2 * n2 + w.k2;
         unsafeAtomicAdd(&rho[c11], v * wx * (1 - w.yf) * (1 - w.zf));
@@ -153,46 +163,45 @@ struct Planes5 {
     const double *f[5];  // ex, ey, ez, by, bz of the slice
 };
 
-// Once slice i's fields are final: bucket i - 1 (slice i is their i2) stores
-// its upper half in the field arrays; bucket i (slice i is their i1) adds its
-// lower half to the upper half stored at slice i + 1, which completes its
-// gather, and derives chi (src/System_3d.cpp:843-860).  No plane of an
-// earlier slice is kept.
+// Once slice i's fields are final, every particle for which slice i is a face
+// of its x-cell in some field array takes that half of the field's trilinear
+// gather.  Each field is read with its array's origin (src/System_3d.cpp:179-183:
+// e_x (-dx/2, 0, 0), e_y and B_z (0, dy/2, 0), e_z and B_y (0, 0, dz/2)), so
+// its x-cell i1 = floor((x - o_x)/dx) is the node cell or the one above.  The
+// sweep reaches the upper face i1 + 1 first: that half is stored in the field
+// array, and the lower half at slice i1 is added to it.  chi
+// (src/System_3d.cpp:843-860) is recomputed at every visit; the last visit
+// completes all five fields.  Half-buckets 2i-3..2i+2 hold every such particle
+// (with a margin for the rounding of (x + dx/2)/dx).
 __global__ __launch_bounds__(kBT) void k_bunch_gather(BunchDev B, Planes5 cur, const Ctl *ctl, int slot,
                                                       int slice) {
     if (ctl && ctl->stop_acc) return;
     const int i = slice_index(ctl, slot, slice);
     if (i < 0 || i >= B.nx) return;
-    int a0 = 0, na = 0, b0 = 0, nb = 0;
-    if (i <= B.nx - 2) {
-        a0 = B.start[i];
-        na = B.start[i + 1] - a0;
-    }
-    if (i >= 1) {
-        b0 = B.start[i - 1];
-        nb = B.start[i] - b0;
-    }
+    int p0, np;
+    bucket_range(B, 2 * i - 3, 2 * i + 2, p0, np);
     const int n2 = B.n2;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < na + nb; t += gridDim.x * blockDim.x) {
-        const bool low = t < na;
-        const int p = low ? a0 + t : b0 + (t - na);
-        const double x = B.x[p];
-        const int i1 = (int)floor(x / B.dx);
-        const double x_frac = x / B.dx - i1;
-        const Tw w = tw_of(B.y[p], B.z[p], B);
+    const double hx = 0.5 * B.dx, hy = 0.5 * B.dy, hz = 0.5 * B.dz;
+    const double ox[5] = {-hx, 0.0, 0.0, 0.0, 0.0}, oy[5] = {0.0, hy, 0.0, 0.0, hy}, oz[5] = {0.0, 0.0, hz, hz, 0.0};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x) {
+        const int p = p0 + t;
+        const double x = B.x[p], y = B.y[p], z = B.z[p];
         double *out[5] = {B.ex, B.ey, B.ez, B.by, B.bz};
-        if (!low) {
-#pragma unroll
-            for (int f = 0; f < 5; ++f) out[f][p] = half_hi(cur.f[f], w, n2, x_frac);
-            continue;
-        }
-        double v[5];
+        bool touched = false;
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-            v[f] = half_lo(cur.f[f], w, n2, x_frac) + out[f][p];
-            out[f][p] = v[f];
+            const int i1 = (int)floor((x - ox[f]) / B.dx);
+            if (i1 < 0 || i1 + 1 >= B.nx || (i1 != i && i1 + 1 != i)) continue;
+            const double x_frac = (x - ox[f]) / B.dx - i1;
+            const Tw w = tw_of(y, z, B, oy[f], oz[f]);
+            if (i1 + 1 == i)
+                out[f][p] = half_hi(cur.f[f], w, n2, x_frac);
+            else
+                out[f][p] = half_lo(cur.f[f], w, n2, x_frac) + out[f][p];
+            touched = true;
         }
-        const double ex = v[0], ey = v[1], ez = v[2], by = v[3], bz = v[4];
+        if (!touched) continue;
+        const double ex = B.ex[p], ey = B.ey[p], ez = B.ez[p], by = B.by[p], bz = B.bz[p];
         const double px = B.px[p], py = B.py[p], pz = B.pz[p], g = B.gamma[p];
         const double fx = g * ex + py * bz - pz * by;
         const double fy = g * ey - px * bz;
@@ -328,8 +337,8 @@ __global__ void k_bunch_dep3d(BunchDev B, double *rho3, double *jx3) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= B.np) return;
     const double x = B.x[p];
-    if (bucket_of(x, B) == B.nx - 1) return;
     const int i1 = (int)floor(x / B.dx), i2 = i1 + 1;
+    if (i1 < 0 || i2 >= B.nx) return;
     const double x_frac = x / B.dx - i1;
     const Tw w = tw_of(B.y[p], B.z[p], B);
     const int n2 = B.n2;
@@ -387,7 +396,7 @@ int alloc_set(BunchDev &d, size_t n) {
 // Sort set S (np particles) into b->cur by bucket; key and count already
 // filled by the caller's kernel on S.
 int bucket_sort(Bunch *b, BunchDev &S, hipStream_t s) {
-    BLAUNCH(k_bunch_scan, 1, 1024, (const unsigned *)b->count, b->start, (unsigned *)b->cursor, b->nx);
+    BLAUNCH(k_bunch_scan, 1, 1024, (const unsigned *)b->count, b->start, (unsigned *)b->cursor, 2 * b->nx);
     BLAUNCH(k_bunch_scatter, bblk(b->np), kBT, S, b->cur, (const int *)b->key, (unsigned *)b->cursor);
     return QP_OK;
 }
@@ -478,15 +487,15 @@ int qp_bunch_create(qp_ctx ctx, int nx, int ny, int nz, double dx, double dy, do
         return st;
     };
     if (hipSetDevice(ctx->device) != hipSuccess) return fail(QP_ERR_HIP);
-    if (hipMalloc(&b->start, sizeof(int) * (nx + 1)) != hipSuccess ||
-        hipMalloc(&b->count, sizeof(unsigned) * nx) != hipSuccess ||
-        hipMalloc(&b->cursor, sizeof(unsigned) * nx) != hipSuccess ||
+    if (hipMalloc(&b->start, sizeof(int) * (2 * nx + 1)) != hipSuccess ||
+        hipMalloc(&b->count, sizeof(unsigned) * 2 * nx) != hipSuccess ||
+        hipMalloc(&b->cursor, sizeof(unsigned) * 2 * nx) != hipSuccess ||
         hipMalloc(&b->planes, sizeof(double) * 2 * b->nn()) != hipSuccess) {
         set_error("qp_bunch_create: device allocation failed");
         return fail(QP_ERR_ALLOC);
     }
     hipStream_t s = b->stream();
-    if (hipMemsetAsync(b->start, 0, sizeof(int) * (nx + 1), s) != hipSuccess ||
+    if (hipMemsetAsync(b->start, 0, sizeof(int) * (2 * nx + 1), s) != hipSuccess ||
         hipMemsetAsync(b->planes, 0, sizeof(double) * 2 * b->nn(), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return fail(QP_ERR_HIP);
@@ -537,7 +546,7 @@ int qp_bunch_set_particles(qp_bunch h, size_t np, const double *x, const double 
     b->np = np;
     b->cur.np = b->tmp.np = (int)np;
     if (np == 0) {
-        QP_HIP_CHECK(hipMemsetAsync(b->start, 0, sizeof(int) * (b->nx + 1), s));
+        QP_HIP_CHECK(hipMemsetAsync(b->start, 0, sizeof(int) * (2 * b->nx + 1), s));
         QP_HIP_CHECK(hipStreamSynchronize(s));
         return QP_OK;
     }
@@ -550,7 +559,7 @@ int qp_bunch_set_particles(qp_bunch h, size_t np, const double *x, const double 
     for (size_t p = 0; p < np; ++p) ids[p] = (int)p;
     QP_HIP_CHECK(hipMemcpy(T.id, ids.data(), sizeof(int) * np, hipMemcpyHostToDevice));
     for (double *f : {T.ex, T.ey, T.ez, T.by, T.bz, T.chi}) QP_HIP_CHECK(hipMemsetAsync(f, 0, sizeof(double) * np, s));
-    QP_HIP_CHECK(hipMemsetAsync(b->count, 0, sizeof(unsigned) * b->nx, s));
+    QP_HIP_CHECK(hipMemsetAsync(b->count, 0, sizeof(unsigned) * 2 * b->nx, s));
     BLAUNCH(k_bunch_key, bblk(np), kBT, T, b->key, (unsigned *)b->count);
     QP_TRY(bucket_sort(b, T, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
@@ -589,7 +598,7 @@ int qp_bunch_push(qp_bunch h, double dt) {
     Bunch *b = &h->b;
     if (!b->np) return QP_OK;
     hipStream_t s = b->stream();
-    QP_HIP_CHECK(hipMemsetAsync(b->count, 0, sizeof(unsigned) * b->nx, s));
+    QP_HIP_CHECK(hipMemsetAsync(b->count, 0, sizeof(unsigned) * 2 * b->nx, s));
     BLAUNCH(k_bunch_push, bblk(b->np), kBT, b->cur, b->tmp, dt, b->key, (unsigned *)b->count);
     QP_TRY(bucket_sort(b, b->tmp, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));

@@ -186,10 +186,73 @@ System_3d::System_3d(System_parameters &params, std::ostream &out)
     wp.resort_every = device.resort_every;
     check(qp_wake_create(qp_host_context(), &wp, &wake), "qp_wake_create");
     check(qp_wake_set_a_sqr(wake, a_sqr.data()), "qp_wake_set_a_sqr");
+    if (!bunch_particles.empty() && !qed && device.bunch_on_device) {
+        check(qp_bunch_create(qp_host_context(), n.x, n.y, n.z, d.x, d.y, d.z, plasma_units.field_schwinger,
+                              &dbunch),
+              "qp_bunch_create");
+        upload_bunch();
+        check(qp_wake_attach_bunch(wake, dbunch), "qp_wake_attach_bunch");
+        out << "Driver bunch: on the device (streamed per slice)" << std::endl;
+    }
+}
+
+// The bunch in qp_bunch's SoA form: q = species charge * n, the species'
+// charge-to-mass ratio and photon flag per particle.
+void System_3d::upload_bunch() {
+    const size_t np = bunch_particles.size();
+    std::vector<double> x(np), y(np), z(np), px(np), py(np), pz(np), g(np), q(np), cmr(np);
+    std::vector<int> ph(np);
+    for (size_t i = 0; i < np; ++i) {
+        const bunch_particle_3d &p = bunch_particles[i];
+        const auto &sp = species[p.species_id];
+        x[i] = p.x;
+        y[i] = p.y;
+        z[i] = p.z;
+        px[i] = p.px;
+        py[i] = p.py;
+        pz[i] = p.pz;
+        g[i] = p.gamma;
+        q[i] = sp.charge * p.n;
+        cmr[i] = sp.charge_to_mass_ratio;
+        ph[i] = sp.is_photon ? 1 : 0;
+    }
+    check(qp_bunch_set_particles(dbunch, np, x.data(), y.data(), z.data(), px.data(), py.data(), pz.data(),
+                                 g.data(), q.data(), cmr.data(), ph.data()),
+          "qp_bunch_set_particles");
+}
+
+// State and gathered fields back into bunch_particles (upload order), for the writer.
+void System_3d::download_bunch() {
+    const size_t np = bunch_particles.size();
+    std::vector<double> v[13];
+    for (auto &a : v) a.resize(np);
+    check(qp_bunch_get_particles(dbunch, np, v[0].data(), v[1].data(), v[2].data(), v[3].data(), v[4].data(),
+                                 v[5].data(), v[6].data()),
+          "qp_bunch_get_particles");
+    check(qp_bunch_get_fields(dbunch, np, v[7].data(), v[8].data(), v[9].data(), v[10].data(), v[11].data(),
+                              v[12].data()),
+          "qp_bunch_get_fields");
+    for (size_t i = 0; i < np; ++i) {
+        bunch_particle_3d &p = bunch_particles[i];
+        p.x = v[0][i];
+        p.y = v[1][i];
+        p.z = v[2][i];
+        p.px = v[3][i];
+        p.py = v[4][i];
+        p.pz = v[5][i];
+        p.gamma = v[6][i];
+        p.ex = v[7][i];
+        p.ey = v[8][i];
+        p.ez = v[9][i];
+        p.by = v[10][i];
+        p.bz = v[11][i];
+        p.chi = v[12][i];
+    }
 }
 
 System_3d::~System_3d() {
     if (wake) qp_wake_destroy(wake);
+    if (dbunch) qp_bunch_destroy(dbunch);
 }
 
 const array3d &System_3d::field3d(const std::string &name) const {
@@ -209,15 +272,25 @@ const array3d &System_3d::field3d(const std::string &name) const {
 void System_3d::run() {
     for (int ti = 0; ti < time_iterations; ++ti) {
         out << "Iteration " << ti << std::endl << "Depositing bunch densities..." << std::endl;
-        deposit_bunch_densities();
+        if (!dbunch) {
+            deposit_bunch_densities();
+        } else if (output_parameters.output3d) {  // the device streams its own sources per slice
+            check(qp_bunch_deposit(dbunch, rho_bunch.data(), jx_bunch.data()), "qp_bunch_deposit");
+        }
         out << "Solving wakefield" << std::flush;
         Output_writer writer(output_parameters, ti);
         solve_wakefield(ti, writer);
         out << "Depositing fields to particles..." << std::endl;
-        deposit_wakefield_to_bunch();
+        if (dbunch) {
+            download_bunch();  // the fields were gathered during the pass
+        } else {
+            deposit_wakefield_to_bunch();
+        }
         writer.write_bunch_parameters(bunch_particles, d.x * d.y * d.z * plasma_units.number_density_norm);
         out << "Advancing particles..." << std::endl;
-        if (ti < time_iterations - 1) {
+        if (dbunch) {
+            if (ti < time_iterations - 1) check(qp_bunch_push(dbunch, dt), "qp_bunch_push");
+        } else if (ti < time_iterations - 1) {
             std::vector<bunch_particle_3d> photons;
             double maxp = 0.0;
             advance_bunch_particles(photons, maxp);
@@ -271,7 +344,7 @@ void System_3d::solve_wakefield(int /*iteration*/, Output_writer &writer) {
     init_wake_particles();
     check(qp_wake_set_particles(wake, wy.size(), wy.data(), wz.data(), wpy.data(), wpz.data(), wn.data()),
           "qp_wake_set_particles");
-    if (bunch_particles.empty())  // no [bunch]: the sources are zero (src/System_3d.cpp:926-946)
+    if (bunch_particles.empty() || dbunch)  // no [bunch], or the attached device bunch streams them
         check(qp_wake_set_bunch_sources(wake, nullptr, nullptr), "qp_wake_set_bunch_sources");
     else
         check(qp_wake_set_bunch_sources(wake, rho_bunch.data(), jx_bunch.data()), "qp_wake_set_bunch_sources");

@@ -70,6 +70,9 @@ private:
     std::mt19937_64 qed_rng;
 
     qp_wake wake = nullptr;
+    qp_bunch dbunch = nullptr;  // driver bunch on the device (no QED), attached to the wake
+    void upload_bunch();
+    void download_bunch();
     Output_writer *cur_writer = nullptr;
     int stop_slice_ = -1;
     double last_wake_ms_ = 0;

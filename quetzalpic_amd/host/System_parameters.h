@@ -31,6 +31,7 @@ struct Device_parameters {
     int resort_every = 0;
     double thermal_momentum = 0.0;  // extension: N(0, sigma) initial p_y, p_z
     unsigned long long seed = 1234;
+    int bunch_on_device = 1;        // driver bunch in qp_bunch (without QED) or on the host
 };
 
 struct System_parameters {

@@ -8,7 +8,7 @@ advance_bunch_particles without QED, src/System_3d.cpp:843-946, pinned on the
 CPU in tests/test_oracle_bunch.py):
   * the streamed wake equals the wake fed the oracle's 3D source arrays;
   * the gathered fields and chi equal the oracle's gather of the 3D field
-    arrays the same pass delivers;
+    arrays the same pass delivers, each array with its staggered origin;
   * the push equals the oracle's push;
   * three beam time steps (deposit -> wake -> gather -> push) equal the oracle
     chain (FP64 path), and the mixed path follows the FP64 one.
@@ -107,8 +107,10 @@ def test_gathered_fields_match_the_oracle(qp_ctx, oracle, prec):
         scale = np.abs(want[f]).max()
         assert scale > 0, f
         assert np.abs(got[f] - want[f]).max() <= 1e-12 * scale, (f, np.abs(got[f] - want[f]).max() / scale)
-    outside = (b["x"] < 0) | (np.floor(b["x"] / g.dx) >= g.nx - 1)
+    # no array reaches these (e_x, staggered by -dx/2, still reaches x in [-dx/2, 0))
+    outside = (b["x"] < -0.5 * g.dx) | (np.floor(b["x"] / g.dx) >= g.nx - 1)
     assert outside.sum() >= 20 and not any(got[f][outside].any() for f in FIELDS3)
+    assert got["ex"][(b["x"] >= -0.5 * g.dx) & (b["x"] < 0)].all()
     wk.close()
     db.close()
 

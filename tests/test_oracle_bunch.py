@@ -4,7 +4,8 @@ parameter (src/System_3d.cpp:843-860) and the explicit push
 (src/System_3d.cpp:862-890).
 
 Closed forms: charge conservation and the x-cut of the deposit, adjointness of
-gather and deposit, exact reproduction of a trilinear field, node and
+gather and deposit, exact reproduction of a trilinear field on node and
+staggered arrays (src/System_3d.cpp:179-183) and through the bunch gather, node and
 periodic-seam cases, free streaming and constant-force momentum gain, the
 photon branch, and chi for pure transverse E and pure B.  Grids have
 ny != nz and dy != dz so that a transposed axis fails.
@@ -49,19 +50,44 @@ def test_gather_is_the_adjoint_of_deposit(oracle, g):
         assert abs((f * a).sum() - v * oracle.gather3d(g, x, y, z, f)) <= 1e-12 * abs(v) * np.abs(f).max()
 
 
-def test_gather_reproduces_trilinear_fields(oracle, g):
+@pytest.mark.parametrize("origin", [(0.0, 0.0, 0.0), "ex", "ey", "ez"])
+def test_gather_reproduces_trilinear_fields(oracle, g, origin):
+    """Node (i, j, k) of an array with origin o sits at o + (i dx, j dy, k dz)
+    (the staggered field arrays, src/System_3d.cpp:179-183)."""
+    o = oracle.field_origins(g)[origin] if isinstance(origin, str) else origin
     i, j, k = np.meshgrid(np.arange(g.nx), np.arange(g.ny), np.arange(g.nz), indexing="ij")
-    X, Y, Z = i * g.dx, j * g.dy, k * g.dz
-    f = 0.3 + 1.7 * X - 0.6 * Y + 2.2 * Z + 0.5 * X * Y - 0.8 * Y * Z + 0.25 * X * Z + 0.1 * X * Y * Z
+    X, Y, Z = o[0] + i * g.dx, o[1] + j * g.dy, o[2] + k * g.dz
+
+    def fun(x, y, z):
+        return 0.3 + 1.7 * x - 0.6 * y + 2.2 * z + 0.5 * x * y - 0.8 * y * z + 0.25 * x * z + 0.1 * x * y * z
+
+    f = fun(X, Y, Z)
     rng = np.random.default_rng(3)
     for _ in range(40):
-        x = rng.uniform(0, (g.nx - 1) * g.dx)
-        y = rng.uniform(0, (g.ny - 1) * g.dy)   # away from the periodic seam
-        z = rng.uniform(0, (g.nz - 1) * g.dz)
-        want = 0.3 + 1.7 * x - 0.6 * y + 2.2 * z + 0.5 * x * y - 0.8 * y * z + 0.25 * x * z + 0.1 * x * y * z
-        assert abs(oracle.gather3d(g, x, y, z, f) - want) <= 1e-12 * np.abs(f).max()
-    assert oracle.gather3d(g, -0.1, 1.0, 1.0, f) == 0.0
-    assert oracle.gather3d(g, (g.nx - 1) * g.dx + 0.01, 1.0, 1.0, f) == 0.0
+        x = o[0] + rng.uniform(0, (g.nx - 1) * g.dx)
+        y = o[1] + rng.uniform(0, (g.ny - 1) * g.dy)   # away from the periodic seam
+        z = o[2] + rng.uniform(0, (g.nz - 1) * g.dz)
+        assert abs(oracle.gather3d(g, x, y, z, f, o) - fun(x, y, z)) <= 1e-12 * np.abs(f).max()
+    assert oracle.gather3d(g, o[0] - 0.01, 1.0, 1.0, f, o) == 0.0
+    assert oracle.gather3d(g, o[0] + (g.nx - 1) * g.dx + 0.01, 1.0, 1.0, f, o) == 0.0
+
+
+def test_bunch_fields_use_the_staggered_origins(oracle, g):
+    """Each field of the bunch gather is read with its own array origin: a field
+    linear in the staggered coordinate comes back exactly at the particle."""
+    rng = np.random.default_rng(9)
+    b = bunch(40, rng)
+    b["x"] = rng.uniform(0.2, (g.nx - 2) * g.dx, 40)
+    b["y"] = rng.uniform(0.5 * g.dy, (g.ny - 2) * g.dy, 40)
+    b["z"] = rng.uniform(0.5 * g.dz, (g.nz - 2) * g.dz, 40)
+    i, j, k = np.meshgrid(np.arange(g.nx), np.arange(g.ny), np.arange(g.nz), indexing="ij")
+    f3 = {}
+    for name, o in oracle.field_origins(g).items():
+        f3[name] = 0.2 + 0.9 * (o[0] + i * g.dx) - 0.3 * (o[1] + j * g.dy) + 0.7 * (o[2] + k * g.dz)
+    out = oracle.bunch_fields(g, b, f3, 1.0)
+    want = 0.2 + 0.9 * b["x"] - 0.3 * b["y"] + 0.7 * b["z"]
+    for name in f3:
+        assert np.abs(out[name] - want).max() < 1e-12, name
 
 
 def test_node_and_seam_cases(oracle, g):
