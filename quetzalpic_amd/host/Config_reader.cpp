@@ -128,6 +128,13 @@ void Config_reader::init_laser() {
     params.laser_r0[0] = x0;
     params.laser_r0[1] = y0;
     params.laser_r0[2] = z0;
+    // extension keys: an evolving laser (no reference counterpart)
+    params.laser_evolve = read_value<bool>("evolve", false, t);
+    if (params.laser_evolve) {
+        params.laser_k0 = read_value<double>("k0", t);
+        if (!(params.laser_k0 > 0)) throw Config_exception("[laser] k0 must be positive");
+        params.laser_chi_iterations = read_value<int>("chi_iterations", 1, t);
+    }
 }
 
 // src/Config_reader.cpp:138-186
@@ -226,9 +233,13 @@ void Config_reader::init_plasma_profile() {
         const double y0 = read_value<double>("y0", 0.5 * params.l.y, t);
         const double z0 = read_value<double>("z0", 0.5 * params.l.z, t);
         params.plasma_profile = channel2d(base, curv, y0, z0, -1.0);
+    } else if (profile == "ramp") {
+        const double slope = read_value<double>("slope", t);
+        const double y0 = read_value<double>("y0", 0.5 * params.l.y, t);
+        params.plasma_profile = ramp2d(slope, y0, -1.0);
     } else {
         throw Config_exception("Plasma profile \"" + profile +
-                               "\" is not supported, use \"powerlaw\", \"constant\" or \"channel\".");
+                               "\" is not supported, use \"powerlaw\", \"constant\", \"channel\" or \"ramp\".");
     }
 }
 

@@ -186,6 +186,17 @@ System_3d::System_3d(System_parameters &params, std::ostream &out)
     wp.resort_every = device.resort_every;
     check(qp_wake_create(qp_host_context(), &wp, &wake), "qp_wake_create");
     check(qp_wake_set_a_sqr(wake, a_sqr.data()), "qp_wake_set_a_sqr");
+    if (params.laser_evolve) {
+        // a = sqrt(2 <a^2>), real: the deck's laser as the initial envelope
+        check(qp_envelope_create(qp_host_context(), n.x, n.y, n.z, d.x, d.y, d.z, params.laser_k0, &env),
+              "qp_envelope_create");
+        std::vector<double> re((size_t)n.x * n.y * n.z);
+        for (size_t t = 0; t < re.size(); ++t) re[t] = std::sqrt(2.0 * a_sqr.data()[t]);
+        check(qp_envelope_set(env, re.data(), nullptr), "qp_envelope_set");
+        check(qp_wake_attach_envelope(wake, env), "qp_wake_attach_envelope");
+        env_chi_iterations = params.laser_chi_iterations;
+        out << "Laser envelope: evolving, k0 = " << params.laser_k0 << std::endl;
+    }
     if (!bunch_particles.empty() && !qed && device.bunch_on_device) {
         check(qp_bunch_create(qp_host_context(), n.x, n.y, n.z, d.x, d.y, d.z, plasma_units.field_schwinger,
                               &dbunch),
@@ -253,6 +264,7 @@ void System_3d::download_bunch() {
 System_3d::~System_3d() {
     if (wake) qp_wake_destroy(wake);
     if (dbunch) qp_bunch_destroy(dbunch);
+    if (env) qp_envelope_destroy(env);
 }
 
 const array3d &System_3d::field3d(const std::string &name) const {
@@ -277,6 +289,7 @@ void System_3d::run() {
         } else if (output_parameters.output3d) {  // the device streams its own sources per slice
             check(qp_bunch_deposit(dbunch, rho_bunch.data(), jx_bunch.data()), "qp_bunch_deposit");
         }
+        if (env) check(qp_envelope_get_a_sqr(env, a_sqr.data()), "qp_envelope_get_a_sqr");  // aSqr output
         out << "Solving wakefield" << std::flush;
         Output_writer writer(output_parameters, ti);
         solve_wakefield(ti, writer);
@@ -287,6 +300,10 @@ void System_3d::run() {
             deposit_wakefield_to_bunch();
         }
         writer.write_bunch_parameters(bunch_particles, d.x * d.y * d.z * plasma_units.number_density_norm);
+        if (env && ti < time_iterations - 1) {
+            out << "Advancing the laser envelope..." << std::endl;
+            check(qp_envelope_advance(env, dt, 1, env_chi_iterations, 1, nullptr), "qp_envelope_advance");
+        }
         out << "Advancing particles..." << std::endl;
         if (dbunch) {
             if (ti < time_iterations - 1) check(qp_bunch_push(dbunch, dt), "qp_bunch_push");

@@ -71,6 +71,8 @@ private:
 
     qp_wake wake = nullptr;
     qp_bunch dbunch = nullptr;  // driver bunch on the device (no QED), attached to the wake
+    qp_envelope env = nullptr;  // evolving laser envelope ([laser] evolve), attached to the wake
+    int env_chi_iterations = 1;
     void upload_bunch();
     void download_bunch();
     Output_writer *cur_writer = nullptr;

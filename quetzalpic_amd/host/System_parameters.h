@@ -47,6 +47,10 @@ struct System_parameters {
     // Gaussian laser parameters when the laser is Gaussian (device evaluation)
     bool laser_gaussian = false;
     double laser_a0 = 0, laser_sigma[3] = {0, 0, 0}, laser_r0[3] = {0, 0, 0};
+    // Extension: the laser envelope evolves between beam steps ([laser] evolve, k0, chi_iterations)
+    bool laser_evolve = false;
+    double laser_k0 = 0;
+    int laser_chi_iterations = 1;
     Output_parameters output_parameters;
     std::vector<Bunch_parameters> bunch_parameters_array;
     Species_register species;

@@ -52,3 +52,10 @@ profile2d channel2d(double base, double curvature, double x0, double y0, double 
         return amplitude * (base + curvature * r2);
     };
 }
+
+profile2d ramp2d(double slope, double y0, double amplitude) {
+    return [=](double y, double) {
+        const double v = 1.0 + slope * (y - y0);
+        return amplitude * (v > 0.0 ? v : 0.0);
+    };
+}

@@ -22,3 +22,5 @@ profile2d powerlaw2d(double power, double radius, double x0, double y0, double m
 // Extension (no reference counterpart): amplitude * (base + curvature r^2), a
 // parabolic density channel.
 profile2d channel2d(double base, double curvature, double x0, double y0, double amplitude = 1.0);
+// Extension: amplitude * max(0, 1 + slope (y - y0)), a linear density ramp across the box.
+profile2d ramp2d(double slope, double y0, double amplitude = 1.0);

@@ -71,3 +71,55 @@ def test_cli_device_bunch_equals_host_bunch(tmp_path):
         b = h5read.read_field(str(runs["host"] / "Bunch_002.h5"), k)
         assert a.shape == b.shape and a.size > 1000
         assert rel_err(a, b)[0] < 1e-9, (k, rel_err(a, b))
+
+
+def run_cli(tmp_path, deck_name):
+    deck = os.path.join(DECKS, deck_name)
+    r = subprocess.run([EXE, deck], cwd=str(tmp_path), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:]
+    return r.stdout
+
+
+def test_cli_evolving_laser(oracle, tmp_path):
+    """[laser] evolve: the envelope starts from the deck's <a^2>, the wake stores
+    chi into it, and one envelope step of dt separates the two beam steps."""
+    from oracle import envelope as envo
+    from quetzalpic_amd import h5read
+    out = run_cli(tmp_path, "laser_evolve_fp64.toml")
+    assert "Laser envelope: evolving" in out
+    g = oracle.grid_from_box(3.0, 20.0, 20.0, 0.1, 0.625, 0.625, K=5)
+    a0 = oracle.gaussian_a_sqr(g, 1.5, 4.0, 6.0, 6.0, 1.5, 10.0, 10.0)
+    f0, f1 = str(tmp_path / "Fields_000.h5"), str(tmp_path / "Fields_001.h5")
+    asq0 = h5read.read_field(f0, "aSqr")
+    assert rel_err(asq0, a0)[0] < 1e-13
+    chi0 = h5read.read_field(f0, "susceptibility")
+    y, z, w = oracle.init_lattice(g, 2, 2)
+    want0, _, _ = oracle.wake_run(g, y, z, w, a0, fields=("chi", "psi"))
+    assert rel_err(chi0, want0["chi"])[0] < 1e-9
+    a1 = envo.advance(np.sqrt(2 * asq0), chi0, 4.0, 10.0, g.dx, g.dy, g.dz, steps=1, m=1)
+    asq1 = h5read.read_field(f1, "aSqr")
+    assert rel_err(asq1, envo.a_sqr(a1))[0] < 1e-10
+    assert rel_err(asq1, asq0)[0] > 1e-3          # the laser did change
+    want1, _, _ = oracle.wake_run(g, y, z, w, asq1, fields=("psi", "ex"))
+    for f in ("psi", "ex"):
+        assert rel_err(h5read.read_field(f1, f), want1[f])[0] < 1e-9, f
+
+
+def test_cli_ramp_profile(oracle, tmp_path):
+    """[plasma] profile = "ramp": n = max(0, 1 + slope (y - ly/2)) on the lattice of
+    init_wake_particles (src/System_3d.cpp:724-748)."""
+    from quetzalpic_amd import h5read
+    run_cli(tmp_path, "ramp_fp64.toml")
+    g = oracle.grid_from_box(3.0, 20.0, 20.0, 0.1, 0.625, 0.625, K=5)
+    ppc = 2
+    yy = (np.arange(ppc * g.ny) + 0.5) * g.dy / ppc
+    zz = (np.arange(ppc * g.nz) + 0.5) * g.dz / ppc
+    y = np.repeat(yy, len(zz))
+    z = np.tile(zz, len(yy))
+    v = -np.maximum(0.0, 1.0 + 0.08 * (y - 10.0))
+    keep = v != 0
+    a = oracle.gaussian_a_sqr(g, 1.0, 4.0, 6.0, 6.0, 1.5, 10.0, 10.0)
+    want, _, _ = oracle.wake_run(g, y[keep], z[keep], v[keep] / ppc / ppc, a, fields=("psi", "ex", "rho"))
+    for f in ("psi", "ex", "rho"):
+        assert rel_err(h5read.read_field(str(tmp_path / "Fields_000.h5"), f), want[f])[0] < 1e-9, f
