@@ -19,6 +19,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FIELD_NAMES_ALL = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
 
 
 def log(*a):
@@ -202,9 +203,9 @@ def calib_copy(mib):
     hip.hipFree(b)
 
 
-def make_wake(capi, ctx, wl, precision, use_graph=1):
+def make_wake(capi, ctx, wl, precision, use_graph=1, async_output=0):
     w = capi.Wake(ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.L, wl.L, K=wl.K, precision=precision,
-                  use_graph=use_graph, resort_every=0)
+                  use_graph=use_graph, resort_every=0, async_output=async_output)
     y, z, wt, py, pz = wl.particles()
     w.set_particles(y, z, wt, py, pz)
     w.set_laser_gaussian(*wl.laser)
@@ -362,6 +363,7 @@ def bench_c4(args, capi, workloads):
         "err_vs_oracle_member": err, "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "envelope_advance": envelope,
+        "output_pass_ms": output_pass,
     }
     print(json.dumps(line), flush=True)
 
@@ -480,6 +482,21 @@ def main():
                             float(np.linalg.norm(d) / np.linalg.norm(fields64[f]))]
             log(f"[bench] mixed vs fp64 GPU (max, rms rel): {err64}")
     wk.close()
+
+    # ---- passes with every slice's eleven output planes handed to a host
+    # callback (outside the timed region): inline vs streamed
+    output_pass = None
+    if not args.no_check:
+        output_pass = {}
+        for mode in (0, 1):
+            wo = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, async_output=mode)
+            wo.run(fields=FIELD_NAMES_ALL)
+            t0 = time.perf_counter()
+            wo.run(fields=FIELD_NAMES_ALL)
+            output_pass["async" if mode else "inline"] = 1e3 * (time.perf_counter() - t0)
+            wo.close()
+        log(f"[bench] pass with all planes to a host callback: inline {output_pass['inline']:.1f} ms, "
+            f"streamed {output_pass['async']:.1f} ms")
 
     # ---- laser-envelope advance on the same grid (outside the timed region)
     envelope = bench_envelope(capi, ctx, wl, max(args.steps, 3))

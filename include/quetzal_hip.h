@@ -84,6 +84,11 @@ typedef struct {
     int resort_every;        /* mixed path: re-bin particles by cell tile every N slices (0 = never) */
     int longitudinal_b;      /* mixed path, extension with no reference counterpart: 1 = also solve
                                 lap_perp B_x = dz j_y - dy j_z per slice (output plane bx) */
+    int async_output;        /* mixed path: 1 = stream the per-slice output: the planes are
+                                snapshotted on the device, copied on a second stream into pinned
+                                host slots and handed to the callback by a writer thread (in
+                                slice order, during the run call) while later slices compute;
+                                the slice step stays a hipGraph.  0 = the callback runs inline. */
 } qp_wake_params;
 
 /* Fields of one slice handed to the per-slice callback (FP64, host memory,

@@ -34,7 +34,7 @@ class WakeParams(ctypes.Structure):
         ("K", ctypes.c_int), ("D", ctypes.c_double),
         ("psi_threshold", ctypes.c_double), ("b_threshold", ctypes.c_double),
         ("precision", ctypes.c_int), ("use_graph", ctypes.c_int), ("resort_every", ctypes.c_int),
-        ("longitudinal_b", ctypes.c_int),
+        ("longitudinal_b", ctypes.c_int), ("async_output", ctypes.c_int),
     ]
 
 
@@ -202,11 +202,12 @@ class Wake:
     """One wake on the device: System_3d::solve_wakefield behind the C-ABI."""
 
     def __init__(self, ctx, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
-                 b_threshold=100.0, precision=QP_PREC_FP64, use_graph=1, resort_every=0, longitudinal_b=0):
+                 b_threshold=100.0, precision=QP_PREC_FP64, use_graph=1, resort_every=0, longitudinal_b=0,
+                 async_output=0):
         self.ctx = ctx
         self.shape = (nx, ny, nz)
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
-                                 precision, use_graph, resort_every, longitudinal_b)
+                                 precision, use_graph, resort_every, longitudinal_b, async_output)
         self.h = ctypes.c_void_p()
         self._env = None
         self._bunch = None
@@ -457,12 +458,12 @@ class WakeBatch:
     member has its own particles, Gaussian laser, fields and stop rule."""
 
     def __init__(self, ctx, nwakes, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
-                 b_threshold=100.0, use_graph=1, longitudinal_b=0):
+                 b_threshold=100.0, use_graph=1, longitudinal_b=0, async_output=0):
         self.ctx = ctx
         self.nwakes = nwakes
         self.shape = (nx, ny, nz)
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
-                                 QP_PREC_MIXED, use_graph, 0, longitudinal_b)
+                                 QP_PREC_MIXED, use_graph, 0, longitudinal_b, async_output)
         self.h = ctypes.c_void_p()
         _check(load().qp_wake_create_batch(ctx.h, ctypes.byref(self.params), nwakes, ctypes.byref(self.h)),
                "qp_wake_create_batch")

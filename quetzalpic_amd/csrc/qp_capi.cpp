@@ -180,6 +180,8 @@ int qp_wake_create_batch(qp_ctx ctx, const qp_wake_params *params, int nwakes, q
     ARG_CHECK(nwakes == 1 || p.resort_every == 0, "qp_wake_create_batch: re-binning is for single wakes only");
     ARG_CHECK(p.longitudinal_b == 0 || p.precision == QP_PREC_MIXED,
               "qp_wake_create: longitudinal_b is an extension of the mixed path");
+    ARG_CHECK(p.async_output == 0 || p.precision == QP_PREC_MIXED,
+              "qp_wake_create: async_output is a mode of the mixed path");
     QP_HIP_CHECK(hipSetDevice(ctx->device));
     auto *h = new (std::nothrow) qp_wake_s();
     if (!h) return QP_ERR_ALLOC;

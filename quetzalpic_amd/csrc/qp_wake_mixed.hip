@@ -20,9 +20,14 @@
 // reads them (:380-397 vs :510-527) and are not recomputed; slice 0 (and
 // K = 0) run S4 + S5 instead of the iterations (:409-411).
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "qp_mixed_kernels.h"
@@ -746,6 +751,16 @@ struct MixedState {
     std::vector<int> h_perm;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     unsigned graph_epoch = 0;  // Wake::inputs_epoch the graphs were captured at
+    // streamed output (async_output): per-slice snapshots of the output planes
+    // and control blocks in two device slots (slice parity), copied on a copy
+    // stream into a ring of pinned host slots that a writer thread hands to the
+    // callback
+    double *snap_dev = nullptr;        // 2 slots
+    double *snap_host = nullptr;       // kHostSlots slots (pinned)
+    size_t snap_words = 0;             // doubles per slot
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_done[2] = {nullptr, nullptr}, ev_copied[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> ev_host;
     int ncb = 0;
     size_t cap = 0;
 };
@@ -784,6 +799,13 @@ void wake_destroy_mixed(Wake *w) {
         if (a) (void)hipFree(a);
     for (auto &g : m->gexec)
         if (g) (void)hipGraphExecDestroy(g);
+    if (m->snap_dev) (void)hipFree(m->snap_dev);
+    if (m->snap_host) (void)hipHostFree(m->snap_host);
+    for (hipEvent_t e : {m->ev_done[0], m->ev_done[1], m->ev_copied[0], m->ev_copied[1]})
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : m->ev_host)
+        if (e) (void)hipEventDestroy(e);
+    if (m->copy_stream) (void)hipStreamDestroy(m->copy_stream);
     delete m;
     w->mixed = nullptr;
 }
@@ -1174,6 +1196,134 @@ int deliver_member(Wake *w, batch_cb cb, void *user, int member, int slice, int 
                          off(ey), off(ez), off(by), off(bz), off(chi), off(bx));
 }
 
+constexpr int kHostSlots = 4;   // pinned host slots of the streamed output
+constexpr int kSnapPlanes = 12; // psi, rho, jx, jy, jz, ex, ey, ez, by, bz, chi, bx
+
+// Slot layout: plane f of member m at (f * nw + m) * nn, then the nw control
+// blocks (rounded up to whole doubles).
+int ensure_snapshots(Wake *w, MixedState *m) {
+    const size_t nn = w->nn(), nw = (size_t)w->nw;
+    const size_t ctl_words = (sizeof(Ctl) * nw + sizeof(double) - 1) / sizeof(double);
+    const size_t words = kSnapPlanes * nw * nn + ctl_words;
+    if (m->snap_dev && m->snap_words == words) return QP_OK;
+    if (m->snap_dev) (void)hipFree(m->snap_dev);
+    if (m->snap_host) (void)hipHostFree(m->snap_host);
+    m->snap_dev = nullptr;
+    m->snap_host = nullptr;
+    QP_HIP_CHECK(hipMalloc(&m->snap_dev, sizeof(double) * 2 * words));
+    QP_HIP_CHECK(hipHostMalloc(&m->snap_host, sizeof(double) * kHostSlots * words, hipHostMallocDefault));
+    m->snap_words = words;
+    if (!m->copy_stream) QP_HIP_CHECK(hipStreamCreateWithFlags(&m->copy_stream, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&m->ev_done[0], &m->ev_done[1], &m->ev_copied[0], &m->ev_copied[1]})
+        if (!*e) QP_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    while ((int)m->ev_host.size() < kHostSlots) {
+        hipEvent_t e;
+        QP_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        m->ev_host.push_back(e);
+    }
+    return QP_OK;
+}
+
+// Snapshot of slice i's output planes and control blocks into device slot d.
+int snapshot(Wake *w, MixedState *m, hipStream_t s, int slot, int d) {
+    const size_t nn = w->nn(), nw = (size_t)w->nw, plane = nn * nw;
+    double *dst = m->snap_dev + (size_t)d * m->snap_words;
+    const double *src[kSnapPlanes] = {w->psi_b[slot], w->rho, w->jx, m->jbuf[1 - slot][0], m->jbuf[1 - slot][1],
+                                      w->ex, w->ey, w->ez, w->by, w->bz, w->chi,
+                                      w->p.longitudinal_b ? w->bx : nullptr};
+    for (int f = 0; f < kSnapPlanes; ++f)
+        if (src[f])
+            QP_HIP_CHECK(hipMemcpyAsync(dst + f * plane, src[f], sizeof(double) * plane, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(dst + kSnapPlanes * plane, w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToDevice, s));
+    return QP_OK;
+}
+
+// View of member mb in a host slot.
+qp_slice_view host_view(const Wake *w, const double *slot, int mb, int slice, int kind) {
+    const size_t nn = w->nn(), nw = (size_t)w->nw;
+    auto pl = [&](int f) { return slot + ((size_t)f * nw + mb) * nn; };
+    qp_slice_view v;
+    v.slice = slice;
+    v.kind = kind;
+    v.psi = pl(0);
+    v.rho = pl(1);
+    v.jx = pl(2);
+    v.jy = pl(3);
+    v.jz = pl(4);
+    v.ex = pl(5);
+    v.ey = pl(6);
+    v.ez = pl(7);
+    v.by = pl(8);
+    v.bz = pl(9);
+    v.chi = pl(10);
+    v.bx = w->p.longitudinal_b ? pl(11) : nullptr;
+    return v;
+}
+
+// The writer side of the streamed output: jobs (slice, host slot) in slice
+// order; slots are handed back as they are consumed.
+struct OutQueue {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::pair<int, int>> jobs;  // (slice, host slot)
+    std::vector<char> slot_free;
+    bool closed = false;
+    std::atomic<bool> all_stopped{false};
+    std::atomic<bool> failed{false};
+    std::string error;
+};
+
+void writer_loop(Wake *w, MixedState *m, OutQueue *q, batch_cb cb, void *user, std::vector<char> *delivered_stop) {
+    (void)hipSetDevice(w->ctx->device);
+    const int nw = w->nw;
+    const size_t plane = w->nn() * nw;
+    const std::vector<double> zero(w->nn(), 0.0);  // the planes a stopped slice does not have
+    for (;;) {
+        std::pair<int, int> job;
+        {
+            std::unique_lock<std::mutex> lk(q->mu);
+            q->cv.wait(lk, [&] { return q->closed || !q->jobs.empty(); });
+            if (q->jobs.empty()) return;
+            job = q->jobs.front();
+            q->jobs.pop_front();
+        }
+        const int h = job.second;
+        if (!q->failed && hipEventSynchronize(m->ev_host[h]) != hipSuccess) {
+            q->error = "streamed output: copy of a slice failed";
+            q->failed = true;
+        }
+        if (!q->failed) {
+            const double *slot = m->snap_host + (size_t)h * m->snap_words;
+            const Ctl *hc = reinterpret_cast<const Ctl *>(slot + kSnapPlanes * plane);
+            int running = 0;
+            for (int mb = 0; mb < nw && !q->failed; ++mb) {
+                if ((*delivered_stop)[mb]) continue;
+                qp_slice_view v;
+                if (hc[mb].stop_acc) {
+                    v = host_view(w, slot, mb, hc[mb].stop_slice, QP_SLICE_STOPPED);
+                    v.psi = v.rho = v.jx = v.jy = v.jz = v.ex = v.ey = v.ez = zero.data();
+                    v.bx = nullptr;
+                    (*delivered_stop)[mb] = 1;
+                } else {
+                    ++running;
+                    v = host_view(w, slot, mb, job.first, QP_SLICE_NORMAL);
+                }
+                const int r = cb(mb, &v, user);
+                if (r != 0) {
+                    q->error = "slice callback returned " + std::to_string(r);
+                    q->failed = true;
+                }
+            }
+            if (!running) q->all_stopped = true;
+        }
+        {
+            std::lock_guard<std::mutex> lk(q->mu);
+            q->slot_free[h] = 1;
+        }
+        q->cv.notify_all();
+    }
+}
+
 }  // namespace
 
 int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
@@ -1230,9 +1380,32 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     MLAUNCH(k_mx_ion, dim3(R.pb, nw), kPT, 0, m->P, R.g, w->rho_ion);
     if (w->bunch) QP_TRY(bunch_clear_fields(w->bunch, s));
 
-    const bool graph_ok = p.use_graph && !cb && !w->prof_class;
+    const bool async = cb && p.async_output && !w->prof_class;
+    const bool graph_ok = p.use_graph && (!cb || async) && !w->prof_class;
     std::vector<char> delivered_stop(nw, 0);
     std::vector<Ctl> hc(nw);
+    OutQueue q;
+    std::thread writer;
+    long seq = 0;
+    if (async) {
+        QP_TRY(ensure_snapshots(w, m));
+        q.slot_free.assign(kHostSlots, 1);
+        writer = std::thread(writer_loop, w, m, &q, cb, user, &delivered_stop);
+    }
+    // ends the writer (after the jobs queued so far) on every exit path
+    struct Join {
+        OutQueue &q;
+        std::thread &t;
+        ~Join() {
+            if (!t.joinable()) return;
+            {
+                std::lock_guard<std::mutex> lk(q.mu);
+                q.closed = true;
+            }
+            q.cv.notify_all();
+            t.join();
+        }
+    } join{q, writer};
     for (int i = p.nx - 1; i >= 0; --i) {
         const int slot = i & 1;
         if (p.resort_every > 0 && i < p.nx - 1 && ((p.nx - 1 - i) % p.resort_every) == 0 && m->P.np)
@@ -1260,7 +1433,32 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
         } else {
             QP_TRY(enqueue_slice(R, s, i));
         }
-        if (cb && i > 0) {
+        if (async && i > 0) {
+            // device slot d was last read by the copy of slice i + 2
+            const int d = i & 1;
+            QP_HIP_CHECK(hipStreamWaitEvent(s, m->ev_copied[d], 0));
+            QP_TRY(snapshot(w, m, s, slot, d));
+            QP_HIP_CHECK(hipEventRecord(m->ev_done[d], s));
+            const int h = (int)(seq++ % kHostSlots);
+            {
+                std::unique_lock<std::mutex> lk(q.mu);
+                q.cv.wait(lk, [&] { return q.slot_free[h] != 0; });
+                q.slot_free[h] = 0;
+            }
+            hipStream_t c = m->copy_stream;
+            QP_HIP_CHECK(hipStreamWaitEvent(c, m->ev_done[d], 0));
+            QP_HIP_CHECK(hipMemcpyAsync(m->snap_host + (size_t)h * m->snap_words,
+                                        m->snap_dev + (size_t)d * m->snap_words, sizeof(double) * m->snap_words,
+                                        hipMemcpyDeviceToHost, c));
+            QP_HIP_CHECK(hipEventRecord(m->ev_copied[d], c));
+            QP_HIP_CHECK(hipEventRecord(m->ev_host[h], c));
+            {
+                std::lock_guard<std::mutex> lk(q.mu);
+                q.jobs.emplace_back(i, h);
+            }
+            q.cv.notify_all();
+            if (q.failed || q.all_stopped) break;
+        } else if (cb && i > 0) {
             QP_HIP_CHECK(hipStreamSynchronize(s));
             QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
             int running = 0;
@@ -1279,6 +1477,18 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
                                       w->bz, w->chi, p.longitudinal_b ? w->bx : nullptr));
             }
             if (!running) break;
+        }
+    }
+    if (async) {  // every streamed slice delivered before the final step
+        {
+            std::lock_guard<std::mutex> lk(q.mu);
+            q.closed = true;
+        }
+        q.cv.notify_all();
+        writer.join();
+        if (q.failed) {
+            set_error(q.error);
+            return QP_ERR_STATE;
         }
     }
     QP_HIP_CHECK(hipStreamSynchronize(s));

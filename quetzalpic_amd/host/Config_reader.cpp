@@ -257,6 +257,7 @@ void Config_reader::init_device() {
     params.device.resort_every = read_value<int>("resort_every", 0, t);
     params.device.thermal_momentum = read_value<double>("thermal_momentum", 0.0, t);
     params.device.seed = (unsigned long long)read_value<int>("seed", 1234, t);
+    params.device.stream_output = read_value<bool>("stream_output", true, t) ? 1 : 0;
     const auto bunch = read_value<std::string>("bunch", std::string("device"), t);
     if (bunch == "device") params.device.bunch_on_device = 1;
     else if (bunch == "host") params.device.bunch_on_device = 0;

@@ -184,6 +184,7 @@ System_3d::System_3d(System_parameters &params, std::ostream &out)
     wp.precision = device.precision;
     wp.use_graph = device.use_graph;
     wp.resort_every = device.resort_every;
+    wp.async_output = (device.precision == QP_PREC_MIXED && device.stream_output) ? 1 : 0;
     check(qp_wake_create(qp_host_context(), &wp, &wake), "qp_wake_create");
     check(qp_wake_set_a_sqr(wake, a_sqr.data()), "qp_wake_set_a_sqr");
     if (params.laser_evolve) {

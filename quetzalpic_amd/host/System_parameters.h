@@ -32,6 +32,7 @@ struct Device_parameters {
     double thermal_momentum = 0.0;  // extension: N(0, sigma) initial p_y, p_z
     unsigned long long seed = 1234;
     int bunch_on_device = 1;        // driver bunch in qp_bunch (without QED) or on the host
+    int stream_output = 1;          // mixed path: per-slice output streamed to a writer thread
 };
 
 struct System_parameters {
