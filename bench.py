@@ -105,7 +105,25 @@ def bench_envelope(capi, ctx, wl, steps, use_graph=1):
     wk.close()
     env.advance(ENV_DTAU, steps=1, chi_iterations=ENV_CHI_IT, use_graph=use_graph)   # graph capture
     ms = env.advance(ENV_DTAU, steps=steps, chi_iterations=ENV_CHI_IT, use_graph=use_graph) / steps
+    # one coupled time step (wake pass + envelope step): two host calls, and the
+    # step enqueued by the pass itself (qp_wake_set_envelope_step)
+    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=use_graph)
+    wk.attach_envelope(env)
+    coupled = {}
+    for mode in ("two_calls", "in_pass"):
+        wk.set_envelope_step(ENV_DTAU if mode == "in_pass" else 0.0, ENV_CHI_IT)
+        for rep in range(2):   # first: capture
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            wk.run()
+            if mode == "two_calls":
+                env.advance(ENV_DTAU, steps=1, chi_iterations=ENV_CHI_IT, use_graph=use_graph)
+            ctx.synchronize()
+            coupled[mode] = 1e3 * (time.perf_counter() - t0)
+    wk.close()
     env.close()
+    log(f"[bench] coupled wake pass + envelope step: two calls {coupled['two_calls']:.1f} ms, "
+        f"in the pass {coupled['in_pass']:.1f} ms")
     pts = wl.nx * wl.ny * wl.nz
     byts = envelope_bytes_per_point(ENV_CHI_IT) * pts
     launches = wl.nx * 3 * (ENV_CHI_IT + 1) + 1
@@ -113,7 +131,7 @@ def bench_envelope(capi, ctx, wl, steps, use_graph=1):
     res = {"k0": ENV_K0, "dtau": ENV_DTAU, "chi_iterations": ENV_CHI_IT, "graph": use_graph, "ms_per_step": ms,
            "slices_per_s": wl.nx / (ms * 1e-3), "launches_per_step": launches,
            "us_per_launch": 1e3 * ms / launches, "algorithmic_bytes_per_step": byts,
-           "achieved_GBs": achieved, "frac_hbm": achieved / HBM_PEAK_GBS}
+           "achieved_GBs": achieved, "frac_hbm": achieved / HBM_PEAK_GBS, "coupled_step_ms": coupled}
     log(f"[bench] envelope: {ms:.2f} ms/step ({launches} launches, {1e3 * ms / launches:.2f} us each), "
         f"{achieved:.0f} GB/s algorithmic")
     return res

@@ -204,6 +204,12 @@ int qp_envelope_advance(qp_envelope e, double dtau, int steps, int chi_iteration
                         double *ms);
 /* Attach e to the wake (NULL detaches). */
 int qp_wake_attach_envelope(qp_wake w, qp_envelope e);
+/* dtau > 0: every qp_wake_run also advances the attached envelope by one step
+ * of dtau with the chi of that pass, as qp_wake_run followed by
+ * qp_envelope_advance(e, dtau, 1, chi_iterations, ...), enqueued on the wake's
+ * stream behind the pass (one coupled time step per call, no host round trip
+ * in between).  dtau = 0 turns it off. */
+int qp_wake_set_envelope_step(qp_wake w, double dtau, int chi_iterations);
 
 /* ---- Driver bunch on the device (System_3d's bunch, src/System_3d.cpp:843-946) --
  * Replaces the host-side bunch of run() (src/System_3d.cpp:199-236):

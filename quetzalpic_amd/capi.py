@@ -97,6 +97,7 @@ def load():
         "qp_envelope_get_a_sqr": ([vp, _dp], ctypes.c_int),
         "qp_envelope_advance": ([vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
         "qp_wake_attach_envelope": ([vp, vp], ctypes.c_int),
+        "qp_wake_set_envelope_step": ([vp, ctypes.c_double, ctypes.c_int], ctypes.c_int),
         "qp_bunch_create": ([vp] + [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.POINTER(vp)], ctypes.c_int),
         "qp_bunch_destroy": ([vp], ctypes.c_int),
         "qp_bunch_set_particles": ([vp, ctypes.c_size_t] + [_dp] * 9 + [ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -300,6 +301,11 @@ class Wake:
         self._env = env
         if env is not None:
             env._wakes.add(self)
+
+    def set_envelope_step(self, dtau, chi_iterations=1):
+        """dtau > 0: every run also advances the attached envelope by one step of dtau
+        with that pass's chi, enqueued behind the pass; 0 turns it off."""
+        _check(load().qp_wake_set_envelope_step(self.h, dtau, chi_iterations), "qp_wake_set_envelope_step")
 
     def attach_bunch(self, bunch):
         """Stream `bunch` through the slice loop: per-slice sources and field gathers (None detaches)."""

@@ -328,6 +328,16 @@ int qp_wake_attach_bunch(qp_wake h, qp_bunch bunch) {
     return QP_OK;
 }
 
+int qp_wake_set_envelope_step(qp_wake h, double dtau, int chi_iterations) {
+    ARG_CHECK(h, "null wake");
+    ARG_CHECK(dtau >= 0 && std::isfinite(dtau), "qp_wake_set_envelope_step: dtau must be >= 0");
+    ARG_CHECK(chi_iterations >= 0 && chi_iterations <= 16, "qp_wake_set_envelope_step: chi_iterations in [0, 16]");
+    Wake *w = &h->w;
+    w->env_dtau = dtau;
+    w->env_chi_it = chi_iterations;
+    return QP_OK;
+}
+
 int qp_wake_attach_envelope(qp_wake h, qp_envelope env) {
     ARG_CHECK(h, "null wake");
     Wake *w = &h->w;

@@ -28,11 +28,16 @@ struct Envelope {
     hipGraphExec_t gexec = nullptr;  // one sweep, captured for dtau = g_dtau, g_iter chi corrections
     double g_dtau = 0.0;
     int g_iter = 0;
+    hipStream_t g_stream = nullptr;  // stream the graph was captured on
 
     size_t nn() const { return (size_t)ny * nz; }
     size_t vol() const { return (size_t)nx * nn(); }
     hipStream_t stream() const { return ctx->stream; }
 };
+
+// One step of the whole envelope after a wake pass (midranges of chi, then the
+// captured sweep), enqueued on s.
+int envelope_step(Envelope *e, double dt, int m, hipStream_t s);
 
 }  // namespace qp
 

@@ -92,48 +92,46 @@ __global__ __launch_bounds__(kET) void k_env_cmid(const double *chi3, double *cm
     }
 }
 
-// Row-pass source of slice i: part `which` (0 re, 1 im) of
-// -(2/dx) d_{i+1} + dt (chi a - lap_perp a) + (dt/2)(chi - c_i) d_i.
+// Row-pass source of slice e: part `which` (0 re, 1 im) of
+// -(2/dx) d_{e+1} + dt (chi a - lap_perp a) + (dt/2)(chi - c_e) d_e.
 struct SrcEnv {
-    const double *a;      // slice i, real part (imaginary part at + vol)
-    size_t vol;
-    const double *chi;    // slice i
-    const double *dprev;  // d_{i+1} (re, im planes) or null at the front slice
-    const double *dcur;   // previous iterate of d_i (re, im planes) or null
-    const double *cmid;   // c_i
-    size_t nn;
-    int n1, n2;
+    const double *a;      // real parts of all slices (imaginary parts at + vol)
+    const double *chi;    // chi of all slices
+    const double *inc;    // 2 slots x (re, im) x nn increments
+    const double *cmid;   // c of all slices
+    size_t vol, nn;
+    int nx, n1, n2, use_dcur;
     double idy2, idz2, dt, c2dx;
+    int e;
     __device__ double operator()(int, int which, int j, int k) const {
-        const double *ac = a + (size_t)which * vol;
+        const double *ac = a + (size_t)which * vol + (size_t)e * nn;
         const int jm = j == 0 ? n1 - 1 : j - 1, jp = j + 1 == n1 ? 0 : j + 1;
         const int km = k == 0 ? n2 - 1 : k - 1, kp = k + 1 == n2 ? 0 : k + 1;
         const size_t t = (size_t)j * n2 + k;
         const double c = ac[t];
         const double lap = (ac[(size_t)jp * n2 + k] - 2.0 * c + ac[(size_t)jm * n2 + k]) * idy2 +
                            (ac[(size_t)j * n2 + kp] - 2.0 * c + ac[(size_t)j * n2 + km]) * idz2;
-        const double x = chi[t];
+        const double x = chi[(size_t)e * nn + t];
         double v = dt * (x * c - lap);
-        if (dprev) v -= c2dx * dprev[(size_t)which * nn + t];
-        if (dcur) v += 0.5 * dt * (x - *cmid) * dcur[(size_t)which * nn + t];
+        if (e != nx - 1) v -= c2dx * inc[(size_t)((e + 1) & 1) * 2 * nn + (size_t)which * nn + t];
+        if (use_dcur) v += 0.5 * dt * (x - cmid[e]) * inc[(size_t)(e & 1) * 2 * nn + (size_t)which * nn + t];
         return v;
     }
 };
 
-// Last row pass: d_i (unnormalised c2r output / (n1 n2)) to the increment
-// buffer, and onto a_i after the last transform of the slice.
+// Last row pass: d_e (unnormalised c2r output / (n1 n2)) to the increment
+// buffer, and onto a_e after the last transform of the slice.
 struct SinkEnv {
-    double *a;     // slice i, real part (imaginary part at + vol); null: store d_i only
-    size_t vol;
-    double *dnew;  // (re, im) planes
-    size_t nn;
-    int n1, n2;
+    double *a;     // all slices (imaginary parts at + vol); null: store d_e only
+    double *inc;
+    size_t vol, nn;
+    int n1, n2, e;
     __device__ void begin(int) {}
     __device__ void operator()(int, int which, int j, int k, double v) const {
         const size_t t = (size_t)j * n2 + k;
         const double d = v / n1 / n2;
-        dnew[(size_t)which * nn + t] = d;
-        if (a) a[(size_t)which * vol + t] += d;
+        inc[(size_t)(e & 1) * 2 * nn + (size_t)which * nn + t] = d;
+        if (a) a[(size_t)which * vol + (size_t)e * nn + t] += d;
     }
 };
 
@@ -222,18 +220,15 @@ int wlaunch_cols_env(const Solver2d &S, double ar, double ai, double hdt, const 
 }
 
 // One transform of slice i: d_i from the bracket with the previous iterate
-// dcur (or none), onto a_i when `last`.
-int env_transform(Envelope *e, int i, double dt, const double *dcur, bool last, hipStream_t s) {
+// (use_dcur) or without it, onto a_i when `last`.
+int env_transform(Envelope *e, int i, double dt, bool use_dcur, bool last, hipStream_t s) {
     const Solver2d &S = e->S;
     const size_t nn = e->nn(), vol = e->vol();
     const double ar = -2.0 / e->dx, ai = 2.0 * e->k0, hdt = 0.5 * dt;
-    const double *dprev = i == e->nx - 1 ? nullptr : e->inc + (size_t)((i + 1) & 1) * 2 * nn;
-    double *dnew = e->inc + (size_t)(i & 1) * 2 * nn;
-    double *a_i = e->a + (size_t)i * nn;
+    SrcEnv src{e->a, e->chi3, e->inc, e->cmid, vol, nn, e->nx, S.n1, S.n2, use_dcur ? 1 : 0,
+               1.0 / (e->dy * e->dy), 1.0 / (e->dz * e->dz), dt, 2.0 / e->dx, i};
+    SinkEnv snk{last ? e->a : nullptr, e->inc, vol, nn, S.n1, S.n2, i};
     const double *cmid = e->cmid + i;
-    SrcEnv src{a_i, vol, e->chi3 + (size_t)i * nn, dprev, dcur, cmid, nn, S.n1, S.n2, 1.0 / (e->dy * e->dy),
-               1.0 / (e->dz * e->dz), dt, 2.0 / e->dx};
-    SinkEnv snk{last ? a_i : nullptr, vol, dnew, nn, S.n1, S.n2};
     if (S.wave) {
         QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[1], 2, s, (const Ctl *)nullptr, 0));
         QP_TRY(wlaunch_cols_env(S, ar, ai, hdt, cmid, s));
@@ -256,11 +251,9 @@ int env_transform(Envelope *e, int i, double dt, const double *dcur, bool last, 
 // One step of the whole envelope (every slice, then <a^2>); m corrections of
 // the (chi - c) d term per slice.  c_i are set before the first step.
 int env_sweep(Envelope *e, double dt, int m, hipStream_t s) {
-    const size_t nn = e->nn(), vol = e->vol();
-    for (int i = e->nx - 1; i >= 0; --i) {
-        const double *dself = e->inc + (size_t)(i & 1) * 2 * nn;
-        for (int it = 0; it <= m; ++it) QP_TRY(env_transform(e, i, dt, it ? dself : nullptr, it == m, s));
-    }
+    const size_t vol = e->vol();
+    for (int i = e->nx - 1; i >= 0; --i)
+        for (int it = 0; it <= m; ++it) QP_TRY(env_transform(e, i, dt, it > 0, it == m, s));
     ELAUNCH(k_env_asq, dim3(eblk(vol)), kET, 0, (const double *)e->a, e->asq3, vol);
     return QP_OK;
 }
@@ -271,6 +264,37 @@ void drop_graph(Envelope *e) {
 }
 
 }  // namespace
+
+// ------------------------------------------------------- wake-side step
+// One step of the whole envelope after a wake pass, enqueued on s: midranges
+// of chi, then the sweep, replayed from the captured graph of this (dt, m).
+int envelope_step(Envelope *e, double dt, int m, hipStream_t s) {
+    ELAUNCH(k_env_cmid, dim3(e->nx), kET, 0, (const double *)e->chi3, e->cmid, e->nn());
+    if (!e->gexec || e->g_dtau != dt || e->g_iter != m || e->g_stream != s) {
+        drop_graph(e);
+        hipGraph_t gr;
+        QP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int st = env_sweep(e, dt, m, s);
+        const hipError_t ce = hipStreamEndCapture(s, &gr);
+        if (st < 0) return st;
+        if (ce != hipSuccess) {
+            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+            return QP_ERR_HIP;
+        }
+        const hipError_t ie = hipGraphInstantiate(&e->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ie != hipSuccess) {
+            e->gexec = nullptr;
+            set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+            return QP_ERR_HIP;
+        }
+        e->g_dtau = dt;
+        e->g_iter = m;
+        e->g_stream = s;
+    }
+    QP_HIP_CHECK(hipGraphLaunch(e->gexec, s));
+    return QP_OK;
+}
 
 }  // namespace qp
 
@@ -342,6 +366,7 @@ int qp_envelope_destroy(qp_envelope h) {
     e->S.destroy();
     for (double *p : {e->a, e->inc, e->chi3, e->asq3, e->cmid})
         if (p) (void)hipFree(p);
+
     delete h;
     return QP_OK;
 }
@@ -438,7 +463,7 @@ int qp_envelope_advance(qp_envelope h, double dtau, int steps, int chi_iteration
     }
     if (steps > 0) ELAUNCH(k_env_cmid, dim3(e->nx), kET, 0, (const double *)e->chi3, e->cmid, e->nn());
     if (use_graph && steps > 0) {
-        if (!e->gexec || e->g_dtau != dtau || e->g_iter != chi_iterations) {
+        if (!e->gexec || e->g_dtau != dtau || e->g_iter != chi_iterations || e->g_stream != s) {
             drop_graph(e);
             hipGraph_t gr;
             QP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -458,6 +483,7 @@ int qp_envelope_advance(qp_envelope h, double dtau, int steps, int chi_iteration
             }
             e->g_dtau = dtau;
             e->g_iter = chi_iterations;
+            e->g_stream = s;
         }
         for (int n = 0; n < steps; ++n) QP_HIP_CHECK(hipGraphLaunch(e->gexec, s));
     } else {

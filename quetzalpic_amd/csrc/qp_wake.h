@@ -67,6 +67,8 @@ struct Wake {
     // Attached laser envelope (or null): <a^2> comes from its |a|^2/2 and the
     // chi of every slice goes to its chi volume.
     Envelope *env = nullptr;
+    double env_dtau = 0.0;   // > 0: each pass also advances the envelope by one step (after it)
+    int env_chi_it = 1;
     // Attached driver bunch (or null): its sources are deposited per slice into
     // two planes (in place of rho_b3/jx_b3) and it gathers the fields per slice.
     Bunch *bunch = nullptr;
@@ -91,6 +93,7 @@ struct Wake {
     // 3D <a^2> the slice loop reads (null: Gaussian laser or none).
     double *asq3() const { return env ? env->asq3 : a_sqr3; }
     double *chi_store(int slice) const { return env ? env->chi3 + (size_t)slice * nn() : nullptr; }
+    bool env_step() const { return env && env_dtau > 0; }
     // Bunch sources the slice loop reads: slice i at base + i * stride (stride 0
     // for the per-slice planes of an attached bunch).
     const double *rho_b_src() const { return bunch ? bunch->rho_plane() : rho_b3; }
