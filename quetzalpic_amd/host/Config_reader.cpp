@@ -97,6 +97,7 @@ void Config_reader::read_all() {
     out << std::endl;
     init_output_parameters();
     init_device();
+    init_wakes();
 }
 
 // src/Config_reader.cpp:109-136
@@ -108,26 +109,9 @@ void Config_reader::init_laser() {
     out << "Parsing [laser] ..." << std::endl;
     auto t = config->get_table("laser");
     if (!t) throw Config_exception("[laser] must be a table");
-    const auto shape = read_value<std::string>("shape", t);
-    if (shape != "gaussian") throw Config_exception("Laser shape \"" + shape + "\" is not supported, use \"gaussian\".");
-    const double a0 = read_value<double>("a0", t);
-    const double xsigma = read_value<double>("xsigma", t);
-    const double ysigma = read_value<double>("ysigma", t);
-    const double zsigma = read_value<double>("zsigma", t);
-    const double x0 = read_value<double>("x0", xsigma, t);
-    const double y0 = read_value<double>("y0", 0.5 * params.l.y, t);
-    const double z0 = read_value<double>("z0", 0.5 * params.l.z, t);
-    vector3d width{xsigma, ysigma, zsigma};
-    width /= std::sqrt(8.0);  // full width at 1/e^2 -> Gaussian parameter
-    params.a_sqr = gaussian3d(0.5 * a0 * a0, width, {x0, y0, z0});
+    parse_laser_table(t, params.laser_a0, params.laser_sigma, params.laser_r0);
+    params.a_sqr = laser_a_sqr(params.laser_a0, params.laser_sigma, params.laser_r0);
     params.laser_gaussian = true;
-    params.laser_a0 = a0;
-    params.laser_sigma[0] = xsigma;
-    params.laser_sigma[1] = ysigma;
-    params.laser_sigma[2] = zsigma;
-    params.laser_r0[0] = x0;
-    params.laser_r0[1] = y0;
-    params.laser_r0[2] = z0;
     // extension keys: an evolving laser (no reference counterpart)
     params.laser_evolve = read_value<bool>("evolve", false, t);
     if (params.laser_evolve) {
@@ -135,6 +119,27 @@ void Config_reader::init_laser() {
         if (!(params.laser_k0 > 0)) throw Config_exception("[laser] k0 must be positive");
         params.laser_chi_iterations = read_value<int>("chi_iterations", 1, t);
     }
+}
+
+// The Gaussian laser keys of a [laser] table (src/Config_reader.cpp:113-127).
+void Config_reader::parse_laser_table(const toml_lite::TablePtr &t, double &a0, double sigma[3], double r0[3]) {
+    const auto shape = read_value<std::string>("shape", t);
+    if (shape != "gaussian") throw Config_exception("Laser shape \"" + shape + "\" is not supported, use \"gaussian\".");
+    a0 = read_value<double>("a0", t);
+    sigma[0] = read_value<double>("xsigma", t);
+    sigma[1] = read_value<double>("ysigma", t);
+    sigma[2] = read_value<double>("zsigma", t);
+    r0[0] = read_value<double>("x0", sigma[0], t);
+    r0[1] = read_value<double>("y0", 0.5 * params.l.y, t);
+    r0[2] = read_value<double>("z0", 0.5 * params.l.z, t);
+}
+
+// <a^2> = a0^2/2 Gaussian with the full widths at 1/e^2 turned into Gaussian
+// parameters (width / sqrt(8), src/Config_reader.cpp:128-131).
+profile3d laser_a_sqr(double a0, const double sigma[3], const double r0[3]) {
+    vector3d width{sigma[0], sigma[1], sigma[2]};
+    width /= std::sqrt(8.0);
+    return gaussian3d(0.5 * a0 * a0, width, {r0[0], r0[1], r0[2]});
 }
 
 // src/Config_reader.cpp:138-186
@@ -208,14 +213,18 @@ void Config_reader::init_output_parameters() {
     params.output_parameters = o;
 }
 
-// src/Config_reader.cpp:236-258, plus the "channel" extension profile
+// src/Config_reader.cpp:236-258, plus the "channel" and "ramp" extension profiles
 void Config_reader::init_plasma_profile() {
     if (!config->contains("plasma")) {
         out << "No [plasma] block in config; uniform plasma will be used";
         return;
     }
     out << "Parsing [plasma] ..." << std::endl;
-    auto t = config->get_table("plasma");
+    params.plasma_profile = parse_plasma_table(config->get_table("plasma"));
+}
+
+profile2d Config_reader::parse_plasma_table(const toml_lite::TablePtr &t) {
+    if (!t) throw Config_exception("[plasma] must be a table");
     const auto profile = read_value<std::string>("profile", t);
     if (profile == "powerlaw") {
         const double power = read_value<double>("power", t);
@@ -224,22 +233,74 @@ void Config_reader::init_plasma_profile() {
         const double mx = read_value<double>("max_factor", 1e10, t);
         const double y0 = read_value<double>("y0", 0.5 * params.l.y, t);
         const double z0 = read_value<double>("z0", 0.5 * params.l.z, t);
-        params.plasma_profile = powerlaw2d(power, radius, y0, z0, mn, mx, -1.0);
-    } else if (profile == "constant") {
-        params.plasma_profile = constant2d(read_value<double>("rho0", -1.0, t));
-    } else if (profile == "channel") {
+        return powerlaw2d(power, radius, y0, z0, mn, mx, -1.0);
+    }
+    if (profile == "constant") return constant2d(read_value<double>("rho0", -1.0, t));
+    if (profile == "channel") {
         const double base = read_value<double>("base", 1.0, t);
         const double curv = read_value<double>("curvature", t);
         const double y0 = read_value<double>("y0", 0.5 * params.l.y, t);
         const double z0 = read_value<double>("z0", 0.5 * params.l.z, t);
-        params.plasma_profile = channel2d(base, curv, y0, z0, -1.0);
-    } else if (profile == "ramp") {
+        return channel2d(base, curv, y0, z0, -1.0);
+    }
+    if (profile == "ramp") {
         const double slope = read_value<double>("slope", t);
         const double y0 = read_value<double>("y0", 0.5 * params.l.y, t);
-        params.plasma_profile = ramp2d(slope, y0, -1.0);
-    } else {
-        throw Config_exception("Plasma profile \"" + profile +
-                               "\" is not supported, use \"powerlaw\", \"constant\", \"channel\" or \"ramp\".");
+        return ramp2d(slope, y0, -1.0);
+    }
+    throw Config_exception("Plasma profile \"" + profile +
+                           "\" is not supported, use \"powerlaw\", \"constant\", \"channel\" or \"ramp\".");
+}
+
+namespace {
+// The member's own sub-table over the deck's top-level table of that name:
+// keys the member leaves out are taken from the deck.
+toml_lite::TablePtr overlay(const toml_lite::TablePtr &base, const toml_lite::TablePtr &member) {
+    auto t = std::make_shared<toml_lite::Table>();
+    if (base) t->values = base->values;
+    if (member)
+        for (const auto &kv : member->values) t->values[kv.first] = kv.second;
+    return t;
+}
+}  // namespace
+
+// [[wake]]: a batch of independent wakes on the deck's geometry (extension, no
+// reference counterpart).  Each member may carry name, ppcy, ppcz and its own
+// [wake.laser] / [wake.plasma] tables, whose keys override the deck's [laser] /
+// [plasma].
+void Config_reader::init_wakes() {
+    if (!config->contains("wake")) return;
+    if (!config->is_table_array("wake")) throw Config_exception("\"wake\" must be an array of tables [[wake]]");
+    if (!params.bunch_parameters_array.empty()) throw Config_exception("[[wake]] batches take no [bunch]");
+    if (params.laser_evolve) throw Config_exception("[[wake]] batches take no evolving laser");
+    if (params.device.precision != 1)
+        throw Config_exception("[[wake]] batches run on the mixed path: [device] precision must be \"mixed\"");
+    const auto *arr = config->get_table_array("wake");
+    out << "Parsing [[wake]] ..." << std::endl << "Size " << arr->size() << std::endl;
+    const auto laser = config->is_table("laser") ? config->get_table("laser") : nullptr;
+    const auto plasma = config->is_table("plasma") ? config->get_table("plasma") : nullptr;
+    for (size_t i = 0; i < arr->size(); ++i) {
+        const auto &m = (*arr)[i];
+        out << "Parsing element " << i << "..." << std::endl;
+        Wake_member w;
+        w.name = read_value<std::string>("name", "wake" + std::to_string(i), m);
+        if (w.name.empty() || w.name.find_first_not_of("abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ"
+                                                        "0123456789_-.") != std::string::npos)
+            throw Config_exception("[[wake]] name \"" + w.name + "\" must be letters, digits, '_', '-' or '.'");
+        for (const auto &o : params.wakes)
+            if (o.name == w.name) throw Config_exception("[[wake]] name \"" + w.name + "\" is used twice");
+        w.ppcy = read_value<int>("ppcy", params.ppcy, m);
+        w.ppcz = read_value<int>("ppcz", params.ppcz, m);
+        if (w.ppcy <= 0 || w.ppcz <= 0) throw Config_exception("[[wake]] ppcy and ppcz must be positive");
+        const auto ml = m->is_table("laser") ? m->get_table("laser") : nullptr;
+        if (!laser && !ml) throw Config_exception("[[wake]] member " + w.name + " has no laser");
+        const auto lt = overlay(laser, ml);
+        parse_laser_table(lt, w.a0, w.sigma, w.r0);
+        bool evolve = false;
+        if (lt->get("evolve", evolve) && evolve) throw Config_exception("[[wake]] batches take no evolving laser");
+        const auto mp = m->is_table("plasma") ? m->get_table("plasma") : nullptr;
+        w.plasma_profile = (mp || plasma) ? parse_plasma_table(overlay(plasma, mp)) : constant2d(-1.0);
+        params.wakes.push_back(w);
     }
 }
 

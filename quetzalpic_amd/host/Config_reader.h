@@ -9,6 +9,10 @@
 #include "System_parameters.h"
 #include "toml_lite.h"
 
+// <a^2> of the deck's Gaussian laser: a0^2/2 with the full widths at 1/e^2
+// (sigma) turned into Gaussian parameters, centred at r0.
+profile3d laser_a_sqr(double a0, const double sigma[3], const double r0[3]);
+
 class Config_exception : public std::exception {
 public:
     explicit Config_exception(std::string message) : message_(std::move(message)) {}
@@ -33,6 +37,9 @@ private:
     template <class T>
     T read_value(const std::string &name, T default_value, const toml_lite::TablePtr &table);
     void init_laser();
+    void parse_laser_table(const toml_lite::TablePtr &t, double &a0, double sigma[3], double r0[3]);
+    profile2d parse_plasma_table(const toml_lite::TablePtr &t);
+    void init_wakes();
     Bunch_parameters parse_bunch_table(const toml_lite::TablePtr &t);
     void init_bunch();
     void init_output_parameters();

@@ -156,11 +156,11 @@ array1d calculate_y_slice(const array2d &a, double z0) {
     return r;
 }
 
-Output_writer::Output_writer(Output_parameters p, int count) : output_parameters(p) {
-    if (p.output3d) fields_file = create_file(numbered("Fields", count));
-    if (p.output_xy) fields_xy_file = create_file(numbered("Fields_xy", count));
-    if (p.output_bunch) bunch_file = create_file(numbered("Bunch", count));
-    if (p.output_photons) photons_file = create_file(numbered("Photons", count));
+Output_writer::Output_writer(Output_parameters p, int count, const std::string &prefix) : output_parameters(p) {
+    if (p.output3d) fields_file = create_file(prefix + numbered("Fields", count));
+    if (p.output_xy) fields_xy_file = create_file(prefix + numbered("Fields_xy", count));
+    if (p.output_bunch) bunch_file = create_file(prefix + numbered("Bunch", count));
+    if (p.output_photons) photons_file = create_file(prefix + numbered("Photons", count));
 }
 
 Output_writer::Output_writer(Output_writer &&o) noexcept

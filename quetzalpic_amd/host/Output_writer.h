@@ -14,7 +14,9 @@
 
 class Output_writer {
 public:
-    Output_writer(Output_parameters output_parameters, int count);
+    // Files <prefix>Fields_NNN.h5 etc. in the working directory (prefix empty
+    // for the reference's names; a [[wake]] member's name + "_" in a batch).
+    Output_writer(Output_parameters output_parameters, int count, const std::string &prefix = "");
     ~Output_writer();
     Output_writer(const Output_writer &) = delete;
     Output_writer &operator=(const Output_writer &) = delete;

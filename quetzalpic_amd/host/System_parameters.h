@@ -2,6 +2,7 @@
 // src/System_parameters.h:8-41, plus the device-path knobs of this build).
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "Species_register.h"
@@ -35,6 +36,15 @@ struct Device_parameters {
     int stream_output = 1;          // mixed path: per-slice output streamed to a writer thread
 };
 
+// One member of a [[wake]] batch (extension): its name (output file prefix),
+// lattice, plasma profile and Gaussian laser.
+struct Wake_member {
+    std::string name;
+    int ppcy = 1, ppcz = 1;
+    profile2d plasma_profile = constant2d(-1.0);
+    double a0 = 0, sigma[3] = {0, 0, 0}, r0[3] = {0, 0, 0};
+};
+
 struct System_parameters {
     vector3d l{0, 0, 0};
     vector3d d{0, 0, 0};
@@ -58,4 +68,5 @@ struct System_parameters {
     double base_frequency_SI = 1;
     bool qed = false;
     Device_parameters device;
+    std::vector<Wake_member> wakes;  // [[wake]] batch; empty for a single System_3d
 };

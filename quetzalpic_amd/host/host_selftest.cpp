@@ -118,6 +118,58 @@ int main(int argc, char **argv) {
               "device table extension");
     }
 
+    // dotted headers: [a.b] inside a table and inside the last [[a]] element
+    {
+        auto t = toml_lite::parse_string("[u]\nx = 1\n[u.v]\ny = 2\n[[w]]\nn = 'a'\n[w.l]\nk = 3\n[[w]]\nn = 'b'\n"
+                                         "[w.l]\nk = 4\n");
+        int y = 0, k0 = 0, k1 = 0;
+        check(t->get_table("u")->get_table("v")->get("y", y) && y == 2, "toml dotted table in a table");
+        const auto *w = t->get_table_array("w");
+        check(w && w->size() == 2 && (*w)[0]->get_table("l")->get("k", k0) && k0 == 3 &&
+                  (*w)[1]->get_table("l")->get("k", k1) && k1 == 4,
+              "toml dotted table in the last array element");
+        check(throws([] { toml_lite::parse_string("[u]\n[u.v]\n[u.v]\n"); }), "toml duplicate dotted table rejected");
+        check(throws([] { toml_lite::parse_string("x = 1\n[x.v]\n"); }), "toml dotted table over a value rejected");
+    }
+
+    // [[wake]] batch members (extension): inheritance from [laser] / [plasma],
+    // per-member overrides, names and the combinations a batch cannot run
+    {
+        const std::string base = "lx=3\nly=20\nlz=20\ndx=0.1\ndy=0.625\ndz=0.625\nppcy=2\nppcz=2\n"
+                                 "[laser]\nshape='gaussian'\na0=1.0\nxsigma=4\nysigma=6\nzsigma=6\nx0=1.5\n";
+        System_parameters p =
+            Config_reader::from_string(base + "[[wake]]\nname='low'\n[[wake]]\nppcy=1\n[wake.laser]\na0=2.5\ny0=9\n"
+                                              "[wake.plasma]\nprofile='channel'\ncurvature=0.05\n",
+                                       sink)
+                .get_parameters();
+        check(p.wakes.size() == 2 && p.wakes[0].name == "low" && p.wakes[1].name == "wake1", "wake names and default");
+        const auto &a = p.wakes[0], &b = p.wakes[1];
+        check(a.a0 == 1.0 && a.sigma[1] == 6 && a.r0[0] == 1.5 && a.r0[1] == 10 && a.ppcy == 2,
+              "wake member inherits [laser] and ppc");
+        check(b.a0 == 2.5 && b.r0[1] == 9 && b.r0[2] == 10 && b.sigma[0] == 4 && b.ppcy == 1 && b.ppcz == 2,
+              "wake member overrides a0, y0 and ppcy");
+        check(a.plasma_profile(3.0, 4.0) == -1.0 && near(b.plasma_profile(10.0 + 3.0, 10.0 + 4.0), -(1.0 + 0.05 * 25.0)),
+              "wake member plasma profile");
+        check(throws([&] { Config_reader::from_string(base + "[[wake]]\nname='a'\n[[wake]]\nname='a'\n", sink); }),
+              "duplicate wake name rejected");
+        check(throws([&] { Config_reader::from_string(base + "[[wake]]\nname='a/b'\n", sink); }),
+              "wake name with a path separator rejected");
+        check(throws([&] { Config_reader::from_string(base + "[wake]\nname='a'\n", sink); }),
+              "[wake] as a single table rejected");
+        check(throws([&] { Config_reader::from_string(base + "[device]\nprecision='fp64'\n[[wake]]\n", sink); }),
+              "wake batch on the fp64 path rejected");
+        check(throws([&] {
+                  Config_reader::from_string(base + "[bunch]\nshape='gaussian'\ngamma=10\nrho0=1\nxsigma=1\n"
+                                                    "ysigma=1\nzsigma=1\n[[wake]]\n",
+                                             sink);
+              }),
+              "wake batch with a bunch rejected");
+        check(throws([&] {
+                  Config_reader::from_string("lx=3\nly=20\nlz=20\ndx=0.1\ndy=0.625\ndz=0.625\n[[wake]]\n", sink);
+              }),
+              "wake member without a laser rejected");
+    }
+
     // Species_register (src/Species_register.cpp:6-30)
     {
         Species_register s;

@@ -131,10 +131,30 @@ TablePtr parse_string(const std::string &text) {
         }
         if (s[0] == '[') {
             if (s.back() != ']') throw parse_error("line " + std::to_string(lineno) + ": bad table header");
-            std::string name = trim(s.substr(1, s.size() - 2));
-            if (root->contains(name)) throw parse_error("line " + std::to_string(lineno) + ": duplicate table " + name);
+            const std::string name = trim(s.substr(1, s.size() - 2));
+            // [a.b]: table b inside table a, or inside the last element of [[a]]
+            TablePtr parent = root;
+            std::string last = name;
+            for (size_t dot; (dot = last.find('.')) != std::string::npos;) {
+                const std::string part = trim(last.substr(0, dot));
+                last = trim(last.substr(dot + 1));
+                if (parent->is_table_array(part)) {
+                    parent = parent->table_arrays[part].back();
+                } else if (parent->is_table(part)) {
+                    parent = parent->tables[part];
+                } else if (!parent->contains(part) && !part.empty()) {
+                    auto t = std::make_shared<Table>();
+                    parent->tables[part] = t;
+                    parent = t;
+                } else {
+                    throw parse_error("line " + std::to_string(lineno) + ": bad table header " + name);
+                }
+            }
+            if (last.empty()) throw parse_error("line " + std::to_string(lineno) + ": bad table header " + name);
+            if (parent->contains(last))
+                throw parse_error("line " + std::to_string(lineno) + ": duplicate table " + name);
             cur = std::make_shared<Table>();
-            root->tables[name] = cur;
+            parent->tables[last] = cur;
             continue;
         }
         size_t eq = s.find('=');

@@ -1,5 +1,6 @@
 // toml_lite.h -- the subset of TOML the input decks use: comments, key = value
-// with integers, floats, booleans and strings, [table] and [[array of tables]].
+// with integers, floats, booleans and strings, [table], [[array of tables]] and
+// dotted [a.b] headers (table b inside table a or the last element of [[a]]).
 // Type rules follow what the reference's reader accepts: a double key takes an
 // integer or a float, an integer key only an integer.
 #pragma once

@@ -123,3 +123,50 @@ def test_cli_ramp_profile(oracle, tmp_path):
     want, _, _ = oracle.wake_run(g, y[keep], z[keep], v[keep] / ppc / ppc, a, fields=("psi", "ex", "rho"))
     for f in ("psi", "ex", "rho"):
         assert rel_err(h5read.read_field(str(tmp_path / "Fields_000.h5"), f), want[f])[0] < 1e-9, f
+
+
+def lattice(g, ppcy, ppcz, profile):
+    """init_wake_particles (src/System_3d.cpp:724-748): i outer, zero cells skipped."""
+    yy = (np.arange(ppcy * g.ny) + 0.5) * g.dy / ppcy
+    zz = (np.arange(ppcz * g.nz) + 0.5) * g.dz / ppcz
+    y = np.repeat(yy, len(zz))
+    z = np.tile(zz, len(yy))
+    v = profile(y, z)
+    keep = v != 0
+    return y[keep], z[keep], v[keep] / ppcy / ppcz
+
+
+def test_cli_wake_batch(qp_ctx, oracle, tmp_path):
+    """[[wake]] members run as one batch; each member's files equal that member
+    run alone in its own mixed wake, and the first member equals the oracle."""
+    from quetzalpic_amd import capi, h5read
+    out = run_cli(tmp_path, "wake_batch_mixed.toml")
+    assert "Batch of 3 wakes" in out
+    g = oracle.grid_from_box(3.0, 20.0, 20.0, 0.1, 0.625, 0.625, K=5)
+    members = {
+        "base": ((1.0, 4.0, 6.0, 6.0, 1.5, 10.0, 10.0), 2, lambda y, z: -np.ones_like(y)),
+        "channel_a2": ((2.0, 4.0, 6.0, 6.0, 1.5, 9.5, 10.0), 2,
+                       lambda y, z: -(1.0 + 0.02 * ((y - 10.0) ** 2 + (z - 10.0) ** 2))),
+        "ramp": ((1.0, 4.0, 6.0, 6.0, 1.5, 10.0, 10.0), 1, lambda y, z: -np.maximum(0.0, 1.0 + 0.08 * (y - 10.0))),
+    }
+    fields = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
+    for name, (laser, ppcy, prof) in members.items():
+        path = str(tmp_path / f"{name}_Fields_000.h5")
+        assert os.path.exists(tmp_path / f"{name}_Fields_xy_000.h5")
+        y, z, w = lattice(g, ppcy, 2, prof)
+        wk = capi.Wake(qp_ctx, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ly, g.lz, K=5, precision=capi.QP_PREC_MIXED)
+        wk.set_particles(y, z, w)
+        wk.set_laser_gaussian(*laser)
+        single, st, _ = wk.run(fields=fields)
+        wk.close()
+        assert st == 0
+        for f in fields:
+            got = h5read.read_field(path, "susceptibility" if f == "chi" else f)
+            assert rel_err(got, single[f])[0] < 1e-9, (name, f, rel_err(got, single[f]))
+        a = oracle.gaussian_a_sqr(g, *laser)
+        assert rel_err(h5read.read_field(path, "aSqr"), a)[0] < 1e-13, name
+        if name == "base":
+            want, _, _ = oracle.wake_run(g, y, z, w, a, fields=("psi", "ex"))
+            for f in ("psi", "ex"):
+                assert rel_err(h5read.read_field(path, f), want[f])[0] < 1e-5, f
+    assert not os.path.exists(tmp_path / "Fields_000.h5")
