@@ -251,17 +251,36 @@ __device__ inline double block_sum(double v) {
     return s;
 }
 
+// Min over the wavefront, the same value in every lane: four DPP steps inside
+// each 16-lane row (lane ^ 1, lane ^ 2, half-row mirror, row mirror) leave the
+// row's min in all of its lanes, then the four rows are read out.  Every lane
+// of the wavefront must call it.
+template <int CTRL>
+__device__ __forceinline__ int dpp_row(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
 __device__ __forceinline__ int wave_min(int v) {
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
-    return v;
+    v = min(v, dpp_row<0xB1>(v));   // quad_perm(1, 0, 3, 2)
+    v = min(v, dpp_row<0x4E>(v));   // quad_perm(2, 3, 0, 1)
+    v = min(v, dpp_row<0x141>(v));  // row_half_mirror
+    v = min(v, dpp_row<0x140>(v));  // row_mirror
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 __device__ __forceinline__ int wave_max(int v) {
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-    return v;
+    v = max(v, dpp_row<0xB1>(v));
+    v = max(v, dpp_row<0x4E>(v));
+    v = max(v, dpp_row<0x141>(v));
+    v = max(v, dpp_row<0x140>(v));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
-// rmin/rmax/cmin/cmax: this thread's relative node range (rmin > rmax when
-// the thread deposits nothing).  `sh` is 4*(blockDim/64)+2 ints of LDS.
+// rmin/rmax/cmin/cmax: this thread's relative node range (INT_MAX / INT_MIN
+// when the thread deposits nothing).  Relative indices come from rel_idx, so
+// they lie in [-n/2, n - n/2].
+// `sh` is 4*(blockDim/64)+2 ints of LDS.  The window is the same in every lane
+// (kept in scalar registers).
 __device__ inline Win block_window(int rmin, int rmax, int cmin, int cmax, int jref, int kref, int n1, int n2,
                                    int *sh) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -284,6 +303,10 @@ __device__ inline Win block_window(int rmin, int rmax, int cmin, int cmax, int j
         d = max(d, sh[4 * i + 3]);
     }
     __syncthreads();
+    a = __builtin_amdgcn_readfirstlane(a);
+    b = __builtin_amdgcn_readfirstlane(b);
+    c = __builtin_amdgcn_readfirstlane(c);
+    d = __builtin_amdgcn_readfirstlane(d);
     Win w;
     if (a > b) {  // no deposits in this block
         w.j0 = w.k0 = 0;
@@ -293,21 +316,27 @@ __device__ inline Win block_window(int rmin, int rmax, int cmin, int cmax, int j
     }
     w.wj = b - a + 1;
     w.wk = d - c + 1;
-    w.j0 = jref + a;
-    w.k0 = kref + c;
-    w.j0 = ((w.j0 % n1) + n1) % n1;
-    w.k0 = ((w.k0 % n2) + n2) % n2;
+    // jref in [0, n1) and a in [-n1/2, n1 - n1/2]: one correction wraps j0
+    w.j0 = __builtin_amdgcn_readfirstlane(jref) + a;
+    if (w.j0 < 0) w.j0 += n1;
+    else if (w.j0 >= n1) w.j0 -= n1;
+    w.k0 = __builtin_amdgcn_readfirstlane(kref) + c;
+    if (w.k0 < 0) w.k0 += n2;
+    else if (w.k0 >= n2) w.k0 -= n2;
     w.lds = (w.wj <= n1 && w.wk <= n2 && w.wj * w.wk <= kWinCap);
     return w;
 }
 
-__device__ __forceinline__ int win_slot(const Win &w, int j, int k, int n1, int n2) {
+// Window slot of node (j, k) for a CIC stencil whose other nodes are (j + 1, k),
+// (j, k + 1), (j + 1, k + 1) (periodic): those sit at + wk, + 1 and + wk + 1,
+// since the stencil stays inside the window.  -1 when it does not (the
+// deposit then goes to global memory).
+__device__ __forceinline__ int win_base(const Win &w, int j, int k, int n1, int n2) {
     int dj = j - w.j0;
     if (dj < 0) dj += n1;
     int dk = k - w.k0;
     if (dk < 0) dk += n2;
-    if (dj >= w.wj || dk >= w.wk) return -1;
-    return dj * w.wk + dk;
+    return (dj + 1 < w.wj && dk + 1 < w.wk) ? dj * w.wk + dk : -1;
 }
 
 // ------------------------------------------------ quad folding of deposits
@@ -368,18 +397,17 @@ __device__ __forceinline__ void cic_w(double v, double yf, double zf, double *a)
 typedef __attribute__((address_space(3))) double lds_f64;
 
 // One CIC deposit of weights a11..a22 at (j1..j2, k1..k2) into a grid (LDS
-// window or global).
+// window or global); j2 = j1 + 1 and k2 = k1 + 1 up to the periodic wrap.
 __device__ __forceinline__ void dep4w(double *lds, const Win &w, double *glob, int n1, int n2, int j1, int j2,
                                       int k1, int k2, double a11, double a21, double a12, double a22) {
     if (w.lds) {
-        const int s11 = win_slot(w, j1, k1, n1, n2), s21 = win_slot(w, j2, k1, n1, n2);
-        const int s12 = win_slot(w, j1, k2, n1, n2), s22 = win_slot(w, j2, k2, n1, n2);
-        if (s11 >= 0 && s21 >= 0 && s12 >= 0 && s22 >= 0) {
-            lds_f64 *l = (lds_f64 *)lds;
-            __hip_atomic_fetch_add(l + s11, a11, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(l + s21, a21, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(l + s12, a12, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            __hip_atomic_fetch_add(l + s22, a22, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const int s = win_base(w, j1, k1, n1, n2);
+        if (s >= 0) {
+            lds_f64 *l = (lds_f64 *)lds + s;
+            __hip_atomic_fetch_add(l, a11, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(l + w.wk, a21, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(l + 1, a12, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(l + w.wk + 1, a22, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             return;
         }
     }
@@ -395,18 +423,17 @@ template <int G>
 __device__ __forceinline__ void dep4w_multi(double *lds, const Win &w, double *const *glob, int n1, int n2, int j1,
                                             int j2, int k1, int k2, const double *a) {
     if (w.lds) {
-        const int s11 = win_slot(w, j1, k1, n1, n2), s21 = win_slot(w, j2, k1, n1, n2);
-        const int s12 = win_slot(w, j1, k2, n1, n2), s22 = win_slot(w, j2, k2, n1, n2);
-        if (s11 >= 0 && s21 >= 0 && s12 >= 0 && s22 >= 0) {
-            lds_f64 *l = (lds_f64 *)lds;
+        const int s = win_base(w, j1, k1, n1, n2);
+        if (s >= 0) {
+            lds_f64 *l = (lds_f64 *)lds + s;
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                __hip_atomic_fetch_add(l + g * kWinCap + s11, a[4 * g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(l + g * kWinCap + s21, a[4 * g + 1], __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(l + g * kWinCap, a[4 * g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(l + g * kWinCap + w.wk, a[4 * g + 1], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(l + g * kWinCap + s12, a[4 * g + 2], __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(l + g * kWinCap + 1, a[4 * g + 2], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(l + g * kWinCap + s22, a[4 * g + 3], __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(l + g * kWinCap + w.wk + 1, a[4 * g + 3], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             return;
@@ -437,11 +464,15 @@ __device__ inline void win_zero(double *lds, int ngrid, const Win &w) {
         for (int i = threadIdx.x; i < m; i += blockDim.x) lds[g * kWinCap + i] = 0.0;
 }
 
+// Row of window slot i = dj * wk + dk: (i + 1/2) / wk sits at least 1/(2 wk)
+// from an integer, and the single-precision product errs by less than
+// 2^-22 kWinCap < 1/(2 kWinCap) for i < kWinCap, wk <= kWinCap.
 __device__ inline void win_flush(const double *lds, double *const *glob, int ngrid, const Win &w, int n1, int n2) {
     if (!w.lds) return;
     const int m = w.wj * w.wk;
+    const float inv_wk = 1.0f / (float)w.wk;
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
-        const int dj = i / w.wk, dk = i - dj * w.wk;
+        const int dj = (int)(((float)i + 0.5f) * inv_wk), dk = i - dj * w.wk;
         int j = w.j0 + dj;
         if (j >= n1) j -= n1;
         int k = w.k0 + dk;
