@@ -191,8 +191,9 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         const Pos q = load_pos(P, p);
         w = P.w[p];
         const double py = P.py[p], pz = P.pz[p];
-        const double a = mx_gather(asq, q, n1, n2);
-        const double ps = mx_gather(psi, q, n1, n2);
+        const ValGrad ga = mx_gather_vg(asq, q, n1, n2, g.inv_d1, g.inv_d2);
+        const ValGrad gp = mx_gather_vg(psi, q, n1, n2, g.inv_d1, g.inv_d2);
+        const double a = ga.v, ps = gp.v;
         const double inv1ps = 1.0 / (1 + ps);  // the FP64 1/(1+psi) factor
         double gm;
         if (first_iter) {
@@ -201,10 +202,7 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
             const double pys = py + (double)P.dpy[p], pzs = pz + (double)P.dpz[p];  // P6 of the previous iteration (:503-505)
             gm = gamma_of(0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs), a, ps, inv1ps);
         }
-        const double day = mx_yder(asq, q, n1, n2, g.inv_d1);
-        const double daz = mx_zder(asq, q, n1, n2, g.inv_d2);
-        const double dpy = mx_yder(psi, q, n1, n2, g.inv_d1);
-        const double dpz = mx_zder(psi, q, n1, n2, g.inv_d2);
+        const double day = ga.dy, daz = ga.dz, dpy = gp.dy, dpz = gp.dz;
         const double byp = mx_gather_hz(by, q, n1, n2);
         const double bzp = mx_gather_hy(bz, q, n1, n2);
         const double dx = g.dx;
