@@ -296,6 +296,7 @@ __device__ inline double block_sum(double v) {
 // each 16-lane row (lane ^ 1, lane ^ 2, half-row mirror, row mirror) leave the
 // row's min in all of its lanes, then the four rows are read out.  Every lane
 // of the wavefront must call it.
+// (The compiler folds these moves into the min / max that uses them.)
 template <int CTRL>
 __device__ __forceinline__ int dpp_row(int x) {
     return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
@@ -385,9 +386,11 @@ __device__ __forceinline__ int win_base(const Win &w, int j, int k, int n1, int 
 // lanes) often deposit onto the same four nodes.  Folding their weights into
 // one lane before the atomics cuts the LDS atomics and their same-address
 // conflicts by up to 4x.  Lane exchange by DPP quad permutations.
+// A quad permutation reads a valid lane for every lane, so no "old" value is
+// needed: mov_dpp is one instruction, without a zeroed destination first.
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int x) {
-    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, false);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double x) {
