@@ -517,7 +517,7 @@ def main():
             f"streamed {output_pass['async']:.1f} ms")
 
     # ---- laser-envelope advance on the same grid (outside the timed region)
-    envelope = bench_envelope(capi, ctx, wl, max(args.steps, 3))
+    envelope = bench_envelope(capi, ctx, wl, max(args.steps, 3), use_graph=0 if args.no_graph else 1)
 
     cpu = None
     err_oracle = None

@@ -36,8 +36,8 @@ struct Envelope {
 };
 
 // One step of the whole envelope after a wake pass (midranges of chi, then the
-// captured sweep), enqueued on s.
-int envelope_step(Envelope *e, double dt, int m, hipStream_t s);
+// sweep: a captured graph when use_graph, else direct launches), enqueued on s.
+int envelope_step(Envelope *e, double dt, int m, int use_graph, hipStream_t s);
 
 }  // namespace qp
 

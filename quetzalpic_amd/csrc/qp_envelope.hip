@@ -267,9 +267,11 @@ void drop_graph(Envelope *e) {
 
 // ------------------------------------------------------- wake-side step
 // One step of the whole envelope after a wake pass, enqueued on s: midranges
-// of chi, then the sweep, replayed from the captured graph of this (dt, m).
-int envelope_step(Envelope *e, double dt, int m, hipStream_t s) {
+// of chi, then the sweep, replayed from the captured graph of this (dt, m)
+// when use_graph, else launched directly.
+int envelope_step(Envelope *e, double dt, int m, int use_graph, hipStream_t s) {
     ELAUNCH(k_env_cmid, dim3(e->nx), kET, 0, (const double *)e->chi3, e->cmid, e->nn());
+    if (!use_graph) return env_sweep(e, dt, m, s);
     if (!e->gexec || e->g_dtau != dt || e->g_iter != m || e->g_stream != s) {
         drop_graph(e);
         hipGraph_t gr;

@@ -474,7 +474,7 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
     // bunch bucket 0: slice 0's fields are final only after output_step(0)
     if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, 0, s));
     // the envelope step after the pass (the plain path keeps the launches in order)
-    if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, s));
+    if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;

@@ -1501,7 +1501,7 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
         QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, s));
     // the envelope step after the pass (chi of slices the pass did not reach is
     // the last one stored)
-    if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, s));
+    if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
