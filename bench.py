@@ -380,8 +380,6 @@ def bench_c4(args, capi, workloads):
         "classes_alone": alone, "kernel_classes_largest": kc, "roofline": roofline,
         "err_vs_oracle_member": err, "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "envelope_advance": envelope,
-        "output_pass_ms": output_pass,
     }
     print(json.dumps(line), flush=True)
 
@@ -553,6 +551,8 @@ def main():
         "err_vs_oracle_sample": err_oracle,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "envelope_advance": envelope,
+        "output_pass_ms": output_pass,
     }
     print(json.dumps(line), flush=True)
 
