@@ -37,10 +37,15 @@ struct MxGeom {
 
 // Member m of a batch owns slots [m * pstride, m * pstride + npw[m]) of every
 // array (npw null: one member with np particles).
+// Pairs that every kernel reads together sit in one element, so a particle's
+// state is five loads: cell, offsets (y, z), momenta (y, z), weight and the
+// p* increments (y, z).
 struct MxPart {
-    uint32_t *cell, *oy, *oz;
-    double *py, *pz;
-    float *w, *dpy, *dpz;
+    uint32_t *cell;
+    uint2 *off;    // in-cell offsets (y, z) in units of 2^-32 of a cell
+    double2 *p;    // p_y, p_z
+    float *w;
+    float2 *dp;    // p* - p (y, z)
     int np;
     int pstride;
     const int *npw;
@@ -75,8 +80,9 @@ __device__ __forceinline__ Pos load_pos(const MxPart &P, int p) {
     Pos r;
     r.j = (int)(c >> 16);
     r.k = (int)(c & 0xffffu);
-    r.fy = (double)P.oy[p] * kInvTwo32;
-    r.fz = (double)P.oz[p] * kInvTwo32;
+    const uint2 o = P.off[p];
+    r.fy = (double)o.x * kInvTwo32;
+    r.fz = (double)o.y * kInvTwo32;
     return r;
 }
 

@@ -94,10 +94,14 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
         w = P.w[p];
         if (advance) {
             // commit p <- p* (advance 2: K = 0, where p* keeps its initial value 0)
-            const double pyv = advance == 2 ? 0.0 : P.py[p] + (double)P.dpy[p];
-            const double pzv = advance == 2 ? 0.0 : P.pz[p] + (double)P.dpz[p];
-            P.py[p] = pyv;
-            P.pz[p] = pzv;
+            double pyv = 0.0, pzv = 0.0;
+            if (advance != 2) {
+                const double2 pp = P.p[p];
+                const float2 dd = P.dp[p];
+                pyv = pp.x + (double)dd.x;
+                pzv = pp.y + (double)dd.y;
+            }
+            P.p[p] = make_double2(pyv, pzv);
             const double ps = mx_gather(psim, q, n1, n2);
             const double s = g.dx / (1 + ps);
             double Y = (q.j + q.fy) + (s * pyv) * g.inv_d1;
@@ -109,8 +113,7 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
             split_coord(Y, n1, j, oy);
             split_coord(Z, n2, k, oz);
             P.cell[p] = ((uint32_t)j << 16) | (uint32_t)k;
-            P.oy[p] = oy;
-            P.oz[p] = oz;
+            P.off[p] = make_uint2(oy, oz);
             q.j = j;
             q.k = k;
             q.fy = (double)oy * kInvTwo32;
@@ -190,7 +193,8 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     if (valid) {
         const Pos q = load_pos(P, p);
         w = P.w[p];
-        const double py = P.py[p], pz = P.pz[p];
+        const double2 pp = P.p[p];
+        const double py = pp.x, pz = pp.y;
         const ValGrad ga = mx_gather_vg(asq, q, n1, n2, g.inv_d1, g.inv_d2);
         const ValGrad gp = mx_gather_vg(psi, q, n1, n2, g.inv_d1, g.inv_d2);
         const double a = ga.v, ps = gp.v;
@@ -199,7 +203,8 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         if (first_iter) {
             gm = gamma_of(py * py + pz * pz, a, ps, inv1ps);  // S4 (:373-374)
         } else {
-            const double pys = py + (double)P.dpy[p], pzs = pz + (double)P.dpz[p];  // P6 of the previous iteration (:503-505)
+            const float2 dd = P.dp[p];
+            const double pys = py + (double)dd.x, pzs = pz + (double)dd.y;  // P6 of the previous iteration (:503-505)
             gm = gamma_of(0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs), a, ps, inv1ps);
         }
         const double day = ga.dy, daz = ga.dz, dpy = gp.dy, dpz = gp.dz;
@@ -214,8 +219,7 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         pzn += s * gm * dpz;
         pzn += dx * byp;
         const float dyf = (float)(pyn - py), dzf = (float)(pzn - pz);
-        P.dpy[p] = dyf;
-        P.dpz[p] = dzf;
+        P.dp[p] = make_float2(dyf, dzf);
         // P2 (:439-442) with the stored p* = p + increment
         const double pyst = py + (double)dyf, pzst = pz + (double)dzf;
         double Ym = (q.j + q.fy) + (0.5 * s * pyst) * g.inv_d1;
@@ -282,8 +286,10 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
     if (valid) {
         q = load_pos(P, p);
         const double w = P.w[p];
-        const double py = P.py[p], pz = P.pz[p];
-        const double pys = py + (double)P.dpy[p], pzs = pz + (double)P.dpz[p];
+        const double2 pp = P.p[p];
+        const float2 dd = P.dp[p];
+        const double py = pp.x, pz = pp.y;
+        const double pys = py + (double)dd.x, pzs = pz + (double)dd.y;
         const double ps = mx_gather(psi, q, n1, n2);
         const double inv1ps = 1.0 / (1 + ps);
         const double s = g.dx * inv1ps;
@@ -379,7 +385,8 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
     if (valid) {
         q = load_pos(P, p);
         const double w = P.w[p];
-        const double py = P.py[p], pz = P.pz[p];
+        const double2 pp = P.p[p];
+        const double py = pp.x, pz = pp.y;
         const double ps = mx_gather(psi, q, n1, n2);
         const double a = mx_gather(asq, q, n1, n2);
         const double inv1ps = 1.0 / (1 + ps);
@@ -708,13 +715,10 @@ __global__ void k_sort_scatter(MxPart P, MxPart Q, const unsigned *offset, unsig
     const unsigned d = offset[t] + atomicAdd(&cursor[t], 1u);
     if (d >= (unsigned)P.np) return;
     Q.cell[d] = c;
-    Q.oy[d] = P.oy[p];
-    Q.oz[d] = P.oz[p];
-    Q.py[d] = P.py[p];
-    Q.pz[d] = P.pz[p];
+    Q.off[d] = P.off[p];
+    Q.p[d] = P.p[p];
     Q.w[d] = P.w[p];
-    Q.dpy[d] = P.dpy[p];
-    Q.dpz[d] = P.dpz[p];
+    Q.dp[d] = P.dp[p];
     perm2[d] = perm[p];
 }
 
@@ -722,13 +726,10 @@ __global__ void k_sort_copyback(MxPart P, MxPart Q, int *perm, const int *perm2)
     const int p = blockIdx.x * kPT + threadIdx.x;
     if (p >= P.np) return;
     P.cell[p] = Q.cell[p];
-    P.oy[p] = Q.oy[p];
-    P.oz[p] = Q.oz[p];
-    P.py[p] = Q.py[p];
-    P.pz[p] = Q.pz[p];
+    P.off[p] = Q.off[p];
+    P.p[p] = Q.p[p];
     P.w[p] = Q.w[p];
-    P.dpy[p] = Q.dpy[p];
-    P.dpz[p] = Q.dpz[p];
+    P.dp[p] = Q.dp[p];
     perm[p] = perm2[p];
 }
 
@@ -766,7 +767,7 @@ struct MixedState {
 static MixedState *mstate(Wake *w) { return static_cast<MixedState *>(w->mixed); }
 
 static void free_part(MxPart &P) {
-    void *arrs[] = {P.cell, P.oy, P.oz, P.py, P.pz, P.w, P.dpy, P.dpz};
+    void *arrs[] = {P.cell, P.off, P.p, P.w, P.dp};
     for (void *a : arrs)
         if (a) (void)hipFree(a);
     P = MxPart{};
@@ -775,13 +776,10 @@ static void free_part(MxPart &P) {
 static int alloc_part(MxPart &P, size_t n) {
     n = n ? n : 1;
     QP_HIP_CHECK(hipMalloc(&P.cell, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.oy, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.oz, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.py, 8 * n));
-    QP_HIP_CHECK(hipMalloc(&P.pz, 8 * n));
+    QP_HIP_CHECK(hipMalloc(&P.off, 8 * n));
+    QP_HIP_CHECK(hipMalloc(&P.p, 16 * n));
     QP_HIP_CHECK(hipMalloc(&P.w, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.dpy, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.dpz, 4 * n));
+    QP_HIP_CHECK(hipMalloc(&P.dp, 8 * n));
     return QP_OK;
 }
 
@@ -894,8 +892,8 @@ static int build_initial_state(Wake *w, MixedState *m) {
     const int n1 = w->p.ny, n2 = w->p.nz;
     const int ntk = (n2 + kTile - 1) / kTile;
     const size_t total = (size_t)w->nw * m->pstride;
-    std::vector<uint32_t> c2(total, 0), y2(total, 0), z2(total, 0);
-    std::vector<double> py(total, 0.0), pz(total, 0.0);
+    std::vector<uint32_t> c2(total, 0), oyz(2 * total, 0);
+    std::vector<double> pyz(2 * total, 0.0);
     std::vector<float> wt(total, 0.0f);
     m->h_perm.assign(total, -1);
     for (int mi = 0; mi < w->nw; ++mi) {
@@ -917,19 +915,17 @@ static int build_initial_state(Wake *w, MixedState *m) {
         for (size_t s = 0; s < np; ++s) {
             const int i = ord[s];
             c2[base + s] = cell[i];
-            y2[base + s] = oy[i];
-            z2[base + s] = oz[i];
-            py[base + s] = mb.h_py[i];
-            pz[base + s] = mb.h_pz[i];
+            oyz[2 * (base + s)] = oy[i];
+            oyz[2 * (base + s) + 1] = oz[i];
+            pyz[2 * (base + s)] = mb.h_py[i];
+            pyz[2 * (base + s) + 1] = mb.h_pz[i];
             wt[base + s] = (float)mb.h_w[i];
             m->h_perm[base + s] = i;
         }
     }
     QP_HIP_CHECK(hipMemcpy(m->I.cell, c2.data(), 4 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.oy, y2.data(), 4 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.oz, z2.data(), 4 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.py, py.data(), 8 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.pz, pz.data(), 8 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.off, oyz.data(), 8 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.p, pyz.data(), 16 * total, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 4 * total, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->perm0, m->h_perm.data(), sizeof(int) * total, hipMemcpyHostToDevice));
     m->init_valid = true;
@@ -946,15 +942,12 @@ int wake_upload_mixed(Wake *w) {
     const size_t n = (size_t)w->nw * m->pstride;
     hipStream_t s = w->stream();
     QP_HIP_CHECK(hipMemcpyAsync(m->P.cell, m->I.cell, 4 * n, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.oy, m->I.oy, 4 * n, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.oz, m->I.oz, 4 * n, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.py, m->I.py, 8 * n, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.pz, m->I.pz, 8 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.off, m->I.off, 8 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.p, m->I.p, 16 * n, hipMemcpyDeviceToDevice, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->P.w, m->I.w, 4 * n, hipMemcpyDeviceToDevice, s));
     // py_next = pz_next = 0 (src/System_3d.cpp:740-741): the first slice reads no
     // increment, and with K = 0 the commit writes p = 0 directly
-    QP_HIP_CHECK(hipMemsetAsync(m->P.dpy, 0, 4 * n, s));
-    QP_HIP_CHECK(hipMemsetAsync(m->P.dpz, 0, 4 * n, s));
+    QP_HIP_CHECK(hipMemsetAsync(m->P.dp, 0, 8 * n, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->perm, m->perm0, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
     return QP_OK;
 }
@@ -967,22 +960,20 @@ int wake_download_mixed(Wake *w, int member, double *y, double *z, double *py, d
     }
     const size_t np = w->mem[member].np, n = (size_t)m->pstride, base = (size_t)member * m->pstride;
     QP_HIP_CHECK(hipStreamSynchronize(w->stream()));
-    std::vector<uint32_t> c(n), oy(n), oz(n);
-    std::vector<double> fy(n), fz(n);
+    std::vector<uint32_t> c(n), oyz(2 * n);
+    std::vector<double> pyz(2 * n);
     std::vector<int> perm(n);
     QP_HIP_CHECK(hipMemcpy(c.data(), m->P.cell + base, 4 * n, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(oy.data(), m->P.oy + base, 4 * n, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(oz.data(), m->P.oz + base, 4 * n, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(fy.data(), m->P.py + base, 8 * n, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(fz.data(), m->P.pz + base, 8 * n, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(oyz.data(), m->P.off + base, 8 * n, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(hipMemcpy(pyz.data(), m->P.p + base, 16 * n, hipMemcpyDeviceToHost));
     QP_HIP_CHECK(hipMemcpy(perm.data(), m->perm + base, sizeof(int) * n, hipMemcpyDeviceToHost));
     for (size_t s = 0; s < n; ++s) {
         const int i = perm[s];
         if (i < 0 || (size_t)i >= np) continue;
-        if (y) y[i] = ((double)(c[s] >> 16) + oy[s] * kInvTwo32) * w->p.dy;
-        if (z) z[i] = ((double)(c[s] & 0xffffu) + oz[s] * kInvTwo32) * w->p.dz;
-        if (py) py[i] = fy[s];
-        if (pz) pz[i] = fz[s];
+        if (y) y[i] = ((double)(c[s] >> 16) + oyz[2 * s] * kInvTwo32) * w->p.dy;
+        if (z) z[i] = ((double)(c[s] & 0xffffu) + oyz[2 * s + 1] * kInvTwo32) * w->p.dz;
+        if (py) py[i] = pyz[2 * s];
+        if (pz) pz[i] = pyz[2 * s + 1];
     }
     return QP_OK;
 }
