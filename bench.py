@@ -537,7 +537,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "storage": "particles: u32 cell + u32 offsets, f64 momenta, f32 weight and p* increments; grids and solves f64",
+        "storage": "particles: u32 cell + u32x2 offsets, f64x2 momenta, f32 weight, f32x2 p* increments; grids and solves f64",
         "data": "synthetic",
         "config": wl.describe(),
         "solves_per_s": solves / (ms_per_step * 1e-3),
