@@ -155,7 +155,8 @@ __device__ __forceinline__ double gamma_of(double p2, double a, double ps, doubl
 
 __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, const double *psi, const double *by,
                                               const double *bz, double *src_acc, double *rho, double *jx,
-                                              double *chi, double *jyn, double *jzn, int first_iter, Ctl *ctl) {
+                                              double *chi, double *jyn, double *jzn, int first_iter, int init_rho,
+                                              Ctl *ctl) {
     const int wk = blockIdx.y;
     ctl += wk;
     MX_SKIP(ctl);
@@ -173,13 +174,15 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     jyn += go;
     jzn += go;
     const int slice = mx_slice(ctl, si);
-    {  // rho = rho_bunch(i), jx = jx_bunch(i), chi = 0, jy* = jz* = 0 before the C phase (:473-478, :511-518)
+    {  // jx = jx_bunch(i), jy* = jz* = 0 before the C phase (:473-478, :511-518); rho =
+       // rho_bunch(i) before the C phase that deposits rho (the last), chi = 0 before the
+       // one that deposits chi (the first)
         const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
         const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
         for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
-            rho[t] = rb ? rb[t] : 0.0;
+            if (init_rho) rho[t] = rb ? rb[t] : 0.0;
             jx[t] = jb ? jb[t] : 0.0;
-            chi[t] = 0.0;
+            if (first_iter) chi[t] = 0.0;
             jyn[t] = 0.0;
             jzn[t] = 0.0;
         }
@@ -254,14 +257,21 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
 }
 
 // ------------------------------------------------------------------ C phase
-// ngr = 5 grids: jy*, jz*, rho, jx, chi.
+// Grids jy*, jz*, jx, and rho (RHO) and chi (CHI).  Within a slice only jx and
+// j* feed the next field solve; rho and chi are read only by the slice's
+// output.  rho depends on the iteration's p*, so only the last iteration
+// deposits it.  chi = -n/(1+psi) at the slice's old positions is the same in
+// every iteration (:526), so the first iteration deposits it.  A pass that
+// stops in a later iteration then still has it, and no iteration repeats it.
 // Second launch bound (waves per SIMD): C fits 80 VGPRs at 6 waves without
 // spills (5 otherwise), which hides more of its gather -> deposit -> flush chain
 // in batches (256^2 x 10 members: 69.8 -> 65.9 us).  The same bound on B
 // (56 VGPRs, 8 waves) made it slower (77.9 -> 83.2 us) and is not used.
+template <bool RHO, bool CHI>
 __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, const double *psi,
                                               const double *psim, double *rho, double *jx, double *chi,
                                               double *jyn, double *jzn, Ctl *ctl) {
+    constexpr int G = 1 + (RHO ? 1 : 0) + (CHI ? 1 : 0);  // node grids
     const int wk = blockIdx.y;
     ctl += wk;
     MX_SKIP(ctl);
@@ -302,15 +312,14 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         vjz = wm * pzs;
         const double a = mx_gather(asq, q, n1, n2);  // P6 (:501-506)
         const double p2 = 0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs);
-        const double gm = gamma_of(p2, a, ps, inv1ps);
         const double px = 0.5 * (1 + p2 + a - (1 + ps) * (1 + ps)) * inv1ps;
         // P7 (:523-526) with vx = px/gamma and 1/(1-vx) = gamma/(gamma-px) = gamma/(1+psi)
         // (gamma - px = 1 + psi exactly, from the two closed forms): n vx/(1-vx) =
         // n px/(1+psi), n/(1-vx) = n gamma/(1+psi), -n/((1-vx) gamma) = -n/(1+psi).
         // No divisions, and no cancellation in gamma - px near psi -> -1.
         vjx = w * px * inv1ps;
-        vrho = w * gm * inv1ps;
-        vchi = -w * inv1ps;
+        if (RHO) vrho = w * gamma_of(p2, a, ps, inv1ps) * inv1ps;
+        if (CHI) vchi = -w * inv1ps;
     }
     if (threadIdx.x == 0) {
         sh[4 * (kPT / 64)] = q.j;
@@ -324,7 +333,7 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
     ax_node(qm.k, qm.fz, n2, za1, za2, zaf);
     ax_node(qm.j, qm.fy, n1, yb1, yb2, ybf);  // jz* on the (0, dz/2) grid
     ax_half(qm.k, qm.fz, n2, zb1, zb2, zbf);
-    ax_node(q.j, q.fy, n1, yc1, yc2, ycf);    // rho, jx, chi on the nodes
+    ax_node(q.j, q.fy, n1, yc1, yc2, ycf);    // jx, rho, chi on the nodes
     ax_node(q.k, q.fz, n2, zc1, zc2, zcf);
     int rmin = INT_MAX, rmax = INT_MIN, cmin = INT_MAX, cmax = INT_MIN;
     if (valid) {
@@ -333,31 +342,37 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         range_add(yc1, zc1, jref, kref, n1, n2, rmin, rmax, cmin, cmax);
     }
     const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
-    win_zero(lds, 5, win);
+    win_zero(lds, 2 + G, win);
     __syncthreads();
+    double *gc[3];  // node grids in deposit order
     {
-        double aa[4], ab[4], ac[12];
+        int ig = 0;
+        if (RHO) gc[ig++] = rho;
+        gc[ig++] = jx;
+        if (CHI) gc[ig] = chi;
+    }
+    {
+        double aa[4], ab[4], ac[4 * G];
         cic_w(vjy, yaf, zaf, aa);
         cic_w(vjz, ybf, zbf, ab);
-        cic_w(vrho, ycf, zcf, ac);
-        cic_w(vjx, ycf, zcf, ac + 4);
-        cic_w(vchi, ycf, zcf, ac + 8);
+        int ig = 0;
+        if (RHO) cic_w(vrho, ycf, zcf, ac + 4 * ig++);
+        cic_w(vjx, ycf, zcf, ac + 4 * ig++);
+        if (CHI) cic_w(vchi, ycf, zcf, ac + 4 * ig);
         // the quad fold also pays in VALU-bound batches: without it the same-address
         // LDS conflicts cost more (256^2 x 10 members: 61.4 us folded, 86.8 us not)
         bool fa = valid, fb = valid, fc = valid;
         quad_fold<4>(ya1 * n2 + za1, fa, aa);
         quad_fold<4>(yb1 * n2 + zb1, fb, ab);
-        quad_fold<12>(yc1 * n2 + zc1, fc, ac);
+        quad_fold<4 * G>(yc1 * n2 + zc1, fc, ac);
         if (fa) dep4w(lds, win, jyn, n1, n2, ya1, ya2, za1, za2, aa[0], aa[1], aa[2], aa[3]);
         if (fb) dep4w(lds + kWinCap, win, jzn, n1, n2, yb1, yb2, zb1, zb2, ab[0], ab[1], ab[2], ab[3]);
-        if (fc) {
-            double *const gc[3] = {rho, jx, chi};
-            dep4w_multi<3>(lds + 2 * kWinCap, win, gc, n1, n2, yc1, yc2, zc1, zc2, ac);
-        }
+        if (fc) dep4w_multi<G>(lds + 2 * kWinCap, win, gc, n1, n2, yc1, yc2, zc1, zc2, ac);
     }
     __syncthreads();
-    double *gl[5] = {jyn, jzn, rho, jx, chi};
-    win_flush(lds, gl, 5, win, n1, n2);
+    double *gl[2 + G] = {jyn, jzn};
+    for (int i = 0; i < G; ++i) gl[2 + i] = gc[i];
+    win_flush(lds, gl, 2 + G, win, n1, n2);
 }
 
 // -------------------------------------------------- S phase (slice 0, K = 0)
@@ -988,7 +1003,7 @@ struct MxRun {
     size_t nn;
     int nw, pb, gb;
     size_t sstride;
-    size_t ldsC, ldsS;
+    size_t ldsS;
 };
 
 #define MLAUNCH(kern, grid, block, lds, ...)                                            \
@@ -1127,14 +1142,26 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     } else {
         for (int it = 0; it < p.K; ++it) {
             prof_begin(w, "push_B");
+            const bool first_it = it == 0, last_it = it == p.K - 1;
             MLAUNCH(k_mx_B, pg, kPT, 0, m->P, R.g, si, (const double *)psi, (const double *)w->by,
                     (const double *)w->bz, w->src, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1],
-                    it == 0 ? 1 : 0, w->ctl);
+                    first_it ? 1 : 0, last_it ? 1 : 0, w->ctl);
             prof_end(w, "push_B");
             QP_TRY(psi_solve(R, s, slot, w->psim));
             prof_begin(w, "push_C");
-            MLAUNCH(k_mx_C, pg, kPT, R.ldsC, m->P, R.g, si, (const double *)psi, (const double *)w->psim, w->rho,
-                    w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
+            const size_t ldsc = sizeof(double) * kWinCap * (3 + (first_it ? 1 : 0) + (last_it ? 1 : 0));
+            if (first_it && last_it)
+                MLAUNCH((k_mx_C<true, true>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                        (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
+            else if (first_it)
+                MLAUNCH((k_mx_C<false, true>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                        (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
+            else if (last_it)
+                MLAUNCH((k_mx_C<true, false>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                        (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
+            else
+                MLAUNCH((k_mx_C<false, false>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                        (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
             prof_end(w, "push_C");
             QP_TRY(b_solve(R, s, slot));
         }
@@ -1343,7 +1370,6 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
         m->gexec[0] = m->gexec[1] = nullptr;
         m->graph_epoch = w->inputs_epoch;
     }
-    R.ldsC = sizeof(double) * 5 * kWinCap;
     R.ldsS = sizeof(double) * 3 * kWinCap;
 
     // members' lasers (a member without one gets amplitude 0)
