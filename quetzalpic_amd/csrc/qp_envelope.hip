@@ -157,6 +157,7 @@ __global__ __launch_bounds__(TB) void kw_cols_env(double2 *spec0, double2 *spec1
     __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N1];
     const int tid = threadIdx.x;
     for (int t = tid; t < N1; t += TB) twl[t] = tw1[t];
+    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
     const int l = blockIdx.x * W::T + tt;
     const bool valid = l < nh;

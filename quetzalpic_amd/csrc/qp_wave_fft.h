@@ -164,6 +164,21 @@ __device__ __forceinline__ double2 wf_tw(const double2 *twl, int t) {
     return w;
 }
 
+// Orders the LDS exchanges of one transform.  A transform of L <= 64 lanes
+// lies in one wavefront (L divides 64), whose LDS instructions execute in
+// program order, so a wavefront-scope fence (which keeps the compiler from
+// moving LDS accesses across it) is enough; wider transforms need the block
+// barrier.
+template <int L>
+__device__ __forceinline__ void wf_sync() {
+    if constexpr (L <= 64) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
+}
+
 // Passes after the first: bin holds sub-transforms of size NS.
 template <int N, int E, int S, int NS>
 __device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2 *bout, const double2 *twl, int j) {
@@ -194,23 +209,24 @@ __device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2
         }
     }
     if constexpr (!LAST) {
-        __syncthreads();
+        wf_sync<L>();
         wf_passes<N, E, S, NS * R>(v, bout, bin, twl, j);
     }
 }
 
 // Transform the E registers of every lane (see the header comment).  buf0 /
 // buf1: this transform's LDS regions (PADN each); twl: the N-entry table
-// e^{-2 pi i t/N} in LDS, written before the call (the first barrier inside
-// orders it).  Every thread of the block must call it.
+// e^{-2 pi i t/N} in LDS, written and published by a block barrier before the
+// call.  Every thread of the block must call it.
 template <int N, int E, int S>
 __device__ __forceinline__ void wave_fft(double2 (&v)[E], double2 *buf0, double2 *buf1, const double2 *twl, int j) {
+    constexpr int L = N / E;
     dft<E, S>(v);
     if constexpr (N > E) {
-        __syncthreads();  // earlier readers of buf0 are done
+        wf_sync<L>();  // earlier readers of buf0 are done
 #pragma unroll
         for (int q = 0; q < E; ++q) buf0[wf_pad(E * j + q)] = v[q];
-        __syncthreads();
+        wf_sync<L>();
         wf_passes<N, E, S, E>(v, buf0, buf1, twl, j);
     }
 }

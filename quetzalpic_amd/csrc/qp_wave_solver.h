@@ -65,6 +65,7 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
     double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     const int tid = threadIdx.x;
     for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
+    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
     const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
     double2 v[E];
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
     if (bpart) bpart += wk * bstride;
     const int tid = threadIdx.x;
     for (int t = tid; t < N1; t += TB) twl[t] = tw1[t];
+    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
     const int l = blockIdx.x * W::T + tt;
     const bool valid = l < nh;
@@ -184,6 +186,7 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
     sink.begin(wk);
     const int tid = threadIdx.x;
     for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
+    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
     const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
     double2 v[E];
