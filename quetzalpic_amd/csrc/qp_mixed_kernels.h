@@ -194,44 +194,15 @@ __device__ __forceinline__ double mx_gather_hy(const double *a, const Pos &q, in
     ax_node(q.k, q.fz, n2, k1, k2, zf);
     return gather4u(a, n2, j1, j2, k1, k2, yf, zf);
 }
-// Staggered-derivative gathers (src/array_utils.cpp:240-288).
-// The half-node differences are interpolated first and scaled once by 1/d.
-__device__ __forceinline__ double mx_yder(const double *a, const Pos &q, int n1, int n2, double inv_d1) {
-    int j1, j2, k1, k2;
-    double yf, zf;
-    ax_half(q.j, q.fy, n1, j1, j2, yf);
-    const int j3 = j2 + 1 == n1 ? 0 : j2 + 1;
-    ax_node(q.k, q.fz, n2, k1, k2, zf);
-    const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2),
-                   r3 = __umul24((unsigned)j3, (unsigned)n2);
-    const double a21 = ld_at(a, r2 + k1), a22 = ld_at(a, r2 + k2);
-    const double g11 = a21 - ld_at(a, r1 + k1);
-    const double g21 = ld_at(a, r3 + k1) - a21;
-    const double g12 = a22 - ld_at(a, r1 + k2);
-    const double g22 = ld_at(a, r3 + k2) - a22;
-    return (g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf) * inv_d1;
-}
-__device__ __forceinline__ double mx_zder(const double *a, const Pos &q, int n1, int n2, double inv_d2) {
-    int j1, j2, k1, k2;
-    double yf, zf;
-    ax_node(q.j, q.fy, n1, j1, j2, yf);
-    ax_half(q.k, q.fz, n2, k1, k2, zf);
-    const int k3 = k2 + 1 == n2 ? 0 : k2 + 1;
-    const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2);
-    const double a12 = ld_at(a, r1 + k2), a22 = ld_at(a, r2 + k2);
-    const double g11 = a12 - ld_at(a, r1 + k1);
-    const double g21 = a22 - ld_at(a, r2 + k1);
-    const double g12 = ld_at(a, r1 + k3) - a12;
-    const double g22 = ld_at(a, r2 + k3) - a22;
-    return (g11 * (1 - yf) * (1 - zf) + g21 * yf * (1 - zf) + g12 * (1 - yf) * zf + g22 * yf * zf) * inv_d2;
-}
-
-// Node value and both staggered derivatives of a node-origin grid at q, as
-// mx_gather, mx_yder and mx_zder with the same arithmetic, from the eight
-// nodes the three stencils share: the cell's 2x2 corners (c), the row beyond
-// them that the y-derivative stencil adds (e_y: row j + 2 when the half-node
-// stencil starts at j, else row j - 1) and the column the z-derivative stencil
-// adds (e_z), instead of 4 + 6 + 6 loads.
+// Node value and both staggered derivatives of a node-origin grid at q: the
+// gather of mx_gather and the derivative gathers of src/array_utils.cpp:240-288
+// (half-node differences (f(j+1) - f(j))/d at j1 = floor(y/d - 1/2), bilinear
+// between the two half nodes and the two nodes of the other axis; the
+// differences are interpolated first and scaled once by 1/d).  The three
+// stencils share eight nodes: the cell's 2x2 corners (c), the row beyond them
+// that the y-derivative stencil adds (e_y: row j + 2 when the half-node stencil
+// starts at j, else row j - 1) and the column the z-derivative stencil adds
+// (e_z), so they take eight loads instead of 4 + 6 + 6.
 struct ValGrad {
     double v, dy, dz;
 };
