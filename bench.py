@@ -54,12 +54,12 @@ def solve_bytes(n1, n2):
     return (real + spec) + 2 * spec + (spec + real)
 
 
-def read_traffic(kernel_cls):
+def read_traffic(kernel_cls, workload):
     """Per-launch HBM bytes of a kernel from the committed rocprofv3 --pmc summary
-    (profiles/traffic_<class>_C2.txt: lines 'FETCH_KIB <v>', 'WRITE_KIB <v>',
+    (profiles/traffic_<class>_<workload>.txt: lines 'FETCH_KIB <v>', 'WRITE_KIB <v>',
     'FETCH_FACTOR <f>', 'WRITE_FACTOR <f>', the factors from a device copy of
-    known size); None when absent."""
-    p = os.path.join(ROOT, "profiles", f"traffic_{kernel_cls}_C2.txt")
+    known size); None when there is no summary for this workload."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{kernel_cls}_{workload}.txt")
     if not os.path.exists(p):
         return None
     vals = {}
@@ -462,7 +462,7 @@ def main():
         dom = max(("push_A", "push_B", "push_C"), key=lambda c: classes[c]["total_ms"])
         dom_bytes = bytes_per_particle(dom, wl.K) * wl.n_particles
         achieved = dom_bytes / (classes[dom]["mean_us"] * 1e-6) / 1e9
-        traffic = read_traffic(dom)
+        traffic = read_traffic(dom, args.workload)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_mx_{dom[-1]}",
                     "algorithmic_bytes_per_launch": dom_bytes}
