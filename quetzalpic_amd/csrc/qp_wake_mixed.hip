@@ -530,18 +530,18 @@ struct SrcPsi {
 struct SrcB {
     const double *by, *bz, *jx, *jy, *jz, *jyn, *jzn;
     int n1, n2;
-    double D, dx, dy, dz;
+    double D, inv_dx, inv_dy, inv_dz;  // the differences are scaled by reciprocals (no divisions)
     __device__ double operator()(int wk, int which, int j, int k) const {
         const size_t go = (size_t)wk * n1 * n2;
         const size_t t = go + (size_t)j * n2 + k;
         if (which == 0) {
             const double jn = k < n2 - 1 ? jx[t + 1] : jx[go + (size_t)j * n2];
-            const double djz = (jz[t] - jzn[t]) / dx;
-            return -D * by[t] - (jn - jx[t]) / dz + djz;
+            const double djz = (jz[t] - jzn[t]) * inv_dx;
+            return -D * by[t] - (jn - jx[t]) * inv_dz + djz;
         }
-        const double djy = (jy[t] - jyn[t]) / dx;
-        if (j < n1 - 1) return -D * bz[t] + (jx[t + n2] - jx[t]) / dy - djy;
-        return -D * bz[go + k] + (jx[go + k] - jx[t]) / dy - djy;
+        const double djy = (jy[t] - jyn[t]) * inv_dx;
+        if (j < n1 - 1) return -D * bz[t] + (jx[t + n2] - jx[t]) * inv_dy - djy;
+        return -D * bz[go + k] + (jx[go + k] - jx[t]) * inv_dy - djy;
     }
 };
 
@@ -554,6 +554,7 @@ struct SinkPsiFinal {
     double thr;
     double b;
     size_t bstride;
+    double scale;  // 1 / (n1 n2): the unnormalised c2r output times this (exact for 2^p sides)
     __device__ void begin(int wk) {
         const double *bp = bpart + wk * bstride;
         double s1 = 0.0, s2 = 0.0;
@@ -568,7 +569,7 @@ struct SinkPsiFinal {
         b = sraw / n1 / n2 / (n1 + n2 - 1);
     }
     __device__ void operator()(int wk, int, int j, int k, double v) const {
-        double r = v / n1 / n2 - b;
+        double r = v * scale - b;
         if (r < thr) r = thr;
         dst[(size_t)wk * n1 * n2 + (size_t)j * n2 + k] = r;
     }
@@ -581,9 +582,10 @@ struct SinkBPair {
     double thr;
     Ctl *ctl;
     int slot;
+    double scale;  // 1 / (n1 n2)
     __device__ void begin(int) {}
     __device__ void operator()(int wk, int which, int j, int k, double v) const {
-        const double b = v / n1 / n2;
+        const double b = v * scale;
         (which == 0 ? by : bz)[(size_t)wk * n1 * n2 + (size_t)j * n2 + k] = b;
         if (fabs(b) > thr || isnan(b)) {
             ctl[wk].stop_acc = 1;
@@ -601,21 +603,22 @@ struct SinkBPair {
 struct SrcCurl {
     const double *jy, *jz;
     int n1, n2;
-    double dy, dz;
+    double inv_dy, inv_dz;
     __device__ double operator()(int wk, int, int j, int k) const {
         const size_t go = (size_t)wk * n1 * n2;
         const int kp = k + 1 == n2 ? 0 : k + 1, jp = j + 1 == n1 ? 0 : j + 1;
         const size_t t = go + (size_t)j * n2 + k;
-        return (jy[go + (size_t)j * n2 + kp] - jy[t]) / dz - (jz[go + (size_t)jp * n2 + k] - jz[t]) / dy;
+        return (jy[go + (size_t)j * n2 + kp] - jy[t]) * inv_dz - (jz[go + (size_t)jp * n2 + k] - jz[t]) * inv_dy;
     }
 };
 
 struct SinkMember {
     double *dst;
     int n1, n2;
+    double scale;  // 1 / (n1 n2)
     __device__ void begin(int) {}
     __device__ void operator()(int wk, int, int j, int k, double v) const {
-        dst[(size_t)wk * n1 * n2 + (size_t)j * n2 + k] = v / n1 / n2;
+        dst[(size_t)wk * n1 * n2 + (size_t)j * n2 + k] = v * scale;
     }
 };
 
@@ -1018,7 +1021,8 @@ int psi_solve(MxRun &R, hipStream_t s, int slot, double *dst) {
     MixedState *m = R.m;
     const int nw = R.nw;
     SrcPsi src{w->src, w->rho_ion, w->rho_b_src(), w->jx_b_src(), w->bsrc_stride(), w->ctl, slot, S.n2, R.nn};
-    SinkPsiFinal snk{dst, m->bpart, m->ncb, S.n1, S.n2, w->p.psi_threshold - 1, 0.0, m->bstride};
+    SinkPsiFinal snk{dst, m->bpart, m->ncb, S.n1, S.n2, w->p.psi_threshold - 1, 0.0, m->bstride,
+                     1.0 / ((double)S.n1 * S.n2)};
     if (S.wave) {
         prof_begin(w, "solve");
         QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[0], 1, s, (const Ctl *)w->ctl, 1, nw));
@@ -1057,8 +1061,8 @@ int b_solve(MxRun &R, hipStream_t s, int slot) {
     const int nw = R.nw;
     double *jy = m->jbuf[1 - slot][0], *jz = m->jbuf[1 - slot][1];
     double *jyn = m->jbuf[slot][0], *jzn = m->jbuf[slot][1];
-    SrcB src{w->by, w->bz, w->jx, jy, jz, jyn, jzn, S.n1, S.n2, p.D, p.dx, p.dy, p.dz};
-    SinkBPair snk{w->by, w->bz, S.n1, S.n2, p.b_threshold, w->ctl, slot};
+    SrcB src{w->by, w->bz, w->jx, jy, jz, jyn, jzn, S.n1, S.n2, p.D, 1.0 / p.dx, 1.0 / p.dy, 1.0 / p.dz};
+    SinkBPair snk{w->by, w->bz, S.n1, S.n2, p.b_threshold, w->ctl, slot, 1.0 / ((double)S.n1 * S.n2)};
     if (S.wave) {
         prof_begin(w, "solve");
         QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[1], 2, s, (const Ctl *)w->ctl, 2, nw));
@@ -1094,8 +1098,8 @@ int bx_solve(MxRun &R, hipStream_t s, int slot) {
     MixedState *m = R.m;
     const qp_wake_params &p = w->p;
     const int nw = R.nw;
-    SrcCurl src{m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], S.n1, S.n2, p.dy, p.dz};
-    SinkMember snk{w->bx, S.n1, S.n2};
+    SrcCurl src{m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], S.n1, S.n2, 1.0 / p.dy, 1.0 / p.dz};
+    SinkMember snk{w->bx, S.n1, S.n2, 1.0 / ((double)S.n1 * S.n2)};
     if (S.wave) {
         QP_TRY(wlaunch_rows_fwd(S, src, S.spec[0], S.spec[0], 1, s, (const Ctl *)w->ctl, 1, nw));
         QP_TRY(wlaunch_cols(S, 0.0, 1, nullptr, s, (const Ctl *)w->ctl, 1, nw));
