@@ -555,15 +555,21 @@ struct SinkPsiFinal {
     double b;
     size_t bstride;
     double scale;  // 1 / (n1 n2): the unnormalised c2r output times this (exact for 2^p sides)
+    // Every wavefront sums the partial sums itself (no block barrier); lane 0's
+    // result is broadcast, so every lane and every block uses the same value.
     __device__ void begin(int wk) {
         const double *bp = bpart + wk * bstride;
         double s1 = 0.0, s2 = 0.0;
-        for (int i = threadIdx.x; i < ncb; i += blockDim.x) {
+        for (int i = threadIdx.x & 63; i < ncb; i += 64) {
             s1 += bp[2 * i];
             s2 += bp[2 * i + 1];
         }
-        s1 = block_sum(s1);
-        s2 = block_sum(s2);
+        for (int off = 32; off > 0; off >>= 1) {
+            s1 += __shfl_xor(s1, off);
+            s2 += __shfl_xor(s2, off);
+        }
+        s1 = __shfl(s1, 0);
+        s2 = __shfl(s2, 0);
         const double g00 = bp[2 * ncb];
         const double sraw = s1 + (double)n2 * g00 - s2;
         b = sraw / n1 / n2 / (n1 + n2 - 1);
