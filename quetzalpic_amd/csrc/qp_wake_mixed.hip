@@ -67,23 +67,6 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
     chi += go;
     const LaserGauss L = Ls[wk];
     const int slice = mx_slice(ctl, si);
-    if (laser || init_rho) {
-        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
-        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
-        for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
-            if (laser) {
-                const int j = (int)(t / n2), k = (int)(t % n2);
-                const double x = slice * g.dx, y = j * g.d1, z = k * g.d2;
-                asq2[t] = L.amp * exp(-(x - L.x0) * (x - L.x0) / L.wx / L.wx - (y - L.y0) * (y - L.y0) / L.wy / L.wy -
-                                      (z - L.z0) * (z - L.z0) / L.wz / L.wz);
-            }
-            if (init_rho) {
-                rho[t] = rb ? rb[t] : 0.0;
-                jx[t] = jb ? jb[t] : 0.0;
-                chi[t] = 0.0;
-            }
-        }
-    }
     const int lp = blockIdx.x * kPT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
     const unsigned p = (unsigned)(wk * P.pstride + lp);
@@ -145,6 +128,26 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
     __syncthreads();
     double *gl[1] = {src_acc};
     win_flush(lds, gl, 1, win, n1, n2);
+    // The slice's laser plane and the S-phase initial rho / jx / chi last, so that
+    // these stores do not queue ahead of the particle loads (nothing in this kernel
+    // reads them).
+    if (laser || init_rho) {
+        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
+        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
+        for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
+            if (laser) {
+                const int j = (int)(t / n2), k = (int)(t % n2);
+                const double x = slice * g.dx, y = j * g.d1, z = k * g.d2;
+                asq2[t] = L.amp * exp(-(x - L.x0) * (x - L.x0) / L.wx / L.wx - (y - L.y0) * (y - L.y0) / L.wy / L.wy -
+                                      (z - L.z0) * (z - L.z0) / L.wz / L.wz);
+            }
+            if (init_rho) {
+                rho[t] = rb ? rb[t] : 0.0;
+                jx[t] = jb ? jb[t] : 0.0;
+                chi[t] = 0.0;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------ B phase
@@ -174,19 +177,6 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     jyn += go;
     jzn += go;
     const int slice = mx_slice(ctl, si);
-    {  // jx = jx_bunch(i), jy* = jz* = 0 before the C phase (:473-478, :511-518); rho =
-       // rho_bunch(i) before the C phase that deposits rho (the last), chi = 0 before the
-       // one that deposits chi (the first)
-        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
-        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
-        for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
-            if (init_rho) rho[t] = rb ? rb[t] : 0.0;
-            jx[t] = jb ? jb[t] : 0.0;
-            if (first_iter) chi[t] = 0.0;
-            jyn[t] = 0.0;
-            jzn[t] = 0.0;
-        }
-    }
     const double *asq = mx_asq(ctl, si, nn, go);
     const int lp = blockIdx.x * kPT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
@@ -254,6 +244,21 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     __syncthreads();
     double *gl[1] = {src_acc};
     win_flush(lds, gl, 1, win, n1, n2);
+    // Last, so that these stores do not queue ahead of the particle loads (nothing
+    // in this kernel reads the grids).
+    {  // jx = jx_bunch(i), jy* = jz* = 0 before the C phase (:473-478, :511-518); rho =
+       // rho_bunch(i) before the C phase that deposits rho (the last), chi = 0 before the
+       // one that deposits chi (the first)
+        const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
+        const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
+        for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
+            if (init_rho) rho[t] = rb ? rb[t] : 0.0;
+            jx[t] = jb ? jb[t] : 0.0;
+            if (first_iter) chi[t] = 0.0;
+            jyn[t] = 0.0;
+            jzn[t] = 0.0;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ C phase
