@@ -49,8 +49,11 @@ def check(got, want, tol_main=TOL_PSI_EX, tol_other=1e-4):
     return errs
 
 
-@pytest.mark.parametrize("K", [5, 1, 0])
+@pytest.mark.parametrize("K", [5, 3, 2, 1, 0])
 def test_mixed_matches_oracle(qp_ctx, oracle, K):
+    """K = 5 and 3: first, middle and last iterations (chi deposited by the first,
+    rho by the last); K = 2: no middle one; K = 1: one iteration deposits all;
+    K = 0: the S phase only."""
     g, y, z, w, a = case(oracle, K=K)
     want, st0, _ = oracle.wake_run(g, y, z, w, a)
     wk, got, st1, _ = run_mixed(qp_ctx, g, y, z, w, a)
@@ -121,3 +124,23 @@ def test_mixed_larger_nonlinear(qp_ctx, oracle):
     for f in ("psi", "ex"):
         mx, rms = rel_err(got[f], want[f])
         assert mx < TOL_PSI_EX and rms < TOL_PSI_EX, (f, mx, rms)
+
+
+def test_mixed_stopped_pass_matches_oracle(qp_ctx, oracle):
+    """THRESHOLD_B = 0.3 with a0 = 8: the pass stops at the oracle's slice.  chi
+    of the stopped slice (deposited by the first iteration of the slice; its
+    value is the same in every iteration, so the iteration the stop comes in does
+    not matter) and the fields of the slices computed before the stop match the
+    oracle."""
+    g = oracle.grid_from_box(4.0, 20.0, 20.0, 0.1, 20.0 / 32, 20.0 / 32, K=5, b_threshold=0.3)
+    y, z, w = oracle.init_lattice(g, 2, 2)
+    a = oracle.gaussian_a_sqr(g, 8.0, 4.0, 6.0, 6.0, 2.0, 10.0, 10.0)
+    want, st0, ss0 = oracle.wake_run(g, y, z, w, a)
+    wk, got, st1, ss1 = run_mixed(qp_ctx, g, y, z, w, a)
+    wk.close()
+    assert st0 == st1 == 1 and ss1 == ss0 > 0
+    assert rel_err(got["chi"][ss0], want["chi"][ss0])[0] < 1e-6
+    assert rel_err(got["chi"], want["chi"])[0] < 1e-6
+    done = slice(ss0 + 1, None)   # slices computed before the stop
+    for f in ("psi", "ex"):
+        assert rel_err(got[f][done], want[f][done])[0] < TOL_PSI_EX, f
