@@ -1,13 +1,13 @@
 // qp_mixed_kernels.h -- device code of the fused mixed-precision slice step.
 //
 // Particle layout (structure of arrays, 36 B persistent + 8 B scratch):
-//   cell  uint32  (j << 16) | k          node-origin cell of the particle
-//   oy,oz uint32  in-cell offset in units of 2^-32 cell (Y = j + oy 2^-32)
-//   py,pz double  transverse momenta (they accumulate over every slice)
-//   w     float   macro-particle weight (wake_particle_2d::n)
-//   dpy,dpz float increment p* - p of the current iteration's predicted
-//                 momenta (scratch): its rounding is relative to the small
-//                 increment, not to p
+//   cell  uint32   (j << 16) | k          node-origin cell of the particle
+//   off   uint2    in-cell offsets (y, z) in units of 2^-32 cell (Y = j + off.x 2^-32)
+//   p     double2  transverse momenta (y, z) (they accumulate over every slice)
+//   w     float    macro-particle weight (wake_particle_2d::n)
+//   dp    float2   increment p* - p of the current iteration's predicted
+//                  momenta (scratch): its rounding is relative to the small
+//                  increment, not to p
 // Arithmetic is FP64 throughout (FP64 is cheap next to the memory traffic);
 // only the stored state is narrowed.  Positions are advanced in FP64 cell
 // units and re-split with round-to-nearest into cell + offset, so the
@@ -16,7 +16,8 @@
 // The 35 particle sweeps of a reference slice (src/System_3d.cpp:328-619)
 // become 2K+1 launches: A (commit + advance + psi source deposit), then per
 // iteration B (gamma, predictor, half step, psi_middle source deposit) and
-// C (transverse current at the half step, gamma/px, rho/jx/chi deposits).
+// C (transverse current at the half step, gamma/px, jx deposits; chi in the
+// first iteration of a slice, rho in the last).
 #pragma once
 
 #include "qp_solver_kernels.h"
