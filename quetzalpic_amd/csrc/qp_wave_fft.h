@@ -1,8 +1,8 @@
-// qp_wave_fft.h -- register-resident complex FFTs of power-of-two length N in
-// [16, 1024] (device code).
+// qp_wave_fft.h -- register-resident complex FFTs of length N = 2^p in
+// [16, 1024] or N = 3 * 2^p in [48, 384] (device code).
 //
-// E elements per lane (E in {4, 8, 16}), L = N/E lanes per transform, a
-// block of TB threads carries T = TB/L transforms.  Lane j of a transform
+// E elements per lane (E in {4, 8, 16} for 2^p, 12 for 3 * 2^p), L = N/E lanes
+// per transform, a block of TB threads carries T = TB/L transforms.  Lane j of a transform
 // holds element j + L*m in register m, on entry and on exit (natural order
 // both ways), so a forward transform, a pointwise operation and an inverse
 // transform chain without an exchange.  Stockham passes: the first has radix
@@ -10,7 +10,9 @@
 // and radix R = min(E, N/Ns) has butterflies jj = j + L b (b < E/R) reading
 // in[jj + r N/R], twiddled by w^{k r N/(Ns R)} (k = jj mod Ns), writing
 // out[(jj/Ns) Ns R + k + q Ns]; the last pass lands in registers.  Passes
-// exchange through two LDS buffers (one barrier per pass).
+// exchange through two LDS buffers (one barrier per pass).  For E = 12 the
+// first pass has radix 12 (3 x 4) and the later radices are 4 or 2, each
+// dividing E (wf_radix).
 #pragma once
 
 #include <type_traits>
@@ -19,10 +21,17 @@
 
 namespace qp {
 
+constexpr bool wf_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// Lengths the wave FFT handles: 2^p in [16, 1024], 3 * 2^p in [48, 384].
+constexpr bool wf_length(int n) {
+    return (n >= 16 && n <= 1024 && wf_pow2(n)) || (n >= 48 && n <= 384 && n % 3 == 0 && wf_pow2(n / 3));
+}
+
 template <int N, int E, int TB>
 struct WfShape {
-    static_assert(N >= 16 && N <= 1024 && (N & (N - 1)) == 0, "wave FFT length must be 2^p in [16, 1024]");
-    static_assert(E == 4 || E == 8 || E == 16, "elements per lane must be 4, 8 or 16");
+    static_assert(wf_length(N), "wave FFT length must be 2^p in [16, 1024] or 3 * 2^p in [48, 384]");
+    static_assert(N % 3 == 0 ? E == 12 : (E == 4 || E == 8 || E == 16),
+                  "elements per lane must be 4, 8 or 16 (12 for lengths 3 * 2^p)");
     static constexpr int L = N / E;                      // lanes per transform
     static_assert(TB % 64 == 0 && TB >= L && TB % L == 0, "block must hold whole transforms");
     static constexpr int T = TB / L;                     // transforms per block
@@ -62,6 +71,26 @@ constexpr double cos16(int m) {
 }
 constexpr double sin16(int m) { return cos16(m - 4); }
 
+// cos / sin of 2 pi m / 12 (sqrt(3)/2, 1/2).
+constexpr double kR3 = 0.86602540378443864676;
+constexpr double cos12(int m) {
+    switch (((m % 12) + 12) % 12) {
+        case 0: return 1.0;
+        case 1: return kR3;
+        case 2: return 0.5;
+        case 3: return 0.0;
+        case 4: return -0.5;
+        case 5: return -kR3;
+        case 6: return -1.0;
+        case 7: return -kR3;
+        case 8: return -0.5;
+        case 9: return 0.0;
+        case 10: return 0.5;
+        default: return kR3;
+    }
+}
+constexpr double sin12(int m) { return cos12(m - 3); }
+
 // (S i) * a
 template <int S>
 __device__ __forceinline__ double2 mul_si(double2 a) {
@@ -87,6 +116,37 @@ __device__ __forceinline__ double2 w16(double2 a) {
     }
 }
 
+// a * e^{S 2 pi i M / 12}
+template <int M, int S>
+__device__ __forceinline__ double2 w12(double2 a) {
+    constexpr int m = M % 12;
+    if constexpr (m == 0) {
+        return a;
+    } else if constexpr (m == 3) {
+        return mul_si<S>(a);
+    } else if constexpr (m == 6) {
+        return make_double2(-a.x, -a.y);
+    } else if constexpr (m == 9) {
+        return mul_si<-S>(a);
+    } else {
+        constexpr double c = cos12(m);
+        constexpr double s = S * sin12(m);
+        return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+// Length-3 DFT with w = e^{S 2 pi i / 3} = -1/2 + S i sqrt(3)/2:
+// v1, v2 = v0 - (v1 + v2)/2 +- (sqrt(3)/2) S i (v1 - v2).
+template <int S>
+__device__ __forceinline__ void dft3(double2 &v0, double2 &v1, double2 &v2) {
+    const double2 t1 = cadd(v1, v2);
+    const double2 t2 = make_double2(v0.x - 0.5 * t1.x, v0.y - 0.5 * t1.y);
+    const double2 d = mul_si<S>(csub(v1, v2));
+    v0 = cadd(v0, t1);
+    v1 = make_double2(t2.x + kR3 * d.x, t2.y + kR3 * d.y);
+    v2 = make_double2(t2.x - kR3 * d.x, t2.y - kR3 * d.y);
+}
+
 template <int S>
 __device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, double2 &v3) {
     const double2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
@@ -97,11 +157,39 @@ __device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, doub
     v3 = csub(t1, t3);
 }
 
-// In-register DFT of length R in {1, 2, 4, 8, 16}: u[q] <- sum_r u[r] e^{S 2 pi i r q / R}.
-// Radix 8 and 16 split as r = r1 + (R/4 or 4) r2 with internal twiddles.
+// In-register DFT of length R in {1, 2, 3, 4, 8, 12, 16}: u[q] <- sum_r u[r] e^{S 2 pi i r q / R}.
+// Radix 8 and 16 split as r = r1 + (R/4 or 4) r2 with internal twiddles; radix
+// 12 as r = r1 + 4 r2, q = q2 + 3 q1: length-3 DFTs over r2, twiddles
+// e^{S 2 pi i r1 q2 / 12}, length-4 DFTs over r1.
 template <int R, int S>
 __device__ __forceinline__ void dft(double2 (&u)[R]) {
-    if constexpr (R == 2) {
+    if constexpr (R == 3) {
+        dft3<S>(u[0], u[1], u[2]);
+    } else if constexpr (R == 12) {
+        double2 a[4][3];
+#pragma unroll
+        for (int r1 = 0; r1 < 4; ++r1) {
+            a[r1][0] = u[r1];
+            a[r1][1] = u[r1 + 4];
+            a[r1][2] = u[r1 + 8];
+            dft3<S>(a[r1][0], a[r1][1], a[r1][2]);
+        }
+        a[1][1] = w12<1, S>(a[1][1]);
+        a[1][2] = w12<2, S>(a[1][2]);
+        a[2][1] = w12<2, S>(a[2][1]);
+        a[2][2] = w12<4, S>(a[2][2]);
+        a[3][1] = w12<3, S>(a[3][1]);
+        a[3][2] = w12<6, S>(a[3][2]);
+#pragma unroll
+        for (int q2 = 0; q2 < 3; ++q2) {
+            double2 c0 = a[0][q2], c1 = a[1][q2], c2 = a[2][q2], c3 = a[3][q2];
+            dft4<S>(c0, c1, c2, c3);
+            u[q2] = c0;
+            u[q2 + 3] = c1;
+            u[q2 + 6] = c2;
+            u[q2 + 9] = c3;
+        }
+    } else if constexpr (R == 2) {
         const double2 a = u[0], b = u[1];
         u[0] = cadd(a, b);
         u[1] = csub(a, b);
@@ -179,18 +267,29 @@ __device__ __forceinline__ void wf_sync() {
     }
 }
 
+// Radix of the pass after sub-transforms of size NS: for 2^p lengths the
+// remaining length capped at E; for E = 12 the remaining length when it divides
+// E (the last pass), else 4 or 2 (every radix must divide E).
+constexpr int wf_radix(int N, int E, int NS) {
+    const int rem = N / NS;
+    if (E % 3 != 0) return rem < E ? rem : E;
+    if (rem <= E && E % rem == 0) return rem;
+    return rem % 4 == 0 ? 4 : 2;
+}
+
 // Passes after the first: bin holds sub-transforms of size NS.
 template <int N, int E, int S, int NS>
 __device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2 *bout, const double2 *twl, int j) {
     constexpr int L = N / E;
     constexpr int REM = N / NS;
-    constexpr int R = REM < E ? REM : E;
+    constexpr int R = wf_radix(N, E, NS);
     constexpr bool LAST = (R == REM);
     constexpr int NB = E / R;
+    static_assert(E % R == 0 && REM % R == 0, "pass radix must divide E and the remaining length");
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int jj = j + L * b;
-        const int k = jj & (NS - 1);
+        const int k = jj % NS;
         double2 u[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) u[r] = bin[wf_pad(jj + r * (N / R))];
@@ -232,21 +331,25 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[E], double2 *buf0, double2
 }
 
 // Run f(std::integral_constant<int, N>) for the wave-FFT length n; returns
-// false when n is not 2^p in [16, 1024].
+// false when n is not a wave-FFT length (wf_length).
 template <class F>
 inline bool wf_dispatch(int n, F &&f) {
     switch (n) {
         case 16: f(std::integral_constant<int, 16>{}); return true;
         case 32: f(std::integral_constant<int, 32>{}); return true;
+        case 48: f(std::integral_constant<int, 48>{}); return true;
         case 64: f(std::integral_constant<int, 64>{}); return true;
+        case 96: f(std::integral_constant<int, 96>{}); return true;
         case 128: f(std::integral_constant<int, 128>{}); return true;
+        case 192: f(std::integral_constant<int, 192>{}); return true;
         case 256: f(std::integral_constant<int, 256>{}); return true;
+        case 384: f(std::integral_constant<int, 384>{}); return true;
         case 512: f(std::integral_constant<int, 512>{}); return true;
         case 1024: f(std::integral_constant<int, 1024>{}); return true;
         default: return false;
     }
 }
 
-inline bool wf_supported(int n) { return n >= 16 && n <= 1024 && (n & (n - 1)) == 0; }
+inline bool wf_supported(int n) { return wf_length(n); }
 
 }  // namespace qp

@@ -21,7 +21,7 @@ namespace qp {
 // 1024^2 (128, 256, 256); smaller grids use one wavefront per block.
 template <int N>
 struct WfCfg {
-    static constexpr int E = 4;
+    static constexpr int E = N % 3 == 0 ? 12 : 4;  // 3 * 2^p lengths: radix-12 first pass
     static constexpr int TB = N >= 512 ? 256 : (N == 256 ? 128 : 64);
 };
 
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
         const int k = kk + W::L * i;
         if (k > N2 / 2) break;
         const double2 Z = z[wf_pad(k)];
-        const double2 Zm = z[wf_pad((N2 - k) & (N2 - 1))];
+        const double2 Zm = z[wf_pad(k == 0 ? 0 : N2 - k)];
         const double cr = Zm.x, ci = -Zm.y;
         spec[(size_t)k * n1 + qa] = make_double2(0.5 * (Z.x + cr), 0.5 * (Z.y + ci));
         if (qb < n1) spec[(size_t)k * n1 + qb] = make_double2(0.5 * (Z.y - ci), -0.5 * (Z.x - cr));

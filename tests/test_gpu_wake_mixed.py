@@ -144,3 +144,15 @@ def test_mixed_stopped_pass_matches_oracle(qp_ctx, oracle):
     done = slice(ss0 + 1, None)   # slices computed before the stop
     for f in ("psi", "ex"):
         assert rel_err(got[f][done], want[f][done])[0] < TOL_PSI_EX, f
+
+
+def test_mixed_three_two_grid_matches_oracle(qp_ctx, oracle):
+    """A 48 x 48 grid (3 * 2^4) runs the register-resident FFT kernels with a
+    radix-12 first pass."""
+    g, y, z, w, a = case(oracle, n=48, K=5)
+    want, st0, _ = oracle.wake_run(g, y, z, w, a)
+    wk, got, st1, _ = run_mixed(qp_ctx, g, y, z, w, a)
+    assert json.loads(wk.describe())["fft"] == "wave"
+    wk.close()
+    assert st0 == st1 == 0
+    check(got, want)
