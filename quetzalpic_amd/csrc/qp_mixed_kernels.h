@@ -414,7 +414,7 @@ __device__ __forceinline__ void cic_w(double v, double yf, double zf, double *a)
 }
 
 // LDS-typed pointer: atomics through it are ds_add_f64.  With a generic
-// pointer the compiler may merge the LDS and global branches of dep4 into one
+// pointer the compiler may merge the LDS and global branches of dep4w into one
 // flat atomic, which is much slower on LDS.
 typedef __attribute__((address_space(3))) double lds_f64;
 
@@ -469,14 +469,6 @@ __device__ __forceinline__ void dep4w_multi(double *lds, const Win &w, double *c
         unsafeAtomicAdd(at(glob[g], t12), a[4 * g + 2]);
         unsafeAtomicAdd(at(glob[g], t22), a[4 * g + 3]);
     }
-}
-
-// Fold-free deposit of value v (CIC weights computed here).
-__device__ __forceinline__ void dep4(double *lds, const Win &w, double *glob, int n1, int n2, int j1, int j2, int k1,
-                                     int k2, double yf, double zf, double v) {
-    double a[4];
-    cic_w(v, yf, zf, a);
-    dep4w(lds, w, glob, n1, n2, j1, j2, k1, k2, a[0], a[1], a[2], a[3]);
 }
 
 __device__ inline void win_zero(double *lds, int ngrid, const Win &w) {
