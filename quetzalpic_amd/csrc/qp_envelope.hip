@@ -154,11 +154,11 @@ __global__ __launch_bounds__(TB) void kw_cols_env(double2 *spec0, double2 *spec1
                                                   const double *lam1, const double *lam2, double ar, double ai,
                                                   double hdt, const double *cmid) {
     using W = WfShape<N1, E, TB>;
-    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N1];
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N1, E, TB>::NL];
     const int tid = threadIdx.x;
-    for (int t = tid; t < N1; t += TB) twl[t] = tw1[t];
-    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
+    WfTw<N1, E, TB> tw;
+    tw.init(tw1, twl, j);
     const int l = blockIdx.x * W::T + tt;
     const bool valid = l < nh;
     double2 x[E], y[E];
@@ -172,12 +172,12 @@ __global__ __launch_bounds__(TB) void kw_cols_env(double2 *spec0, double2 *spec1
     }
     const double l2 = (valid ? lam2[l] : 0.0) - *cmid;
     double2 *b0 = sb0 + tt * W::PADN, *b1 = sb1 + tt * W::PADN;
-    wave_fft<N1, E, -1>(x, b0, b1, twl, j);
-    wave_fft<N1, E, -1>(y, b0, b1, twl, j);
+    wave_fft<N1, E, -1>(x, b0, b1, tw, j);
+    wave_fft<N1, E, -1>(y, b0, b1, tw, j);
 #pragma unroll
     for (int m = 0; m < E; ++m) env_mix(x[m], y[m], lm[m] + l2, ar, ai, hdt);
-    wave_fft<N1, E, +1>(x, b0, b1, twl, j);
-    wave_fft<N1, E, +1>(y, b0, b1, twl, j);
+    wave_fft<N1, E, +1>(x, b0, b1, tw, j);
+    wave_fft<N1, E, +1>(y, b0, b1, tw, j);
     if (!valid) return;
     double2 *o0 = spec0 + (size_t)l * N1, *o1 = spec1 + (size_t)l * N1;
 #pragma unroll

@@ -12,7 +12,10 @@
 // out[(jj/Ns) Ns R + k + q Ns]; the last pass lands in registers.  Passes
 // exchange through two LDS buffers (one barrier per pass).  For E = 12 the
 // first pass has radix 12 (3 x 4) and the later radices are 4 or 2, each
-// dividing E (wf_radix).
+// dividing E (wf_radix).  Twiddles come from the axis table e^{-2 pi i t/N},
+// t < N, in device memory: with E = 4 each lane loads the ones it applies into
+// registers at kernel start, with E = 12 the block stages the table in LDS
+// (WfTw).
 #pragma once
 
 #include <type_traits>
@@ -244,13 +247,6 @@ __device__ __forceinline__ void dft(double2 (&u)[R]) {
     }
 }
 
-// Twiddle e^{S 2 pi i t / N} from the table e^{-2 pi i t / N}.
-template <int S>
-__device__ __forceinline__ double2 wf_tw(const double2 *twl, int t) {
-    double2 w = twl[t];
-    if constexpr (S > 0) w.y = -w.y;
-    return w;
-}
 
 // Orders the LDS exchanges of one transform.  A transform of L <= 64 lanes
 // lies in one wavefront (L divides 64), whose LDS instructions execute in
@@ -277,9 +273,69 @@ constexpr int wf_radix(int N, int E, int NS) {
     return rem % 4 == 0 ? 4 : 2;
 }
 
+// Twiddles one lane applies in the passes after the first (sub-transforms of
+// size NS upward).
+constexpr int wf_ntw(int N, int E, int NS) {
+    return NS >= N ? 0 : (E / wf_radix(N, E, NS)) * (wf_radix(N, E, NS) - 1) + wf_ntw(N, E, NS * wf_radix(N, E, NS));
+}
+
+// Loads, in pass order, the twiddles e^{-2 pi i t/N} lane j applies: slot
+// OFF + b (R - 1) + r - 1 of a pass holds the one of its butterfly b, input r.
+template <int N, int E, int NS, int OFF, int NT>
+__device__ __forceinline__ void wf_tw_load(double2 (&t)[NT], const double2 *__restrict__ g, int j) {
+    if constexpr (NS < N) {
+        constexpr int L = N / E;
+        constexpr int R = wf_radix(N, E, NS);
+        constexpr int NB = E / R;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int k = (j + L * b) % NS;
+#pragma unroll
+            for (int r = 1; r < R; ++r) t[OFF + b * (R - 1) + r - 1] = g[k * r * (N / (NS * R))];
+        }
+        wf_tw_load<N, E, NS * R, OFF + NB * (R - 1)>(t, g, j);
+    }
+}
+
+// Twiddle source of wave_fft.  With 4 elements per lane (at most 3 twiddles
+// per pass) every lane loads the twiddles it applies into registers at kernel
+// start, next to its data loads: no LDS table and no barrier to publish it.
+// With 12 (up to 9 per pass) the block stages the N-entry table in LDS.
+template <int N, int E, int TB>
+struct WfTw {
+    static constexpr bool kRegs = E == 4;
+    static constexpr int NT = kRegs ? wf_ntw(N, E, E) : 1;
+    static constexpr int NL = kRegs ? 1 : N;  // entries of the LDS table
+    double2 reg[NT > 0 ? NT : 1];
+    const double2 *lds;
+    // g: the table e^{-2 pi i t/N} in global memory; table: NL entries of LDS.
+    // Every thread of the block calls it.
+    __device__ __forceinline__ void init(const double2 *__restrict__ g, double2 *table, int j) {
+        if constexpr (kRegs) {
+            wf_tw_load<N, E, E, 0>(reg, g, j);
+        } else {
+            for (int t = threadIdx.x; t < N; t += TB) table[t] = g[t];
+            __syncthreads();  // twiddle table in LDS for every transform of the block
+            lds = table;
+        }
+    }
+    // e^{S 2 pi i t/N}, from register slot s or table entry t.
+    template <int S>
+    __device__ __forceinline__ double2 get(int s, int t) const {
+        double2 w;
+        if constexpr (kRegs) {
+            w = reg[s];
+        } else {
+            w = lds[t];
+        }
+        if constexpr (S > 0) w.y = -w.y;
+        return w;
+    }
+};
+
 // Passes after the first: bin holds sub-transforms of size NS.
-template <int N, int E, int S, int NS>
-__device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2 *bout, const double2 *twl, int j) {
+template <int N, int E, int S, int NS, int OFF, class TW>
+__device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2 *bout, const TW &tw, int j) {
     constexpr int L = N / E;
     constexpr int REM = N / NS;
     constexpr int R = wf_radix(N, E, NS);
@@ -295,7 +351,8 @@ __device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2
         for (int r = 0; r < R; ++r) u[r] = bin[wf_pad(jj + r * (N / R))];
         if (k != 0) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) u[r] = cmul(u[r], wf_tw<S>(twl, k * r * (N / (NS * R))));
+            for (int r = 1; r < R; ++r)
+                u[r] = cmul(u[r], tw.template get<S>(OFF + b * (R - 1) + r - 1, k * r * (N / (NS * R))));
         }
         dft<R, S>(u);
         if constexpr (LAST) {
@@ -309,16 +366,15 @@ __device__ __forceinline__ void wf_passes(double2 (&v)[E], double2 *bin, double2
     }
     if constexpr (!LAST) {
         wf_sync<L>();
-        wf_passes<N, E, S, NS * R>(v, bout, bin, twl, j);
+        wf_passes<N, E, S, NS * R, OFF + NB * (R - 1)>(v, bout, bin, tw, j);
     }
 }
 
 // Transform the E registers of every lane (see the header comment).  buf0 /
-// buf1: this transform's LDS regions (PADN each); twl: the N-entry table
-// e^{-2 pi i t/N} in LDS, written and published by a block barrier before the
-// call.  Every thread of the block must call it.
-template <int N, int E, int S>
-__device__ __forceinline__ void wave_fft(double2 (&v)[E], double2 *buf0, double2 *buf1, const double2 *twl, int j) {
+// buf1: this transform's LDS regions (PADN each); tw: the WfTw<N, E, TB> the
+// block initialised.  Every thread of the block must call it.
+template <int N, int E, int S, class TW>
+__device__ __forceinline__ void wave_fft(double2 (&v)[E], double2 *buf0, double2 *buf1, const TW &tw, int j) {
     constexpr int L = N / E;
     dft<E, S>(v);
     if constexpr (N > E) {
@@ -326,7 +382,7 @@ __device__ __forceinline__ void wave_fft(double2 (&v)[E], double2 *buf0, double2
 #pragma unroll
         for (int q = 0; q < E; ++q) buf0[wf_pad(E * j + q)] = v[q];
         wf_sync<L>();
-        wf_passes<N, E, S, E>(v, buf0, buf1, twl, j);
+        wf_passes<N, E, S, E, 0>(v, buf0, buf1, tw, j);
     }
 }
 

@@ -61,12 +61,12 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
     using W = WfShape<N2, E, TB>;
     const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
     if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
-    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N2];
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N2, E, TB>::NL];
     double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     const int tid = threadIdx.x;
-    for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
-    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
+    WfTw<N2, E, TB> tw;
+    tw.init(tw2, twl, j);
     const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
     double2 v[E];
 #pragma unroll
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
         if (rb < n1) xb = src(wk, which, rb, k);
         v[r] = make_double2(xa, xb);
     }
-    wave_fft<N2, E, -1>(v, sb0 + tt * W::PADN, sb1 + tt * W::PADN, twl, j);
+    wave_fft<N2, E, -1>(v, sb0 + tt * W::PADN, sb1 + tt * W::PADN, tw, j);
     // Z in natural order to LDS (after the barrier no lane reads sb0 any more)
     double2 *zb = sb0;
     __syncthreads();
@@ -119,13 +119,13 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
     using W = WfShape<N1, E, TB>;
     const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
     if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
-    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N1];
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N1, E, TB>::NL];
     double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     if (bpart) bpart += wk * bstride;
     const int tid = threadIdx.x;
-    for (int t = tid; t < N1; t += TB) twl[t] = tw1[t];
-    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
+    WfTw<N1, E, TB> tw;
+    tw.init(tw1, twl, j);
     const int l = blockIdx.x * W::T + tt;
     const bool valid = l < nh;
     double2 v[E];
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
     }
     const double l2 = valid ? lam2[l] : 0.0;
     double2 *b0 = sb0 + tt * W::PADN, *b1 = sb1 + tt * W::PADN;
-    wave_fft<N1, E, -1>(v, b0, b1, twl, j);
+    wave_fft<N1, E, -1>(v, b0, b1, tw, j);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         if (!valid || (l == 0 && j == 0 && m == 0)) {
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
             if (blockIdx.x == 0) bpart[2 * gridDim.x] = g00;
         }
     }
-    wave_fft<N1, E, +1>(v, b0, b1, twl, j);
+    wave_fft<N1, E, +1>(v, b0, b1, tw, j);
     if (!valid) return;
     double2 *out = spec + (size_t)l * N1;
 #pragma unroll
@@ -181,13 +181,13 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
     using W = WfShape<N2, E, TB>;
     const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
     if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
-    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[N2];
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N2, E, TB>::NL];
     const double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     sink.begin(wk);
     const int tid = threadIdx.x;
-    for (int t = tid; t < N2; t += TB) twl[t] = tw2[t];
-    __syncthreads();  // twiddle table in LDS for every transform of the block
     const int tt = tid / W::L, j = tid % W::L;
+    WfTw<N2, E, TB> tw;
+    tw.init(tw2, twl, j);
     const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
     double2 v[E];
 #pragma unroll
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
         }
         v[r] = make_double2(X.x - Y.y, X.y + Y.x);
     }
-    wave_fft<N2, E, +1>(v, sb0 + tt * W::PADN, sb1 + tt * W::PADN, twl, j);
+    wave_fft<N2, E, +1>(v, sb0 + tt * W::PADN, sb1 + tt * W::PADN, tw, j);
     if (ra >= n1) return;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
