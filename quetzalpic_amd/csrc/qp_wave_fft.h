@@ -14,8 +14,8 @@
 // first pass has radix 12 (3 x 4) and the later radices are 4 or 2, each
 // dividing E (wf_radix).  Twiddles come from the axis table e^{-2 pi i t/N},
 // t < N, in device memory: with E = 4 each lane loads the ones it applies into
-// registers at kernel start, with E = 12 the block stages the table in LDS
-// (WfTw).
+// registers at kernel start, and so with E = 12 up to N = 192; at N = 384
+// the block stages the table in LDS (WfTw).
 #pragma once
 
 #include <type_traits>
@@ -297,13 +297,14 @@ __device__ __forceinline__ void wf_tw_load(double2 (&t)[NT], const double2 *__re
     }
 }
 
-// Twiddle source of wave_fft.  With 4 elements per lane (at most 3 twiddles
-// per pass) every lane loads the twiddles it applies into registers at kernel
-// start, next to its data loads: no LDS table and no barrier to publish it.
-// With 12 (up to 9 per pass) the block stages the N-entry table in LDS.
+// Twiddle source of wave_fft.  Every lane loads the twiddles it applies into
+// registers at kernel start, next to its data loads: no LDS table and no
+// barrier to publish it.  That holds 3 per pass with 4 elements per lane, and
+// up to 18 in all for lengths 3 * 2^p <= 192 (12 per lane); for 384 (24) the
+// block stages the N-entry table in LDS instead.
 template <int N, int E, int TB>
 struct WfTw {
-    static constexpr bool kRegs = E == 4;
+    static constexpr bool kRegs = E == 4 || N <= 192;
     static constexpr int NT = kRegs ? wf_ntw(N, E, E) : 1;
     static constexpr int NL = kRegs ? 1 : N;  // entries of the LDS table
     double2 reg[NT > 0 ? NT : 1];
