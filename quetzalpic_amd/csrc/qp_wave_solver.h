@@ -144,9 +144,9 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
         if (!valid || (l == 0 && j == 0 && m == 0)) {
             v[m] = make_double2(0.0, 0.0);
         } else {
-            const double mult = -D + lm[m] + l2;
-            v[m].x /= mult;
-            v[m].y /= mult;
+            const double inv = 1.0 / (-D + lm[m] + l2);  // one division per complex value
+            v[m].x *= inv;
+            v[m].y *= inv;
         }
     }
     if (bpart) {
