@@ -40,8 +40,8 @@ struct WfShape {
     static constexpr int T = TB / L;                     // transforms per block
     static constexpr int PADN = N + N / 16;              // padded LDS length of one transform
     static constexpr int LDS = T * PADN;
-    // two exchange buffers + the twiddle table within 64 KiB of LDS
-    static_assert((2 * LDS + N) * 16 <= 65536, "transforms per block exceed the LDS budget");
+    // two exchange buffers + the twiddle table within the 160 KiB of LDS a gfx950 workgroup may use
+    static_assert((2 * LDS + N) * 16 <= 163840, "transforms per block exceed the LDS budget");
 };
 
 // One pad slot every 16 elements keeps 16-byte LDS accesses of consecutive

@@ -18,11 +18,15 @@ namespace qp {
 // flight beat fewer exchanges: at 256^2 the psi solve in a dependent chain
 // measured 17.7 us with E = 4 against 23.1 us with E = 8 and 34.2 us with
 // E = 16) and the threads per block that measured best at 256^2, 512^2 and
-// 1024^2 (128, 256, 256); smaller grids use one wavefront per block.
+// 1024^2 (128, 256, 512); smaller grids use one wavefront per block.  At 1024
+// two transforms per block (69.6 KB of LDS, within gfx950's 160 KB per
+// workgroup) make the row passes' column-major spectrum accesses 64 B
+// contiguous instead of 32 B (one C5 beam step 5.86 -> 5.74 s); the same at
+// 512 (four transforms) was slower (C3 665 -> 733 ms).
 template <int N>
 struct WfCfg {
     static constexpr int E = N % 3 == 0 ? 12 : 4;  // 3 * 2^p lengths: radix-12 first pass
-    static constexpr int TB = N >= 512 ? 256 : (N == 256 ? 128 : 64);
+    static constexpr int TB = N == 1024 ? 512 : (N == 512 ? 256 : (N == 256 ? 128 : 64));
 };
 
 // Sum of three values over a block of TB threads (every thread gets the sums).
