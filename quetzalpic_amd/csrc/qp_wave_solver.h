@@ -178,7 +178,7 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
 // Rows inverse (c2r): Z[k] = X[k] + i Y[k] with X[-k] = conj X[k]; the
 // imaginary parts of the k = 0 and k = n2/2 terms are ignored (FFTW c2r
 // semantics).  Re z -> row 2p, Im z -> row 2p+1, handed to the sink.
-template <int N2, int E, int TB, class Sink>
+template <int N2, int E, int TB, class Sink, bool STAGE>
 __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const double2 *spec1, Sink sink, int n1,
                                                   const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
                                                   size_t sstride) {
@@ -193,6 +193,28 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
     WfTw<N2, E, TB> tw;
     tw.init(tw2, twl, j);
     const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
+    constexpr int NHF = N2 / 2 + 1;  // frequencies of the half spectrum
+    // STAGE (lengths >= 32): the block first copies its 2T rows of the
+    // column-major half spectrum to LDS, neighbouring lanes reading
+    // neighbouring rows of one frequency (one contiguous run per frequency
+    // instead of one 16-byte piece per lane).  Row pair t lands in sb1 + t PADN
+    // (row ra at [0, NHF), rb at [NHF, 2 NHF)); its lanes read it into
+    // registers before wave_fft's first write to sb1.  Rows beyond n1 are
+    // staged as zeros.  The extra barrier and LDS round trip pay off only in
+    // large launches (the launcher decides).
+    constexpr bool staged = STAGE && 2 * NHF <= W::PADN;
+    if constexpr (staged) {
+        constexpr int R2 = 2 * W::T;
+        const int row0 = 2 * blockIdx.x * W::T;
+        for (int e = tid; e < R2 * NHF; e += TB) {
+            const int r2 = e % R2, f = e / R2;
+            const int row = row0 + r2;
+            sb1[(r2 >> 1) * W::PADN + (r2 & 1) * NHF + f] =
+                row < n1 ? spec[(size_t)f * n1 + row] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+    }
+    const double2 *st = sb1 + tt * W::PADN;
     double2 v[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) {
@@ -200,8 +222,13 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
         const bool cj = k > N2 / 2;
         const int kk = cj ? N2 - k : k;
         double2 X = make_double2(0.0, 0.0), Y = X;
-        if (ra < n1) X = spec[(size_t)kk * n1 + ra];
-        if (rb < n1) Y = spec[(size_t)kk * n1 + rb];
+        if constexpr (staged) {
+            X = st[kk];
+            Y = st[NHF + kk];
+        } else {
+            if (ra < n1) X = spec[(size_t)kk * n1 + ra];
+            if (rb < n1) Y = spec[(size_t)kk * n1 + rb];
+        }
         if (cj) {
             X.y = -X.y;
             Y.y = -Y.y;
@@ -273,8 +300,16 @@ int wlaunch_rows_inv(const Solver2d &S, const double2 *spec0, const double2 *spe
     wf_dispatch(S.n2, [&](auto NC) {
         constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
         const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec * nw);
-        hipLaunchKernelGGL((kw_rows_inv<N, E, TB, Sink>), grid, dim3(TB), 0, s, spec0, spec1, sink, S.n1,
-                           (const double2 *)S.ax2.dev.tw, ctl, skip_mode, nspec, sstride);
+        // Staged spectrum reads for large launches: one C5 beam step (1024^2)
+        // 5.75 -> 5.68 s, a batch of ten 256^2 wakes 648.5 -> 640.6 ms per pass.
+        // One 256^2 wake (128 or 256 blocks, latency bound) was slower staged,
+        // 268.7 -> 282.2 ms per pass, so small launches read directly.
+        if (N >= 1024 || grid.x * grid.y >= 1024)
+            hipLaunchKernelGGL((kw_rows_inv<N, E, TB, Sink, true>), grid, dim3(TB), 0, s, spec0, spec1, sink, S.n1,
+                               (const double2 *)S.ax2.dev.tw, ctl, skip_mode, nspec, sstride);
+        else
+            hipLaunchKernelGGL((kw_rows_inv<N, E, TB, Sink, false>), grid, dim3(TB), 0, s, spec0, spec1, sink, S.n1,
+                               (const double2 *)S.ax2.dev.tw, ctl, skip_mode, nspec, sstride);
     });
     QP_HIP_CHECK(hipGetLastError());
     return QP_OK;
