@@ -302,7 +302,7 @@ int wlaunch_rows_inv(const Solver2d &S, const double2 *spec0, const double2 *spe
         const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec * nw);
         // Staged spectrum reads for large launches: one C5 beam step (1024^2)
         // 5.75 -> 5.68 s, a batch of ten 256^2 wakes 648.5 -> 640.6 ms per pass.
-        // One 256^2 wake (128 or 256 blocks, latency bound) was slower staged,
+        // One 256^2 wake (64 or 128 blocks, latency bound) was slower staged,
         // 268.7 -> 282.2 ms per pass, so small launches read directly.
         if (N >= 1024 || grid.x * grid.y >= 1024)
             hipLaunchKernelGGL((kw_rows_inv<N, E, TB, Sink, true>), grid, dim3(TB), 0, s, spec0, spec1, sink, S.n1,

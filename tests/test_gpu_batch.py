@@ -109,3 +109,34 @@ def test_batch_members_stop_independently(qp_ctx, oracle):
         assert ss1 == stops[k]
         for f in FIELDS:
             assert rel_err(outs[k][f], single[f])[0] < 1e-9, (k, f)
+
+
+def test_batch_staged_spectrum_reads(qp_ctx, oracle):
+    """Eight 256 x 256 members: the batch's paired B_y / B_z inverse row pass
+    launches 64 x 2 x 8 = 1024 blocks, which read the column-major spectrum
+    through LDS (staged reads); a single 256 x 256 wake launches 128 blocks,
+    which read it directly.  Both must give the same fields."""
+    fields = ("psi", "jx", "ex", "by", "bz")
+    g = oracle.grid_from_box(LX, L, L, 0.2, L / 256, L / 256, K=5)
+    assert (g.ny, g.nz) == (256, 256)
+    inps = [member_inputs(oracle, g, k % 3) for k in range(8)]
+    from quetzalpic_amd import capi
+    b = capi.WakeBatch(qp_ctx, len(inps), g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ly, g.lz, K=g.K, D=g.D,
+                       psi_threshold=g.psi_threshold, b_threshold=g.b_threshold, use_graph=1)
+    for m, (y, z, w, py, pz, laser) in enumerate(inps):
+        b.set_particles(m, y, z, w, py, pz)
+        b.set_laser_gaussian(m, *laser)
+    outs, st, stops = b.run(fields=fields)
+    b.close()
+    assert st == 0 and stops == [-1] * 8
+    for k in range(3):
+        y, z, w, py, pz, laser = inps[k]
+        wk = capi.Wake(qp_ctx, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ly, g.lz, K=g.K, D=g.D,
+                       psi_threshold=g.psi_threshold, b_threshold=g.b_threshold, precision=capi.QP_PREC_MIXED)
+        wk.set_particles(y, z, w, py, pz)
+        wk.set_laser_gaussian(*laser)
+        single, st1, _ = wk.run(fields=fields)
+        wk.close()
+        assert st1 == 0
+        for f in fields:
+            assert rel_err(outs[k][f], single[f])[0] < 1e-9, (k, f, rel_err(outs[k][f], single[f]))
