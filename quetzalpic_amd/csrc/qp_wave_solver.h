@@ -21,8 +21,9 @@ namespace qp {
 // 1024^2 (128, 256, 512); smaller grids use one wavefront per block.  At 1024
 // two transforms per block (69.6 KB of LDS, within gfx950's 160 KB per
 // workgroup) make the row passes' column-major spectrum accesses 64 B
-// contiguous instead of 32 B (one C5 beam step 5.86 -> 5.74 s); the same at
-// 512 (four transforms) was slower (C3 665 -> 733 ms).
+// contiguous instead of 32 B (a 1024^2 x 4000-slice beam-driven pass 5.86 ->
+// 5.74 s); the same at 512 (four transforms) was slower (a 512^2 x 2000-slice
+// pass 665 -> 733 ms).
 template <int N>
 struct WfCfg {
     static constexpr int E = N % 3 == 0 ? 12 : 4;  // 3 * 2^p lengths: radix-12 first pass
@@ -300,8 +301,8 @@ int wlaunch_rows_inv(const Solver2d &S, const double2 *spec0, const double2 *spe
     wf_dispatch(S.n2, [&](auto NC) {
         constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
         const dim3 grid(wrows_blocks_t(S.n1, WfShape<N, E, TB>::T), nspec * nw);
-        // Staged spectrum reads for large launches: one C5 beam step (1024^2)
-        // 5.75 -> 5.68 s, a batch of ten 256^2 wakes 648.5 -> 640.6 ms per pass.
+        // Staged spectrum reads for large launches: a 1024^2 x 4000-slice
+        // beam-driven pass 5.75 -> 5.68 s, a batch of ten 256^2 wakes 648.5 -> 640.6 ms per pass.
         // One 256^2 wake (64 or 128 blocks, latency bound) was slower staged,
         // 268.7 -> 282.2 ms per pass, so small launches read directly.
         if (N >= 1024 || grid.x * grid.y >= 1024)
