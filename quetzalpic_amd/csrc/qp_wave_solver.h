@@ -28,6 +28,10 @@ template <int N>
 struct WfCfg {
     static constexpr int E = N % 3 == 0 ? 12 : 4;  // 3 * 2^p lengths: radix-12 first pass
     static constexpr int TB = N == 1024 ? 512 : (N == 512 ? 256 : (N == 256 ? 128 : 64));
+    // Column pass of the wake solves: at 256 one transform per 64-thread block
+    // (129 blocks instead of 65; the border-mean partial sums then need only
+    // wavefront shuffles): one 256^2 wake 269.5 -> 266.6 ms per 1000-slice pass.
+    static constexpr int TBC = N == 256 ? 64 : TB;
 };
 
 // Sum of three values over a block of TB threads (every thread gets the sums).
@@ -259,7 +263,7 @@ inline int wcols_blocks(int n1, int nh) {
     int nb = 0;
     wf_dispatch(n1, [&](auto NC) {
         constexpr int N = decltype(NC)::value;
-        nb = wcols_blocks_t(nh, WfShape<N, WfCfg<N>::E, WfCfg<N>::TB>::T);
+        nb = wcols_blocks_t(nh, WfShape<N, WfCfg<N>::E, WfCfg<N>::TBC>::T);
     });
     return nb;
 }
@@ -284,7 +288,7 @@ inline int wlaunch_cols(const Solver2d &S, double D, int nspec, double *bpart, h
                         int skip_mode, int nw = 1, size_t bstride = 0) {
     const size_t sstride = (size_t)S.n1 * S.nh;
     wf_dispatch(S.n1, [&](auto NC) {
-        constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
+        constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TBC;
         const dim3 grid(wcols_blocks_t(S.nh, WfShape<N, E, TB>::T), nspec * nw);
         hipLaunchKernelGGL((kw_cols<N, E, TB>), grid, dim3(TB), 0, s, S.spec[0], S.spec[1], S.nh, S.n2,
                            (const double2 *)S.ax1.dev.tw, (const double *)S.d_lam1, (const double *)S.d_lam2, D,
