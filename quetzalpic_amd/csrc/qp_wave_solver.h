@@ -28,10 +28,12 @@ template <int N>
 struct WfCfg {
     static constexpr int E = N % 3 == 0 ? 12 : 4;  // 3 * 2^p lengths: radix-12 first pass
     static constexpr int TB = N == 1024 ? 512 : (N == 512 ? 256 : (N == 256 ? 128 : 64));
-    // Column pass of the wake solves: at 256 one transform per 64-thread block
-    // (129 blocks instead of 65; the border-mean partial sums then need only
-    // wavefront shuffles): one 256^2 wake 269.5 -> 266.6 ms per 1000-slice pass.
-    static constexpr int TBC = N == 256 ? 64 : TB;
+    // Column pass of the wake solves: one transform per block (columns are
+    // contiguous, so the row passes' reason for several does not apply; more
+    // blocks, and a smaller border-mean reduction).  A 256^2 x 1000-slice wake
+    // 269.5 -> 266.6 ms per pass, 512^2 x 2000 slices 667 -> 662 ms, a
+    // 1024^2 x 4000-slice beam-driven pass 5.69 -> 5.57 s.
+    static constexpr int TBC = N == 256 ? 64 : (N == 512 ? 128 : (N == 1024 ? 256 : TB));
 };
 
 // Sum of three values over a block of TB threads (every thread gets the sums).
