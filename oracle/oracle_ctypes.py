@@ -53,13 +53,20 @@ _libs = {}
 
 
 def load(variant="det"):
-    """variant 'det' = deterministic serial oracle, 'omp' = OpenMP -Ofast timing build."""
+    """variant 'det' = deterministic serial oracle, 'omp' = OpenMP -Ofast timing build,
+    or a key registered with load_path."""
     if variant in _libs:
         return _libs[variant]
     name = {"det": "liboracle.so", "omp": "liboracle_omp.so"}[variant]
     path = os.path.join(_BUILD, name)
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} missing: run `make -C oracle` (or __graft_entry__.build())")
+    return load_path(variant, path)
+
+
+def load_path(variant, path):
+    """Bind an oracle build at `path` under the key `variant` (the mutation tests
+    load deliberately broken copies of the oracle this way)."""
     lib = ctypes.CDLL(path)
     lib.qpo_fft.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
     lib.qpo_rfft2.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
