@@ -94,12 +94,18 @@ def load_path(variant, path):
                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(QpoProfile), _dp, _dp, _dp]
     lib.qpo_init_lattice.restype = ctypes.c_size_t
     lib.qpo_fill_a_sqr_gaussian.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 7 + [_dp]
+    lib.qpo_fill_a_sqr_gaussian_window.argtypes = ([ctypes.POINTER(QpoGrid), ctypes.c_int, ctypes.c_int] +
+                                                   [ctypes.c_double] * 7 + [_dp])
     run_args = [ctypes.POINTER(QpoGrid), ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                 ctypes.POINTER(QpoFields), ctypes.POINTER(ctypes.c_int)]
     lib.qpo_wake_run.argtypes = run_args
     lib.qpo_wake_run.restype = ctypes.c_int
     lib.qpo_wake_run_partial.argtypes = run_args + [ctypes.c_int]
     lib.qpo_wake_run_partial.restype = ctypes.c_int
+    lib.qpo_wake_run_window.argtypes = run_args + [ctypes.c_int]
+    lib.qpo_wake_run_window.restype = ctypes.c_int
+    lib.qpo_bunch_deposit_window.argtypes = ([ctypes.POINTER(QpoGrid), ctypes.c_int, ctypes.c_int, ctypes.c_size_t] +
+                                             [_dp] * 8)
     lib.qpo_deposit3d.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 4 + [_dp]
     lib.qpo_gather3d.argtypes = [ctypes.POINTER(QpoGrid)] + [ctypes.c_double] * 6 + [_dp]
     lib.qpo_gather3d.restype = ctypes.c_double
@@ -196,6 +202,36 @@ def wake_run(g, y, z, w, a_sqr, py=None, pz=None, rho_b=None, jx_b=None, n_slice
     return out, st, stop.value
 
 
+def wake_run_window(g, y, z, w, a_sqr, n_slices, py=None, pz=None, rho_b=None, jx_b=None,
+                    fields=FIELD_NAMES, variant="det"):
+    """The first n_slices slices of the loop with windowed 3D arrays: a_sqr,
+    rho_b, jx_b are [n_slices, ny, nz] (slice i at i - (nx - n_slices)); the
+    outputs are returned in the same layout."""
+    lib = load(variant)
+    shape = (n_slices, g.ny, g.nz)
+    assert a_sqr.shape == shape
+    for b in (rho_b, jx_b):
+        assert b is None or b.shape == shape
+    out = {k: np.zeros(shape) for k in fields}
+    f = QpoFields(*[_ptr(out[k]) if k in out else None for k in FIELD_NAMES])
+    stop = ctypes.c_int(-1)
+    c = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (y, z, py, pz, w)]
+    a_sqr = np.ascontiguousarray(a_sqr, dtype=np.float64)
+    rho_b = None if rho_b is None else np.ascontiguousarray(rho_b, dtype=np.float64)
+    jx_b = None if jx_b is None else np.ascontiguousarray(jx_b, dtype=np.float64)
+    st = lib.qpo_wake_run_window(ctypes.byref(g), len(c[0]), *[_ptr(v) for v in c], _ptr(a_sqr), _ptr(rho_b),
+                                 _ptr(jx_b), ctypes.byref(f), ctypes.byref(stop), int(n_slices))
+    return out, st, stop.value
+
+
+def gaussian_a_sqr_window(g, n_slices, a0, xsigma, ysigma, zsigma, x0, y0, z0):
+    """The top n_slices slices of gaussian_a_sqr (i from nx - n_slices to nx - 1)."""
+    a = np.zeros((n_slices, g.ny, g.nz))
+    load().qpo_fill_a_sqr_gaussian_window(ctypes.byref(g), g.nx - n_slices, n_slices, a0, xsigma, ysigma, zsigma,
+                                          x0, y0, z0, _ptr(a))
+    return a
+
+
 # ------------------------------------------------------------ driver bunch
 BUNCH_STATE = ("x", "y", "z", "px", "py", "pz", "gamma")
 BUNCH_FIELDS = ("ex", "ey", "ez", "by", "bz", "chi")
@@ -226,6 +262,16 @@ def bunch_deposit(g, b):
     jx = np.zeros((g.nx, g.ny, g.nz))
     a = [_f64(b[k]) for k in ("x", "y", "z", "px", "gamma", "q")]
     load().qpo_bunch_deposit(ctypes.byref(g), len(a[0]), *[_ptr(v) for v in a], _ptr(rho), _ptr(jx))
+    return rho, jx
+
+
+def bunch_deposit_window(g, b, base, nheld):
+    """bunch_deposit into arrays holding slices base .. base + nheld - 1 only."""
+    rho = np.zeros((nheld, g.ny, g.nz))
+    jx = np.zeros((nheld, g.ny, g.nz))
+    a = [_f64(b[k]) for k in ("x", "y", "z", "px", "gamma", "q")]
+    load().qpo_bunch_deposit_window(ctypes.byref(g), int(base), int(nheld), len(a[0]), *[_ptr(v) for v in a],
+                                    _ptr(rho), _ptr(jx))
     return rho, jx
 
 

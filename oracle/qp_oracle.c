@@ -501,21 +501,27 @@ size_t qpo_init_lattice(int ny, int nz, double dy, double dz, int ppcy, int ppcz
     return idx;
 }
 
-void qpo_fill_a_sqr_gaussian(const qpo_grid *g, double a0, double xsigma, double ysigma,
-                             double zsigma, double x0, double y0, double z0, double *a_sqr) {
+void qpo_fill_a_sqr_gaussian_window(const qpo_grid *g, int i0, int nheld, double a0, double xsigma,
+                                    double ysigma, double zsigma, double x0, double y0, double z0,
+                                    double *a_sqr) {
     /* Config_reader::init_laser divides the full widths by sqrt(8) and uses
      * amplitude 0.5*a0^2 (src/Config_reader.cpp:124-128). */
     const double s8 = sqrt(8.0);
     const double wx = xsigma / s8, wy = ysigma / s8, wz = zsigma / s8;
     const double amp = 0.5 * a0 * a0;
-    for (int i = 0; i < g->nx; ++i)
+    for (int i = i0; i < i0 + nheld; ++i)
         for (int j = 0; j < g->ny; ++j)
             for (int k = 0; k < g->nz; ++k) {
                 double x = i * g->dx, y = j * g->dy, z = k * g->dz;
-                a_sqr[((size_t)i * g->ny + j) * g->nz + k] =
+                a_sqr[((size_t)(i - i0) * g->ny + j) * g->nz + k] =
                     amp * exp(-(x - x0) * (x - x0) / wx / wx - (y - y0) * (y - y0) / wy / wy -
                               (z - z0) * (z - z0) / wz / wz);
             }
+}
+
+void qpo_fill_a_sqr_gaussian(const qpo_grid *g, double a0, double xsigma, double ysigma,
+                             double zsigma, double x0, double y0, double z0, double *a_sqr) {
+    qpo_fill_a_sqr_gaussian_window(g, 0, g->nx, a0, xsigma, ysigma, zsigma, x0, y0, z0, a_sqr);
 }
 
 unsigned long long qpo_splitmix64(unsigned long long seed, unsigned long long stream,
@@ -539,18 +545,19 @@ static void put_slice(double *dst3d, int i, const double *src2d, size_t nn) {
 }
 
 /* output_step, src/System_3d.cpp:629-666. */
-static void output_step(const qpo_grid *g, qpo_fields *out, int i, const double *psi,
+static void output_step(const qpo_grid *g, qpo_fields *out, int i, int base, const double *psi,
                         const double *psi_prev, const double *by, const double *bz,
                         const double *rho, const double *jx, const double *jy, const double *jz) {
     const int ny = g->ny, nz = g->nz;
     const size_t nn = (size_t)ny * nz;
+    const int io = i - base; /* slice i sits at io in the output arrays */
     if (!out) return;
     if (out->ex && i < g->nx - 1) {
-        double *ex = out->ex + (size_t)i * nn;
+        double *ex = out->ex + (size_t)io * nn;
         for (size_t t = 0; t < nn; ++t) ex[t] = (psi[t] - psi_prev[t]) / g->dx;
     }
     if (out->ey) {
-        double *ey = out->ey + (size_t)i * nn;
+        double *ey = out->ey + (size_t)io * nn;
         for (int j = 0; j < ny - 1; ++j)
             for (int k = 0; k < nz; ++k)
                 ey[(size_t)nz * j + k] = -(psi[(size_t)nz * (j + 1) + k] - psi[(size_t)nz * j + k]) / g->dy +
@@ -560,7 +567,7 @@ static void output_step(const qpo_grid *g, qpo_fields *out, int i, const double 
                                             bz[(size_t)nz * (ny - 1) + k];
     }
     if (out->ez) {
-        double *ez = out->ez + (size_t)i * nn;
+        double *ez = out->ez + (size_t)io * nn;
         for (int j = 0; j < ny; ++j) {
             for (int k = 0; k < nz - 1; ++k)
                 ez[(size_t)nz * j + k] = -(psi[(size_t)nz * j + k + 1] - psi[(size_t)nz * j + k]) / g->dz -
@@ -569,11 +576,11 @@ static void output_step(const qpo_grid *g, qpo_fields *out, int i, const double 
                                           by[(size_t)nz * j + nz - 1];
         }
     }
-    put_slice(out->psi, i, psi, nn);
-    put_slice(out->rho, i, rho, nn);
-    put_slice(out->jx, i, jx, nn);
-    put_slice(out->jy, i, jy, nn);
-    put_slice(out->jz, i, jz, nn);
+    put_slice(out->psi, io, psi, nn);
+    put_slice(out->rho, io, rho, nn);
+    put_slice(out->jx, io, jx, nn);
+    put_slice(out->jy, io, jy, nn);
+    put_slice(out->jz, io, jz, nn);
 }
 
 /* Periodic solve of fourier.in plus the caller's division by ny, nz
@@ -587,10 +594,13 @@ static void solve_and_scale(const plan2d *pl, const qpo_grid *g, double D, doubl
         for (int k = 0; k < nz; ++k) dst[(size_t)nz * j + k] = (src[(size_t)nz * j + k]) / ny / nz;
 }
 
-int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, const double *z0,
+/* The loop over slices nx-1 .. i_end (i_end = nx - n_slices, or 0 when
+ * n_slices <= 0).  windowed: the 3D inputs and outputs hold only slices
+ * i_end .. nx-1, slice i at index i - i_end; otherwise all nx slices. */
+static int wake_run_impl(const qpo_grid *g, size_t np_in, const double *y0, const double *z0,
                          const double *py0, const double *pz0, const double *w,
                          const double *a_sqr, const double *rho_b, const double *jx_b,
-                         qpo_fields *out, int *stop_slice, int n_slices) {
+                         qpo_fields *out, int *stop_slice, int n_slices, int windowed) {
     const int nx = g->nx, ny = g->ny, nz = g->nz;
     const size_t nn = (size_t)ny * nz;
     const double dx = g->dx, dy = g->dy, dz = g->dz;
@@ -621,10 +631,14 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
 
     const double half_dy = 0.5 * dy, half_dz = 0.5 * dz;
 
+    const int i_end = n_slices > 0 ? (nx - n_slices > 0 ? nx - n_slices : 0) : 0;
+    const int base = windowed ? i_end : 0;
+    const int nheld = nx - base;
+
     /* ex(n.x-1) = 0 (:265-270); psi, by, bz, rho_ion, jy, jz zeroed (:272-316). */
-    if (out && out->ex) memset(out->ex + (size_t)(nx - 1) * nn, 0, sizeof(double) * nn);
-    if (out && out->by) memset(out->by, 0, sizeof(double) * nn * nx);
-    if (out && out->bz) memset(out->bz, 0, sizeof(double) * nn * nx);
+    if (out && out->ex) memset(out->ex + (size_t)(nx - 1 - base) * nn, 0, sizeof(double) * nn);
+    if (out && out->by) memset(out->by, 0, sizeof(double) * nn * nheld);
+    if (out && out->bz) memset(out->bz, 0, sizeof(double) * nn * nheld);
 
     /* rho_ion deposition (:318-323). */
 #pragma omp parallel for
@@ -632,12 +646,11 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
 
     int stop_flag = 0;
     int stopped_at = -1;
-    const int i_end = n_slices > 0 ? (nx - n_slices > 0 ? nx - n_slices : 0) : 0;
 
     for (int i = nx - 1; i >= i_end; --i) {
-        const double *asq = a_sqr + (size_t)i * nn;
-        const double *rb = rho_b ? rho_b + (size_t)i * nn : NULL;
-        const double *jb = jx_b ? jx_b + (size_t)i * nn : NULL;
+        const double *asq = a_sqr + (size_t)(i - base) * nn;
+        const double *rb = rho_b ? rho_b + (size_t)(i - base) * nn : NULL;
+        const double *jb = jx_b ? jx_b + (size_t)(i - base) * nn : NULL;
 
         /* S1 (:334-339) */
         memcpy(psi_prev, psi, sizeof(double) * nn);
@@ -673,10 +686,10 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
             qpo_deposit(q->y, q->z, q->n / (1 - vx), rho, ny, nz, dy, dz, 0, 0);
             qpo_deposit(q->y, q->z, -q->n / (1 - vx) / q->gamma, chi, ny, nz, dy, dz, 0, 0);
         }
-        put_slice(out ? out->chi : NULL, i, chi, nn);
+        put_slice(out ? out->chi : NULL, i - base, chi, nn);
         /* S6 (:399-407): by(i) = by(i+1) is the carried 2D buffer */
-        put_slice(out ? out->by : NULL, i, by, nn);
-        put_slice(out ? out->bz : NULL, i, bz, nn);
+        put_slice(out ? out->by : NULL, i - base, by, nn);
+        put_slice(out ? out->bz : NULL, i - base, bz, nn);
 
         if (i == 0) break; /* :409-411 */
 
@@ -757,7 +770,7 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
                 qpo_deposit(q->y, q->z, q->n / (1 - vx), rho, ny, nz, dy, dz, 0, 0);
                 qpo_deposit(q->y, q->z, -q->n / (1 - vx) / q->gamma, chi, ny, nz, dy, dz, 0, 0);
             }
-            put_slice(out ? out->chi : NULL, i, chi, nn);
+            put_slice(out ? out->chi : NULL, i - base, chi, nn);
             /* P8 B_y (:529-552) */
             for (int j = 0; j < ny; ++j) {
                 for (int k = 0; k < nz - 1; ++k)
@@ -787,8 +800,8 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
                 if (fabs(bz[t]) > g->b_threshold) stop_flag = 1;
                 if (isnan(bz[t])) stop_flag = 1;
             }
-            put_slice(out ? out->by : NULL, i, by, nn);
-            put_slice(out ? out->bz : NULL, i, bz, nn);
+            put_slice(out ? out->by : NULL, i - base, by, nn);
+            put_slice(out ? out->bz : NULL, i - base, bz, nn);
             if (stop_flag) break; /* :582-584 */
         }
         if (stop_flag) { /* :588-591 */
@@ -810,7 +823,7 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
             qpo_normalize_coordinates(&q->y, &q->z, g->ly, g->lz);
         }
         /* A3 output_step(i) (:610) */
-        output_step(g, out, i, psi, psi_prev, by, bz, rho, jx, jy, jz);
+        output_step(g, out, i, base, psi, psi_prev, by, bz, rho, jx, jy, jz);
         /* A4 swap (:612-618) */
         memcpy(jy, jyn, sizeof(double) * nn);
         memcpy(jz, jzn, sizeof(double) * nn);
@@ -823,7 +836,7 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
             memset(by, 0, sizeof(double) * nn);
             memset(bz, 0, sizeof(double) * nn);
         }
-        output_step(g, out, 0, psi, psi_prev, by, bz, rho, jx, jy, jz);
+        output_step(g, out, 0, base, psi, psi_prev, by, bz, rho, jx, jy, jz);
     }
 
     plan2d_free(&pl);
@@ -833,11 +846,25 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np_in, const double *y0, cons
     return stop_flag ? 1 : 0;
 }
 
+int qpo_wake_run_partial(const qpo_grid *g, size_t np, const double *y, const double *z,
+                         const double *py, const double *pz, const double *w,
+                         const double *a_sqr, const double *rho_b, const double *jx_b,
+                         qpo_fields *out, int *stop_slice, int n_slices) {
+    return wake_run_impl(g, np, y, z, py, pz, w, a_sqr, rho_b, jx_b, out, stop_slice, n_slices, 0);
+}
+
+int qpo_wake_run_window(const qpo_grid *g, size_t np, const double *y, const double *z,
+                        const double *py, const double *pz, const double *w,
+                        const double *a_sqr, const double *rho_b, const double *jx_b,
+                        qpo_fields *out, int *stop_slice, int n_slices) {
+    return wake_run_impl(g, np, y, z, py, pz, w, a_sqr, rho_b, jx_b, out, stop_slice, n_slices, 1);
+}
+
 int qpo_wake_run(const qpo_grid *g, size_t np, const double *y, const double *z,
                  const double *py, const double *pz, const double *w,
                  const double *a_sqr, const double *rho_b, const double *jx_b,
                  qpo_fields *out, int *stop_slice) {
-    return qpo_wake_run_partial(g, np, y, z, py, pz, w, a_sqr, rho_b, jx_b, out, stop_slice, 0);
+    return wake_run_impl(g, np, y, z, py, pz, w, a_sqr, rho_b, jx_b, out, stop_slice, 0, 0);
 }
 
 
@@ -849,8 +876,10 @@ static int wrap_index(int j, int n) {
     return j < 0 ? j + n : j;
 }
 
-/* src/array_utils.cpp:290-338 */
-void qpo_deposit3d(const qpo_grid *g, double x, double y, double z, double v, double *a3) {
+/* src/array_utils.cpp:290-338, into an array holding slices base .. base+nheld-1
+ * (slice i at index i - base); contributions to other slices are dropped. */
+static void deposit3d_win(const qpo_grid *g, int base, int nheld, double x, double y, double z, double v,
+                          double *a3) {
     const int i1 = (int)floor(x / g->dx), i2 = i1 + 1;
     const double x_frac = x / g->dx - i1;
     if (i1 < 0 || i2 >= g->nx) return;
@@ -864,15 +893,24 @@ void qpo_deposit3d(const qpo_grid *g, double x, double y, double z, double v, do
     const int k2 = wrap_index(k1 + 1, n2);
     k1 = wrap_index(k1, n2);
     const size_t nn = (size_t)n1 * n2;
-    double *a = a3 + (size_t)i1 * nn, *b = a3 + (size_t)i2 * nn;
-    a[(size_t)j1 * n2 + k1] += v * (1 - x_frac) * (1 - y_frac) * (1 - z_frac);
-    a[(size_t)j2 * n2 + k1] += v * (1 - x_frac) * y_frac * (1 - z_frac);
-    a[(size_t)j1 * n2 + k2] += v * (1 - x_frac) * (1 - y_frac) * z_frac;
-    a[(size_t)j2 * n2 + k2] += v * (1 - x_frac) * y_frac * z_frac;
-    b[(size_t)j1 * n2 + k1] += v * x_frac * (1 - y_frac) * (1 - z_frac);
-    b[(size_t)j2 * n2 + k1] += v * x_frac * y_frac * (1 - z_frac);
-    b[(size_t)j1 * n2 + k2] += v * x_frac * (1 - y_frac) * z_frac;
-    b[(size_t)j2 * n2 + k2] += v * x_frac * y_frac * z_frac;
+    if (i1 >= base && i1 < base + nheld) {
+        double *a = a3 + (size_t)(i1 - base) * nn;
+        a[(size_t)j1 * n2 + k1] += v * (1 - x_frac) * (1 - y_frac) * (1 - z_frac);
+        a[(size_t)j2 * n2 + k1] += v * (1 - x_frac) * y_frac * (1 - z_frac);
+        a[(size_t)j1 * n2 + k2] += v * (1 - x_frac) * (1 - y_frac) * z_frac;
+        a[(size_t)j2 * n2 + k2] += v * (1 - x_frac) * y_frac * z_frac;
+    }
+    if (i2 >= base && i2 < base + nheld) {
+        double *b = a3 + (size_t)(i2 - base) * nn;
+        b[(size_t)j1 * n2 + k1] += v * x_frac * (1 - y_frac) * (1 - z_frac);
+        b[(size_t)j2 * n2 + k1] += v * x_frac * y_frac * (1 - z_frac);
+        b[(size_t)j1 * n2 + k2] += v * x_frac * (1 - y_frac) * z_frac;
+        b[(size_t)j2 * n2 + k2] += v * x_frac * y_frac * z_frac;
+    }
+}
+
+void qpo_deposit3d(const qpo_grid *g, double x, double y, double z, double v, double *a3) {
+    deposit3d_win(g, 0, g->nx, x, y, z, v, a3);
 }
 
 /* src/array_utils.cpp:340-374 */
@@ -903,15 +941,21 @@ double qpo_gather3d(const qpo_grid *g, double ox, double oy, double oz, double x
 }
 
 /* src/System_3d.cpp:926-946 */
-void qpo_bunch_deposit(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
-                       const double *px, const double *gamma, const double *q, double *rho_b, double *jx_b) {
-    const size_t vol = (size_t)g->nx * g->ny * g->nz;
+void qpo_bunch_deposit_window(const qpo_grid *g, int base, int nheld, size_t np, const double *x,
+                              const double *y, const double *z, const double *px, const double *gamma,
+                              const double *q, double *rho_b, double *jx_b) {
+    const size_t vol = (size_t)nheld * g->ny * g->nz;
     memset(rho_b, 0, sizeof(double) * vol);
     memset(jx_b, 0, sizeof(double) * vol);
     for (size_t p = 0; p < np; ++p) {
-        qpo_deposit3d(g, x[p], y[p], z[p], q[p], rho_b);
-        qpo_deposit3d(g, x[p], y[p], z[p], q[p] * px[p] / gamma[p], jx_b);
+        deposit3d_win(g, base, nheld, x[p], y[p], z[p], q[p], rho_b);
+        deposit3d_win(g, base, nheld, x[p], y[p], z[p], q[p] * px[p] / gamma[p], jx_b);
     }
+}
+
+void qpo_bunch_deposit(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
+                       const double *px, const double *gamma, const double *q, double *rho_b, double *jx_b) {
+    qpo_bunch_deposit_window(g, 0, g->nx, np, x, y, z, px, gamma, q, rho_b, jx_b);
 }
 
 /* src/System_3d.cpp:843-860 */

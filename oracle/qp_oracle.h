@@ -117,6 +117,10 @@ size_t qpo_init_lattice(int ny, int nz, double dy, double dz, int ppcy, int ppcz
  * a_sqr is nx*ny*nz, a_sqr(i,j,k) = f(i*dx, j*dy, k*dz). */
 void qpo_fill_a_sqr_gaussian(const qpo_grid *g, double a0, double xsigma, double ysigma,
                              double zsigma, double x0, double y0, double z0, double *a_sqr);
+/* The same for slices i0 .. i0+nheld-1 only, slice i at index i - i0. */
+void qpo_fill_a_sqr_gaussian_window(const qpo_grid *g, int i0, int nheld, double a0, double xsigma,
+                                    double ysigma, double zsigma, double x0, double y0, double z0,
+                                    double *a_sqr);
 
 /* ---- The slice loop ------------------------------------------------------ */
 /* System_3d::solve_wakefield, src/System_3d.cpp:238-627, for one pass with
@@ -134,6 +138,15 @@ int qpo_wake_run_partial(const qpo_grid *g, size_t np, const double *y, const do
                          const double *a_sqr, const double *rho_b, const double *jx_b,
                          qpo_fields *out, int *stop_slice, int n_slices);
 
+/* The same loop over the first n_slices slices (nx-1 down to nx-n_slices) with
+ * windowed 3D arrays: a_sqr, rho_b, jx_b and every output hold only those
+ * slices, slice i at index i - (nx - n_slices).  For large grids, where the
+ * full nx*ny*nz arrays would not fit in host memory. */
+int qpo_wake_run_window(const qpo_grid *g, size_t np, const double *y, const double *z,
+                        const double *py, const double *pz, const double *w,
+                        const double *a_sqr, const double *rho_b, const double *jx_b,
+                        qpo_fields *out, int *stop_slice, int n_slices);
+
 /* ---- Driver bunch (src/System_3d.cpp:843-946) ----------------------------
  * 3D arrays are nx*ny*nz row-major [i][j][k] on the nodes (origin 0).
  * Index wrap: the reference computes j2 = (j1+1) % n before wrapping a
@@ -150,6 +163,12 @@ double qpo_gather3d(const qpo_grid *g, double ox, double oy, double oz, double x
  * jx_b = sum q px/gamma S (both zeroed first); q = species charge * n. */
 void qpo_bunch_deposit(const qpo_grid *g, size_t np, const double *x, const double *y, const double *z,
                        const double *px, const double *gamma, const double *q, double *rho_b, double *jx_b);
+/* The same into arrays holding slices base .. base+nheld-1 only (slice i at
+ * index i - base); deposits to other slices are dropped, the x cut of the
+ * full grid still applies. */
+void qpo_bunch_deposit_window(const qpo_grid *g, int base, int nheld, size_t np, const double *x,
+                              const double *y, const double *z, const double *px, const double *gamma,
+                              const double *q, double *rho_b, double *jx_b);
 /* deposit_wakefield_to_bunch, src/System_3d.cpp:843-860: the five fields at
  * each particle and the QED parameter chi (field_schwinger in plasma units).
  * The field arrays carry the reference's staggered origins
