@@ -11,10 +11,40 @@ import math
 
 import numpy as np
 
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, stream, index):
+    """Counter-based SplitMix64 value for (seed, stream, index), element-wise
+    over numpy arrays (uint64 arithmetic wraps mod 2^64): the key
+    seed ^ stream*phi ^ index*c is advanced once and finalised with
+    SplitMix64's mixer (Steele, Lea, Flood 2014)."""
+    with np.errstate(over="ignore"):
+        x = (np.uint64(seed) ^ (np.asarray(stream, dtype=np.uint64) * _GOLDEN) ^
+             (np.asarray(index, dtype=np.uint64) * np.uint64(0xD1B54A32D192ED03)))
+        x = x + _GOLDEN
+        x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return x ^ (x >> np.uint64(31))
+
+
+def uniform01(seed, stream, index):
+    """Uniform in (0, 1) from the top 53 bits of splitmix64."""
+    return ((splitmix64(seed, stream, index) >> np.uint64(11)).astype(np.float64) + 0.5) * 2.0 ** -53
+
+
+def normal_pair(seed, stream, index, sigma):
+    """Two independent N(0, sigma^2) draws per index (Box-Muller on streams
+    2 stream and 2 stream + 1)."""
+    u1 = uniform01(seed, 2 * stream, index)
+    u2 = uniform01(seed, 2 * stream + 1, index)
+    r = sigma * np.sqrt(-2.0 * np.log(u1))
+    return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
+
 
 class Workload:
     def __init__(self, name, lx, dx, L, n, a0, w0, sigma, profile=("constant", -1.0), ppc=2, K=5,
-                 thermal=0.0, seed=1234, note="", laser_x0=None, bunch=None):
+                 thermal=0.0, seed=1234, note="", laser_x0=None, bunch=None, instance=0):
         self.name = name
         self.lx, self.dx0 = lx, dx
         self.L, self.n0 = L, n
@@ -24,6 +54,7 @@ class Workload:
         self.K = K
         self.thermal = thermal
         self.seed = seed
+        self.instance = instance
         self.note = note
         self.laser_x0 = laser_x0
         self.bunch = bunch
@@ -101,10 +132,8 @@ class Workload:
         y, z, v = y[keep], z[keep], v[keep]
         w = v / float(ppc) / float(ppc)
         py = pz = None
-        if self.thermal > 0:
-            rng = np.random.default_rng(self.seed)
-            py = rng.normal(0.0, self.thermal, len(y))
-            pz = rng.normal(0.0, self.thermal, len(y))
+        if self.thermal > 0:   # N(0, thermal^2) per particle, keyed (seed, instance, particle)
+            py, pz = normal_pair(self.seed, self.instance, np.arange(len(y)), self.thermal)
         return y, z, w, py, pz
 
     @property
@@ -128,17 +157,18 @@ C4_SIZES = (64, 96, 128, 192, 256)
 def c4_members(count=64, seed=2024):
     """C4: `count` independent wakes over [-12,12]^2 with grids drawn uniformly
     from C4_SIZES, 4 ppc, 800 slices dxi = 0.025, a0 ~ U[0.5, 3], w0 ~ U[2, 5],
-    sigma_xi = 2, a linear density up-ramp across the box with slope ~ U[0, 0.1],
-    one generator seeded per instance."""
+    sigma_xi = 2, a linear density up-ramp across the box with slope ~ U[0, 0.1].
+    Instance i draws grid, a0, w0, slope in that order as indices 0..3 of
+    SplitMix64 stream i under one seed."""
     out = []
     for i in range(count):
-        rng = np.random.default_rng(seed + i)
-        n = int(rng.choice(C4_SIZES))
-        a0 = float(rng.uniform(0.5, 3.0))
-        w0 = float(rng.uniform(2.0, 5.0))
-        slope = float(rng.uniform(0.0, 0.1))
+        u = uniform01(seed, i, np.arange(4))
+        n = C4_SIZES[min(int(u[0] * len(C4_SIZES)), len(C4_SIZES) - 1)]
+        a0 = float(0.5 + 2.5 * u[1])
+        w0 = float(2.0 + 3.0 * u[2])
+        slope = float(0.1 * u[3])
         out.append(Workload(f"C4 member {i}", lx=20.0, dx=0.025, L=24.0, n=n, a0=a0, w0=w0, sigma=2.0,
-                            profile=("ramp", slope), seed=seed + i))
+                            profile=("ramp", slope), seed=seed, instance=i))
     return out
 
 
