@@ -33,8 +33,9 @@ extern "C" {
 
 /* Arithmetic of the slice loop. */
 #define QP_PREC_FP64 0   /* plain path: FP64 SoA particles, one kernel per reference loop */
-#define QP_PREC_MIXED 1  /* fused path: int cell + 32-bit offsets, FP32 momenta/weights,
-                            FP64 grids, solves, 1/(1+psi) and gamma, hipGraph slice step */
+#define QP_PREC_MIXED 1  /* fused path: u32 cell + u32 fixed-point in-cell offsets, FP64
+                            momenta, FP32 weights and FP32 predictor increments p* - p;
+                            FP64 arithmetic, grids and solves; hipGraph slice step */
 
 typedef struct qp_ctx_s *qp_ctx;
 typedef struct qp_fourier2d_s *qp_fourier2d;
@@ -89,6 +90,11 @@ typedef struct {
                                 host slots and handed to the callback by a writer thread (in
                                 slice order, during the run call) while later slices compute;
                                 the slice step stays a hipGraph.  0 = the callback runs inline. */
+    int slice_limit;         /* no reference counterpart: > 0 runs only the first slice_limit
+                                slices of the pass (nx-1 down to nx-slice_limit), each exactly as
+                                in the whole pass; the final output_step(0), the bunch gather of
+                                slice 0 and an enqueued envelope step are then not run.  0 = the
+                                whole pass.  For bounded checks and timings of large grids. */
 } qp_wake_params;
 
 /* Fields of one slice handed to the per-slice callback (FP64, host memory,

@@ -34,7 +34,7 @@ class WakeParams(ctypes.Structure):
         ("K", ctypes.c_int), ("D", ctypes.c_double),
         ("psi_threshold", ctypes.c_double), ("b_threshold", ctypes.c_double),
         ("precision", ctypes.c_int), ("use_graph", ctypes.c_int), ("resort_every", ctypes.c_int),
-        ("longitudinal_b", ctypes.c_int), ("async_output", ctypes.c_int),
+        ("longitudinal_b", ctypes.c_int), ("async_output", ctypes.c_int), ("slice_limit", ctypes.c_int),
     ]
 
 
@@ -204,11 +204,11 @@ class Wake:
 
     def __init__(self, ctx, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
                  b_threshold=100.0, precision=QP_PREC_FP64, use_graph=1, resort_every=0, longitudinal_b=0,
-                 async_output=0):
+                 async_output=0, slice_limit=0):
         self.ctx = ctx
         self.shape = (nx, ny, nz)
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
-                                 precision, use_graph, resort_every, longitudinal_b, async_output)
+                                 precision, use_graph, resort_every, longitudinal_b, async_output, slice_limit)
         self.h = ctypes.c_void_p()
         self._env = None
         self._bunch = None
@@ -237,22 +237,26 @@ class Wake:
         jx_b = None if jx_b is None else np.ascontiguousarray(jx_b, dtype=np.float64)
         _check(load().qp_wake_set_bunch_sources(self.h, _ptr(rho_b), _ptr(jx_b)), "qp_wake_set_bunch_sources")
 
-    def run(self, fields=None, callback=None):
+    def run(self, fields=None, callback=None, slices=None):
         """Run one pass.  fields: iterable of names to collect into [nx,ny,nz]
         arrays with the reference's semantics (2D per-slice writes for psi..jz,
-        3D arrays for ex..chi).  Returns (dict, status, stop_slice)."""
+        3D arrays for ex..chi).  slices=(lo, hi): collect only slices lo..hi-1,
+        into [hi-lo,ny,nz] arrays (slice i at i - lo).  Returns (dict, status,
+        stop_slice)."""
         nx, ny, nz = self.shape
+        lo, hi = slices if slices is not None else (0, nx)
+        assert 0 <= lo < hi <= nx
         out = {}
         if fields:
-            out = {k: np.zeros(self.shape) for k in fields}
+            out = {k: np.zeros((hi - lo, ny, nz)) for k in fields}
         stop = ctypes.c_int(-1)
         if fields or callback:
             nn = ny * nz
 
             def _cb(vp, _user):
                 v = vp.contents
-                i = v.slice
-                for k in out:
+                i = v.slice - lo
+                for k in (out if 0 <= i < hi - lo else ()):
                     if v.kind == QP_SLICE_STOPPED and k not in ("by", "bz", "chi"):
                         continue
                     ptr = getattr(v, k)
@@ -469,7 +473,7 @@ class WakeBatch:
         self.nwakes = nwakes
         self.shape = (nx, ny, nz)
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
-                                 QP_PREC_MIXED, use_graph, 0, longitudinal_b, async_output)
+                                 QP_PREC_MIXED, use_graph, 0, longitudinal_b, async_output, 0)
         self.h = ctypes.c_void_p()
         _check(load().qp_wake_create_batch(ctx.h, ctypes.byref(self.params), nwakes, ctypes.byref(self.h)),
                "qp_wake_create_batch")
@@ -487,11 +491,14 @@ class WakeBatch:
         _check(load().qp_wake_set_laser_gaussian_at(self.h, member, a0, xsigma, ysigma, zsigma, x0, y0, z0),
                "qp_wake_set_laser_gaussian_at")
 
-    def run(self, fields=None, callback=None):
+    def run(self, fields=None, callback=None, members=None):
         """One pass of every member.  Returns (list of per-member field dicts as
-        Wake.run, status, list of stop slices)."""
+        Wake.run, status, list of stop slices); members: indices whose fields are
+        collected (default all; the others get empty dicts)."""
         nx, ny, nz = self.shape
-        outs = [{k: np.zeros(self.shape) for k in fields} if fields else {} for _ in range(self.nwakes)]
+        keep = set(range(self.nwakes)) if members is None else set(members)
+        outs = [{k: np.zeros(self.shape) for k in fields} if fields and m in keep else {}
+                for m in range(self.nwakes)]
         stops = (ctypes.c_int * self.nwakes)()
         if fields or callback:
             nn = ny * nz

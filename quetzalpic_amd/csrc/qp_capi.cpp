@@ -182,6 +182,7 @@ int qp_wake_create_batch(qp_ctx ctx, const qp_wake_params *params, int nwakes, q
               "qp_wake_create: longitudinal_b is an extension of the mixed path");
     ARG_CHECK(p.async_output == 0 || p.precision == QP_PREC_MIXED,
               "qp_wake_create: async_output is a mode of the mixed path");
+    ARG_CHECK(p.slice_limit >= 0, "qp_wake_create: slice_limit must be >= 0");
     QP_HIP_CHECK(hipSetDevice(ctx->device));
     auto *h = new (std::nothrow) qp_wake_s();
     if (!h) return QP_ERR_ALLOC;

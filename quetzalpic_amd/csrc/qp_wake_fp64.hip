@@ -388,7 +388,8 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
     if (w->bunch) QP_TRY(bunch_clear_fields(w->bunch, s));
 
     int delivered_stop = 0;
-    for (int i = p.nx - 1; i >= 0; --i) {
+    const int i_end = (p.slice_limit > 0 && p.slice_limit < p.nx) ? p.nx - p.slice_limit : 0;
+    for (int i = p.nx - 1; i >= i_end; --i) {
         const double *asq;
         QP_TRY(wake_asq_slice(w, i, &asq));
         if (w->bunch) QP_TRY(bunch_slice_sources(w->bunch, ctl, -1, i, s));
@@ -468,13 +469,15 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
         LAUNCH(k_copy, gb, w->jy, w->jyn, nn, ctl);
         LAUNCH(k_copy, gb, w->jz, w->jzn, nn, ctl);
     }
-    // output_step(0) after the loop (src/System_3d.cpp:622)
-    LAUNCH(k_output, gb, w->ex, w->ey, w->ez, w->psi, w->psi_prev, w->by, w->bz, n1, n2, p.dx, p.dy, p.dz,
-           p.nx > 1 ? 1 : 0, ctl, 1);
-    // bunch bucket 0: slice 0's fields are final only after output_step(0)
-    if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, 0, s));
-    // the envelope step after the pass (the plain path keeps the launches in order)
-    if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
+    if (i_end == 0) {
+        // output_step(0) after the loop (src/System_3d.cpp:622)
+        LAUNCH(k_output, gb, w->ex, w->ey, w->ez, w->psi, w->psi_prev, w->by, w->bz, n1, n2, p.dx, p.dy, p.dz,
+               p.nx > 1 ? 1 : 0, ctl, 1);
+        // bunch bucket 0: slice 0's fields are final only after output_step(0)
+        if (w->bunch) QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, ctl, -1, 0, s));
+        // the envelope step after the pass (the plain path keeps the launches in order)
+        if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
+    }
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     float ms = 0.f;
@@ -486,7 +489,7 @@ int wake_run_fp64(Wake *w, qp_slice_cb cb, void *user, int *stop_slice) {
         QP_TRY(deliver_slice(w, cb, user, hc.stop_slice, QP_SLICE_STOPPED, w->psi, w->rho, w->jx, w->jy, w->jz,
                              w->ex, w->ey, w->ez, w->by, w->bz, w->chi));
     }
-    if (cb) {
+    if (cb && i_end == 0) {
         const double *bz0 = hc.stop_acc ? w->zero2 : w->bz;
         const double *by0 = hc.stop_acc ? w->zero2 : w->by;
         const double *chi0 = hc.stop_acc ? w->zero2 : w->chi;

@@ -1436,7 +1436,8 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
             t.join();
         }
     } join{q, writer};
-    for (int i = p.nx - 1; i >= 0; --i) {
+    const int i_end = (p.slice_limit > 0 && p.slice_limit < p.nx) ? p.nx - p.slice_limit : 0;
+    for (int i = p.nx - 1; i >= i_end; --i) {
         const int slot = i & 1;
         if (p.resort_every > 0 && i < p.nx - 1 && ((p.nx - 1 - i) % p.resort_every) == 0 && m->P.np)
             QP_TRY(resort(R, s));
@@ -1522,18 +1523,20 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
         }
     }
     QP_HIP_CHECK(hipStreamSynchronize(s));
-    // output_step(0) after the loop (src/System_3d.cpp:622); the kernel picks each
-    // member's psi slot from its own stop slice
-    hipLaunchKernelGGL(k_mx_out, dim3(R.gb, nw), dim3(256), 0, s, w->ex, w->ey, w->ez, (const double *)w->psi_b[0],
-                       (const double *)w->psi_b[1], (const double *)w->by, (const double *)w->bz, p.ny, p.nz, p.dx,
-                       p.dy, p.dz, p.nx > 1 ? 1 : 0, w->ctl, 0, 1);
-    QP_HIP_CHECK(hipGetLastError());
-    // bunch bucket 0: slice 0's fields are final only after output_step(0)
-    if (w->bunch)
-        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, s));
-    // the envelope step after the pass (chi of slices the pass did not reach is
-    // the last one stored)
-    if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
+    if (i_end == 0) {
+        // output_step(0) after the loop (src/System_3d.cpp:622); the kernel picks each
+        // member's psi slot from its own stop slice
+        hipLaunchKernelGGL(k_mx_out, dim3(R.gb, nw), dim3(256), 0, s, w->ex, w->ey, w->ez,
+                           (const double *)w->psi_b[0], (const double *)w->psi_b[1], (const double *)w->by,
+                           (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz, p.nx > 1 ? 1 : 0, w->ctl, 0, 1);
+        QP_HIP_CHECK(hipGetLastError());
+        // bunch bucket 0: slice 0's fields are final only after output_step(0)
+        if (w->bunch)
+            QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, s));
+        // the envelope step after the pass (chi of slices the pass did not reach is
+        // the last one stored)
+        if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
+    }
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
     QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
@@ -1548,7 +1551,7 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
             QP_TRY(deliver_member(w, cb, user, mb, hc[mb].stop_slice, QP_SLICE_STOPPED, nullptr, nullptr, nullptr,
                                   nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz, w->chi));
         }
-        if (cb) {
+        if (cb && i_end == 0) {
             const int cslot = (stopped ? hc[mb].stop_slice : 0) & 1;
             // a stopped member's B and chi planes hold the stopped slice: zero planes
             // instead (zero2 has a zero plane for every member)

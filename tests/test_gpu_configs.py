@@ -1,0 +1,224 @@
+"""GPU parity on the five benchmark configurations, against the FP64 oracle.
+
+Each test runs the configuration's own inputs (quetzalpic_amd.workloads:
+grid, box, slices, plasma profile, thermal momenta, laser, bunch) through
+the C-ABI and compares, per field, max |d| / max |ref| and ||d|| / ||ref||:
+
+* mixed path: <= 1e-5 on psi and e_x (e_x is the
+  longitudinal field, the reference's `ex`), <= 1e-4 on the other fields;
+* plain FP64 path: <= 1e-10 against the deterministic oracle build, <= 1e-8
+  against its -Ofast OpenMP build (reassociated sums and FMA contraction);
+  the OpenMP build runs the large configurations in seconds.
+
+Where the oracle would take minutes for the whole pass, the first S slices
+are compared (slice_limit on the device, the windowed oracle entry points),
+and the mixed path is compared with the plain FP64 path over a longer run.
+
+  C1  64^2,  200 slices, laser a0 = 1:           whole pass, both paths, all fields
+  C2  256^2, 1000 slices, channel, a0 = 2:       first 400 slices (laser peak at 300)
+  C3  512^2, 2000 slices, a0 = 4, thermal p:     first 500 slices (laser peak at 450),
+                                                 mixed vs FP64 over 1000 slices
+  C4  64 wakes, 64..256^2, 800 slices:           one member of each grid class, whole
+                                                 pass inside its class batch, and equal
+                                                 to the member run alone
+  C5  1024^2, 4000 slices, bunch + laser:        first 100 slices with the streamed bunch,
+                                                 mixed vs FP64 over 700 slices (bunch peak)
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+ALL = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ogrid(oracle, wl):
+    return oracle.grid_from_box(wl.lx, wl.L, wl.L, wl.dx0, wl.L / wl.n0, wl.L / wl.n0, K=wl.K)
+
+
+def make(qp_ctx, wl, precision, slice_limit=0, async_output=0):
+    from quetzalpic_amd import capi
+    wk = capi.Wake(qp_ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.L, wl.L, K=wl.K, precision=precision,
+                   slice_limit=slice_limit, async_output=async_output)
+    y, z, w, py, pz = wl.particles()
+    wk.set_particles(y, z, w, py, pz)
+    wk.set_laser_gaussian(*wl.laser)
+    return wk
+
+
+def compare(name, got, want, fields, tol_main, tol_other=None):
+    errs = {}
+    for f in fields:
+        errs[f] = rel_err(got[f], want[f])
+    log(f"[{name}] " + ", ".join(f"{f} {e[0]:.2e}/{e[1]:.2e}" for f, e in errs.items()))
+    for f in fields:
+        tol = tol_main if (f in ("psi", "ex") or tol_other is None) else tol_other
+        assert errs[f][0] < tol and errs[f][1] < tol, (name, f, errs[f])
+    return errs
+
+
+def test_c1_whole_pass(qp_ctx, oracle):
+    from quetzalpic_amd import capi, workloads
+    wl = workloads.get("C1")
+    g = ogrid(oracle, wl)
+    assert (g.nx, g.ny, g.nz) == (200, 64, 64)
+    y, z, w, _, _ = wl.particles()
+    a = oracle.gaussian_a_sqr(g, *wl.laser)
+    want, st0, _ = oracle.wake_run(g, y, z, w, a)
+    assert st0 == 0
+    wk = make(qp_ctx, wl, capi.QP_PREC_FP64)
+    wk.set_a_sqr(a)
+    got, st, _ = wk.run(fields=ALL)
+    wk.close()
+    assert st == 0
+    compare("C1 fp64", got, want, ALL, 1e-10)
+    wk = make(qp_ctx, wl, capi.QP_PREC_MIXED)
+    got, st, _ = wk.run(fields=ALL)
+    wk.close()
+    assert st == 0
+    compare("C1 mixed", got, want, ALL, 1e-5, 1e-4)
+
+
+def windowed(qp_ctx, oracle, wl, S, fields, rho_b=None, jx_b=None, bunch=None, precisions=None):
+    """First S slices of the configuration: oracle (OpenMP build, windowed
+    arrays) and the device paths with slice_limit = S."""
+    from quetzalpic_amd import capi
+    g = ogrid(oracle, wl)
+    y, z, w, py, pz = wl.particles()
+    a = oracle.gaussian_a_sqr_window(g, S, *wl.laser)
+    t0 = time.perf_counter()
+    want, st0, _ = oracle.wake_run_window(g, y, z, w, a, S, py=py, pz=pz, rho_b=rho_b, jx_b=jx_b, fields=fields,
+                                          variant="omp")
+    log(f"[{wl.name[:2]}] oracle {S} slices: {time.perf_counter() - t0:.1f} s")
+    assert st0 == 0
+    got = {}
+    for prec in precisions or (capi.QP_PREC_MIXED, capi.QP_PREC_FP64):
+        wk = make(qp_ctx, wl, prec, slice_limit=S)
+        if bunch is not None:
+            wk.attach_bunch(bunch)
+        out, st, _ = wk.run(fields=fields, slices=(wl.nx - S, wl.nx))
+        wk.close()
+        assert st == 0
+        got[prec] = out
+    return want, got
+
+
+def test_c2_leading_slices(qp_ctx, oracle):
+    from quetzalpic_amd import capi, workloads
+    wl = workloads.get("C2")
+    fields = ("psi", "ex", "ey", "by", "bz", "rho", "jx", "chi")
+    want, got = windowed(qp_ctx, oracle, wl, 400, fields)
+    compare("C2 mixed", got[capi.QP_PREC_MIXED], want, fields, 1e-5, 1e-4)
+    compare("C2 fp64", got[capi.QP_PREC_FP64], want, fields, 1e-8)
+
+
+def test_c3_leading_slices(qp_ctx, oracle):
+    from quetzalpic_amd import capi, workloads
+    wl = workloads.get("C3")
+    fields = ("psi", "ex", "ey", "by", "bz", "rho", "jx")
+    want, got = windowed(qp_ctx, oracle, wl, 500, fields)
+    compare("C3 mixed", got[capi.QP_PREC_MIXED], want, fields, 1e-5, 1e-4)
+    compare("C3 fp64", got[capi.QP_PREC_FP64], want, fields, 1e-8)
+
+
+def every(qp_ctx, wl, precision, S, step, fields, bunch=None):
+    """Run the first S slices and keep every `step`-th slice (host memory)."""
+    wk = make(qp_ctx, wl, precision, slice_limit=S)
+    if bunch is not None:
+        wk.attach_bunch(bunch)
+    keep = {f: {} for f in fields}
+
+    def cb(v):
+        if v.slice % step == 0 and v.kind == 0:
+            nn = wl.ny * wl.nz
+            for f in fields:
+                keep[f][v.slice] = np.ctypeslib.as_array(getattr(v, f), shape=(nn,)).copy()
+        return 0
+
+    _, st, _ = wk.run(callback=cb)
+    wk.close()
+    assert st == 0
+    return {f: np.stack([keep[f][i] for i in sorted(keep[f])]) for f in fields}
+
+
+def test_c3_mixed_vs_fp64_through_blowout(qp_ctx):
+    """Slices 2000 .. 1000: the a0 = 4 blowout (peak at slice 1550) and the wake behind it."""
+    from quetzalpic_amd import capi, workloads
+    wl = workloads.get("C3")
+    fields = ("psi", "ex")
+    m = every(qp_ctx, wl, capi.QP_PREC_MIXED, 1000, 5, fields)
+    f = every(qp_ctx, wl, capi.QP_PREC_FP64, 1000, 5, fields)
+    assert np.abs(f["psi"]).max() > 0.1   # a nonlinear wake
+    compare("C3 mixed vs fp64", m, f, fields, 1e-5)
+
+
+def test_c4_one_member_per_class(qp_ctx, oracle):
+    """Every grid class of the 64 wakes runs as one batch (as the benchmark does);
+    the first member of each class is compared with the oracle over the whole
+    pass, and with itself run alone."""
+    from quetzalpic_amd import capi, workloads
+    members = workloads.c4_members()
+    classes = {}
+    for i, m in enumerate(members):
+        classes.setdefault(m.ny, []).append(m)
+    assert sorted(classes) == [64, 96, 128, 192, 256]
+    for n in sorted(classes):
+        ms = classes[n]
+        m0 = ms[0]
+        b = capi.WakeBatch(qp_ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K)
+        for i, m in enumerate(ms):
+            y, z, w, py, pz = m.particles()
+            b.set_particles(i, y, z, w, py, pz)
+            b.set_laser_gaussian(i, *m.laser)
+        outs, st, stops = b.run(fields=("psi", "ex"), members=[0])
+        b.close()
+        assert st == 0
+        g = ogrid(oracle, m0)
+        y, z, w, py, pz = m0.particles()
+        t0 = time.perf_counter()
+        want, st0, _ = oracle.wake_run(g, y, z, w, oracle.gaussian_a_sqr(g, *m0.laser), py=py, pz=pz,
+                                       fields=("psi", "ex"), variant="omp")
+        log(f"[C4 {n}] oracle whole pass: {time.perf_counter() - t0:.1f} s")
+        assert st0 == 0
+        compare(f"C4 {n}x{n} member of {len(ms)}", outs[0], want, ("psi", "ex"), 1e-5)
+        wk = make(qp_ctx, m0, capi.QP_PREC_MIXED)
+        single, st1, _ = wk.run(fields=("psi", "ex"))
+        wk.close()
+        assert st1 == 0
+        compare(f"C4 {n}x{n} batch vs alone", outs[0], single, ("psi", "ex"), 1e-9)
+
+
+def test_c5_streamed_bunch(qp_ctx, oracle):
+    """The beam-driven wake: the bunch (7.7 M particles, x in [14.4, 19.6]) is
+    streamed through the slice loop on the device; the oracle gets its 3D
+    deposit on the same slices."""
+    from quetzalpic_amd import capi, workloads
+    from bench import FIELD_SCHWINGER
+    wl = workloads.get("C5")
+    g = ogrid(oracle, wl)
+    assert (g.nx, g.ny, g.nz) == (4000, 1024, 1024)
+    b0 = wl.bunch_particles()
+    bunch = capi.Bunch(qp_ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, FIELD_SCHWINGER)
+    bunch.set_particles(b0)
+    S = 100
+    rb, jb = oracle.bunch_deposit_window(g, b0, wl.nx - S, S)
+    assert np.abs(rb).max() > 0   # the bunch's head reaches these slices
+    fields = ("psi", "ex", "ey", "by", "bz", "jx")
+    want, got = windowed(qp_ctx, oracle, wl, S, fields, rho_b=rb, jx_b=jb, bunch=bunch)
+    compare("C5 mixed", got[capi.QP_PREC_MIXED], want, fields, 1e-5, 1e-4)
+    compare("C5 fp64", got[capi.QP_PREC_FP64], want, fields, 1e-8)
+    # through the bunch's peak (slice 600) and into the wake behind it
+    fields = ("psi", "ex")
+    m = every(qp_ctx, wl, capi.QP_PREC_MIXED, 700, 7, fields, bunch=bunch)
+    f = every(qp_ctx, wl, capi.QP_PREC_FP64, 700, 7, fields, bunch=bunch)
+    bunch.close()
+    compare("C5 mixed vs fp64, 700 slices", m, f, fields, 1e-5)
