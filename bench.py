@@ -1,10 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark of the quasistatic slice loop on one MI355X.
 
-A step is one pass of the slice loop (System_3d::solve_wakefield) over the
-workload's n_x slices: value = particle-slice pushes/s = N_p (n_x - 1) / t.
-Default workload: C2 (256^2, 262144 particles, 1000 slices).  Prints one JSON
-line on stdout (progress on stderr).
+Default workload: C5, the largest single-GPU configuration (1024^2 grid,
+4 194 304 plasma particles, 4000 slices, a 7.7 M-particle electron bunch
+streamed through the slice loop plus a trailing laser).  A step is one beam
+time step of System_3d::run (src/System_3d.cpp:199-236): one mixed-precision
+wake pass with the bunch's per-slice sources and field gathers on the device,
+then the bunch push.  value = plasma particle-slice pushes of the timed steps
+/ wall time = N_p (n_x - 1) K / t.  C1-C3 (one pass per step) and C4 (one
+pass of 64 wakes per step) are selected with --workload.  Prints one JSON line
+on stdout (progress on stderr).
 """
 import argparse
 import json
@@ -27,22 +32,46 @@ def log(*a):
 
 
 def bytes_per_particle(cls, K):
-    """Algorithmic HBM bytes per particle and launch of a particle phase
-    (persistent state S = 36 B: u32 cell, 2 x u32 offsets, 2 x f64 momenta,
-    f32 weight; scratch T = 8 B: 2 x f32 increments p* - p):
-      A: read S + T, write position + momenta (28 B)          = 72
-      B: read S (+ T after the first iteration), write T      = 44 | 52
-      C: read S + T                                           = 44
-      S: read S                                               = 36"""
+    """Algorithmic HBM bytes per particle and launch of a particle phase in the
+    build's own layout (persistent state S = 32 B: u32 cell, 2 x u32 in-cell
+    offsets, 2 x f64 momenta, f32 weight; scratch T = 8 B: 2 x f32 increments
+    p* - p):
+      A: read S + T, write cell, offsets, momenta (28 B)      = 68
+      B: read S (+ T after the first iteration), write T      = 40 | 48
+      C: read S + T                                           = 40
+      S: read S                                               = 32
+    500 B per particle-slice at K = 5."""
     if cls == "push_A":
-        return 72.0
+        return 68.0
     if cls == "push_B":
-        return (44.0 + 52.0 * (K - 1)) / K if K > 0 else 44.0
+        return (40.0 + 48.0 * (K - 1)) / K if K > 0 else 40.0
     if cls == "push_C":
-        return 44.0
+        return 40.0
     if cls == "push_S":
-        return 36.0
+        return 32.0
     raise KeyError(cls)
+
+
+def bytes_per_particle_spec(cls, K):
+    """The same for the reference layout of the target statement (persistent
+    S = 24 B: int cell 4, 2 x f32 offset, 2 x f32 momenta, f32 weight; scratch
+    T = 8 B: 2 x f32 p*): A reads S + T and writes position and momentum (20 B)
+    = 52; B_1 reads S and writes T = 32; B_k reads S + T and writes T = 40;
+    C reads S + T = 32.  116 + 72 (K - 1) = 404 B per particle-slice at K = 5.
+    The roofline's `achieved` uses these bytes."""
+    if cls == "push_A":
+        return 52.0
+    if cls == "push_B":
+        return (32.0 + 40.0 * (K - 1)) / K if K > 0 else 32.0
+    if cls == "push_C":
+        return 32.0
+    if cls == "push_S":
+        return 24.0
+    raise KeyError(cls)
+
+
+def spec_bytes_per_particle_slice(K):
+    return 116.0 + 72.0 * (K - 1)
 
 
 def solve_bytes(n1, n2):
@@ -141,67 +170,239 @@ def bench_envelope(capi, ctx, wl, steps, use_graph=1):
 FIELD_SCHWINGER = 1.4e7
 
 
-def bench_bunch_job(args, capi, workloads):
-    """C5: the whole beam-driven job, `steps` beam time steps of run()
-    (src/System_3d.cpp:199-236) without QED: the mixed wake pass with the
-    bunch streamed through it (per-slice sources and field gathers on the
-    device), then the bunch push.  value = plasma pushes of all passes / time."""
-    wl = workloads.get(args.workload)
+def cpu_info():
+    """Host CPU model and logical core count (the CPU baseline's hardware)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def class_profile(wk, classes):
+    """Mean device duration per launch of each kernel class (hipEvents on the
+    wake's stream around every launch of the class, one instrumented pass each)."""
+    out = {}
+    for cls in classes:
+        n, us = wk.profile_kernel(cls)
+        out[cls] = {"launches": n, "mean_us": us, "total_ms": n * us * 1e-3}
+        log(f"[bench]   {cls}: {n} launches x {us:.2f} us = {n * us * 1e-3:.1f} ms")
+    return out
+
+
+def roofline_of(classes, K, n_particles, workload):
+    """Roofline object of the push/deposit class with the largest total time:
+    achieved = the reference-layout algorithmic bytes of one launch (the
+    target's 404-B-per-particle-slice accounting) / its mean duration; the
+    build's own-layout bytes are reported beside it, and `traffic` is the
+    counter-measured HBM bytes per launch when a summary is committed."""
+    dom = max(("push_A", "push_B", "push_C"), key=lambda c: classes[c]["total_ms"] if c in classes else -1)
+    t = classes[dom]["mean_us"] * 1e-6
+    spec = bytes_per_particle_spec(dom, K) * n_particles
+    own = bytes_per_particle(dom, K) * n_particles
+    ach = spec / t / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom, workload),
+            "kernel": f"k_mx_{dom[-1]}", "algorithmic_bytes_per_launch": spec,
+            "own_layout_bytes_per_launch": own, "achieved_own_layout": round(own / t / 1e9, 1),
+            "frac_own_layout": round(own / t / 1e9 / HBM_PEAK_GBS, 4), "mean_us": classes[dom]["mean_us"]}
+
+
+def solve_roofline(classes, n1, n2):
+    """Bytes per transform launch (a third of a three-launch solve) / mean duration."""
+    if "solve" not in classes:
+        return None
+    sb = solve_bytes(n1, n2) / 3.0
+    ach = sb / (classes["solve"]["mean_us"] * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": sb,
+            "mean_us": classes["solve"]["mean_us"], "launches": classes["solve"]["launches"]}
+
+
+def err_pair(got, ref):
+    d = got - ref
+    return [float(np.abs(d).max() / max(np.abs(ref).max(), 1e-300)),
+            float(np.linalg.norm(d) / max(np.linalg.norm(ref), 1e-300))]
+
+
+def leading_slices(capi, ctx, wl, precision, S, step, bunch=None, fields=("psi", "ex")):
+    """The first S slices of one pass (slice_limit); keeps every step-th slice."""
+    wk = make_wake(capi, ctx, wl, precision, slice_limit=S)
+    if bunch is not None:
+        wk.attach_bunch(bunch)
+    keep = {f: {} for f in fields}
+    nn = wl.ny * wl.nz
+
+    def cb(v):
+        if v.kind == 0 and (wl.nx - 1 - v.slice) % step == 0:
+            for f in fields:
+                keep[f][v.slice] = np.ctypeslib.as_array(getattr(v, f), shape=(nn,)).reshape(wl.ny, wl.nz).copy()
+        return 0
+
+    t0 = time.perf_counter()
+    wk.run(callback=cb)
+    t = time.perf_counter() - t0
+    wk.close()
+    order = sorted(keep[fields[0]], reverse=True)
+    return {f: np.stack([keep[f][i] for i in order]) for f in fields}, order, t
+
+
+def cpu_baseline(wl, S, bunch=None):
+    """The oracle's OpenMP -Ofast build (the reference's loop structure and
+    Release flags, own FFT: FFTW is absent) over the first S slices of the same
+    workload, with windowed (slice-streamed) 3D arrays; timed on the host's
+    cores.  Returns the baseline object and the oracle's psi / e_x slices."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    from oracle import oracle_ctypes as oc
+    lib = oc.load("omp")
+    g = oc.grid_from_box(wl.lx, wl.L, wl.L, wl.dx0, wl.L / wl.n0, wl.L / wl.n0, K=wl.K)
+    y, z, w, py, pz = wl.particles()
+    a = oc.gaussian_a_sqr_window(g, S, *wl.laser)
+    rb = jb = None
+    if bunch is not None:
+        rb, jb = oc.bunch_deposit_window(g, bunch, g.nx - S, S)
+    t0 = time.perf_counter()
+    out, st, _ = oc.wake_run_window(g, y, z, w, a, S, py=py, pz=pz, rho_b=rb, jx_b=jb, fields=("psi", "ex"),
+                                    variant="omp")
+    t = time.perf_counter() - t0
+    model, ncpu = cpu_info()
+    np_ = len(y)
+    res = {"value": np_ * S / t, "unit": "particle-slice pushes/s", "cores": int(lib.qpo_num_threads()),
+           "kind": "port", "cpu_model": model, "host_logical_cpus": ncpu,
+           "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "sample": f"first {S} of {g.nx} slices of the same workload ({np_} particles), slice-streamed "
+                     f"3D arrays, FP64, OpenMP atomics, own mixed-radix FFT (FFTW absent), -Ofast; {t:.1f} s"}
+    return res, {f: out[f][::-1] for f in ("psi", "ex")}
+
+
+def timed_steps(ctx, steps, one_step):
+    """W untimed steps are the caller's; here: synchronize, K steps, synchronize."""
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        one_step(k)
+    ctx.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_c5(args, capi, workloads):
+    """C5: the beam-driven job.  A step is one beam time step of run()
+    (src/System_3d.cpp:199-236) without QED: the mixed wake pass with the bunch
+    streamed through it (per-slice sources and field gathers on the device),
+    then the bunch push.  value = plasma particle-slice pushes / wall time."""
+    wl = workloads.get("C5")
     bp = wl.bunch
-    steps = args.beam_steps or bp["steps"]
     b0 = wl.bunch_particles()
     nb = len(b0["x"])
     log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles} bunch={nb}")
     ctx = capi.Context(0)
-    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=0 if args.no_graph else 1)
+    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=0 if args.no_graph else 1,
+                   slice_limit=args.slice_limit)
     db = capi.Bunch(ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, FIELD_SCHWINGER)
     wk.attach_bunch(db)
     db.set_particles(b0)
-    for _ in range(args.warmup):     # one beam step, then back to the initial bunch
-        wk.run()
-        db.push(bp["dt"])
-    db.set_particles(b0)
-    ctx.synchronize()
     dev_ms = []
-    t0 = time.perf_counter()
-    for s_ in range(steps):
-        _, st, stop = wk.run()
+
+    def step(k):
+        wk.run()
         dev_ms.append(wk.last_time_ms())
         db.push(bp["dt"])
-        log(f"[bench]   beam step {s_}: pass {dev_ms[-1]:.1f} ms (status {st})")
-    ctx.synchronize()
-    t = time.perf_counter() - t0
-    pushes = wl.n_particles * (wl.nx - 1) * steps
+        log(f"[bench]   beam step {k}: pass {dev_ms[-1]:.1f} ms")
+
+    for k in range(args.warmup):
+        step(k)
+    db.set_particles(b0)
+    dev_ms.clear()
+    t = timed_steps(ctx, args.steps, step)
+    slices = args.slice_limit or wl.nx
+    pushes = wl.n_particles * (slices - 1) * args.steps
     value = pushes / t
     final = db.get_particles()
-    log(f"[bench] C5: {steps} beam steps in {t:.2f} s -> {value:.3e} pushes/s; bunch <gamma> "
-        f"{float(np.mean(final['gamma'])):.2f}")
-    classes, roofline = {}, None
+    gamma_mean = float(np.mean(final["gamma"]))
+    log(f"[bench] C5: {args.steps} beam steps in {t:.2f} s -> {value:.3e} pushes/s; bunch <gamma> {gamma_mean:.2f}")
+    classes, roofline, sroof = {}, None, None
     if not args.no_profile:
-        for cls in ("push_B", "push_C"):
-            n, us = wk.profile_kernel(cls)
-            classes[cls] = {"launches": n, "mean_us": us, "total_ms": n * us * 1e-3}
-            log(f"[bench]   {cls}: {n} launches x {us:.2f} us")
-        dom = max(classes, key=lambda c: classes[c]["total_ms"])
-        byts = bytes_per_particle(dom, wl.K) * wl.n_particles
-        ach = byts / (classes[dom]["mean_us"] * 1e-6) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_mx_{dom[-1]}",
-                    "algorithmic_bytes_per_launch": byts}
+        db.set_particles(b0)
+        classes = class_profile(wk, ("push_A", "push_B", "push_C", "solve"))
+        roofline = roofline_of(classes, wl.K, wl.n_particles, "C5")
+        sroof = solve_roofline(classes, wl.ny, wl.nz)
+        log(f"[bench] roofline {roofline}")
     wk.close()
+    err64 = fp64_rate = None
+    mixed_lead = None
+    if not args.no_fp64:
+        S, step_k = 300, 3
+        db.set_particles(b0)
+        mixed_lead, order, _ = leading_slices(capi, ctx, wl, capi.QP_PREC_MIXED, S, step_k, bunch=db)
+        db.set_particles(b0)
+        f64, order64, t64 = leading_slices(capi, ctx, wl, capi.QP_PREC_FP64, S, step_k, bunch=db)
+        assert order == order64
+        err64 = {f: err_pair(mixed_lead[f], f64[f]) for f in ("psi", "ex")}
+        fp64_rate = wl.n_particles * S / t64
+        log(f"[bench] mixed vs fp64 GPU over the first {S} slices (max, rms rel): {err64}")
+    cpu = err_cpu = None
+    if not args.no_cpu:
+        S = args.cpu_slices
+        cpu, oref = cpu_baseline(wl, S, bunch=b0)
+        if mixed_lead is not None:
+            k = (S + 2) // 3   # the kept slices among the first S (every 3rd)
+            err_cpu = {f: err_pair(mixed_lead[f][:k], oref[f][::3][:k]) for f in ("psi", "ex")}
+        log(f"[bench] cpu baseline: {cpu['value']:.3e} pushes/s on {cpu['cores']} threads ({cpu['cpu_model']}); "
+            f"err vs oracle {err_cpu}")
     db.close()
+    extra = {} if args.no_extra else bench_extras(args, capi, workloads, ctx)
     ctx.close()
     line = {
-        "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, beam-driven job)",
-        "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * t / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic", "config": wl.describe(),
-        "bunch_particles": nb, "device_ms_per_pass": float(np.median(dev_ms)),
-        "kernel_classes": classes, "roofline": roofline, "cpu_baseline": None,
+        "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, <=1e-5 rel err in psi and Ez vs FP64)",
+        "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "storage": "plasma particles: u32 cell + u32x2 offsets, f64x2 momenta, f32 weight, f32x2 p* increments; "
+                   "grids and solves f64; bunch f64 SoA",
+        "config": {**wl.describe(), "bunch_particles": nb,
+                   "step": "one beam time step: mixed wake pass with the bunch streamed through it + bunch push"},
+        "solves_per_s": ((1 + 3 * wl.K) * (slices - 1) + 1) * args.steps / t,
+        "device_ms_per_pass": float(np.median(dev_ms)) if dev_ms else None,
+        "bunch_mean_gamma_after": gamma_mean,
+        "kernel_classes": classes, "roofline": roofline, "solve_roofline": sroof,
+        "fp64_gpu_pushes_per_s_leading": fp64_rate,
+        "err_vs_fp64_gpu_leading": err64, "err_vs_oracle_sample": err_cpu,
+        "cpu_baseline": cpu, "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "other_workloads": extra,
         "note": "3D arrays of the reference's layout (281 GiB at this size) are never formed: the bunch "
                 "sources and field gathers are streamed slice by slice",
     }
     print(json.dumps(line), flush=True)
+
+
+def pass_rate(capi, ctx, wl, steps):
+    """Whole-pass rate of one wake (mixed, hipGraph slice step): (ms, pushes/s)."""
+    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED)
+    wk.run()
+    t = timed_steps(ctx, steps, lambda k: wk.run())
+    wk.close()
+    ms = 1e3 * t / steps
+    return ms, wl.n_particles * (wl.nx - 1) / (ms * 1e-3)
+
+
+def bench_extras(args, capi, workloads, ctx):
+    """The smaller configurations beside the headline (no checks: their parity
+    is in tests/test_gpu_configs.py)."""
+    out = {}
+    for name, steps in (("C2", 3), ("C3", 2)):
+        wl = workloads.get(name)
+        ms, rate = pass_rate(capi, ctx, wl, steps)
+        out[name] = {"ms_per_pass": ms, "pushes_per_s": rate, "particles": wl.n_particles, "slices": wl.nx}
+        log(f"[bench] {name}: {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
+    ms, rate, desc = c4_pass(capi, workloads, 2)
+    out["C4"] = {"ms_per_pass": ms, "pushes_per_s": rate, **desc}
+    log(f"[bench] C4: {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
+    return out
 
 
 def calib_copy(mib):
@@ -221,79 +422,51 @@ def calib_copy(mib):
     hip.hipFree(b)
 
 
-def make_wake(capi, ctx, wl, precision, use_graph=1, async_output=0):
+def make_wake(capi, ctx, wl, precision, use_graph=1, async_output=0, slice_limit=0):
     w = capi.Wake(ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.L, wl.L, K=wl.K, precision=precision,
-                  use_graph=use_graph, resort_every=0, async_output=async_output)
+                  use_graph=use_graph, resort_every=0, async_output=async_output, slice_limit=slice_limit)
     y, z, wt, py, pz = wl.particles()
     w.set_particles(y, z, wt, py, pz)
     w.set_laser_gaussian(*wl.laser)
     return w
 
 
-def cpu_baseline(wl, budget_s, gpu_fields):
-    """The oracle's OpenMP -Ofast build (the reference's loop structure and
-    Release flags) on the first S slices of the same workload."""
-    from oracle import oracle_ctypes as oc
-    lib = oc.load("omp")
-    g = oc.grid_from_box(wl.lx, wl.L, wl.L, wl.dx0, wl.L / wl.n0, wl.L / wl.n0, K=wl.K)
-    y, z, w, py, pz = wl.particles()
-    a = oc.gaussian_a_sqr(g, *wl.laser)
-    # calibrate on 3 slices, then size the sample to the time budget
-    t0 = time.perf_counter()
-    oc.wake_run(g, y, z, w, a, py=py, pz=pz, n_slices=3, fields=(), variant="omp")
-    per = (time.perf_counter() - t0) / 3
-    s = int(max(3, min(g.nx - 1, budget_s / max(per, 1e-6))))
-    t0 = time.perf_counter()
-    out, _, _ = oc.wake_run(g, y, z, w, a, py=py, pz=pz, n_slices=s, fields=("psi", "ex"), variant="omp")
-    t = time.perf_counter() - t0
-    np_ = len(y)
-    res = {"value": np_ * s / t, "unit": "particle-slice pushes/s", "cores": int(lib.qpo_num_threads()),
-           "kind": "port",
-           "sample": f"first {s} of {g.nx} slices of the same workload ({np_} particles), FP64, "
-                     f"OpenMP atomics, own mixed-radix FFT, -Ofast; {t:.1f} s"}
-    errs = {}
-    if gpu_fields is not None:
-        lo = g.nx - s
-        for f in ("psi", "ex"):
-            ref = out[f][lo:g.nx]
-            got = gpu_fields[f][lo:g.nx]
-            if f == "ex":  # the last slice of ex is zero in both
-                ref, got = ref[:-1], got[:-1]
-            d = got - ref
-            errs[f] = [float(np.abs(d).max() / max(np.abs(ref).max(), 1e-300)),
-                       float(np.linalg.norm(d) / max(np.linalg.norm(ref), 1e-300))]
-    return res, errs
-
-
-def bench_c4(args, capi, workloads):
+class C4Job:
     """C4: 64 independent wakes.  The members of one grid size form a batch
     (qp_wake_create_batch: every launch of a batch carries all its members); the
     batches run concurrently, each on its own context (stream) from its own host
-    thread.  value = all members' particle-slice pushes of a pass / wall time."""
-    import threading
-    members = workloads.c4_members()
-    if args.c4_class:  # one grid-size class only (profiling runs)
-        members = [m for m in members if m.ny == args.c4_class]
-    classes = {}
-    for m in members:
-        classes.setdefault(m.ny, []).append(m)
-    sizes = sorted(classes)
-    batches = {}
-    for n in sizes:
-        ms = classes[n]
-        m0 = ms[0]
-        ctx = capi.Context(0)
-        b = capi.WakeBatch(ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K,
-                           use_graph=0 if args.no_graph else 1)
-        for i, m in enumerate(ms):
-            y, z, w, py, pz = m.particles()
-            b.set_particles(i, y, z, w, py, pz)
-            b.set_laser_gaussian(i, *m.laser)
-        batches[n] = (ctx, b, sum(m.n_particles for m in ms))
-        log(f"[bench] C4 class {n}x{n}: {len(ms)} members, {batches[n][2]} particles")
-    pushes = sum(m.n_particles * m.nx for m in members)
+    thread."""
 
-    def run_all():
+    def __init__(self, capi, workloads, use_graph=1, only_class=0):
+        members = workloads.c4_members()
+        if only_class:  # one grid-size class only (profiling runs)
+            members = [m for m in members if m.ny == only_class]
+        self.members = members
+        self.classes = {}
+        for m in members:
+            self.classes.setdefault(m.ny, []).append(m)
+        self.sizes = sorted(self.classes)
+        self.batches = {}
+        for n in self.sizes:
+            ms = self.classes[n]
+            m0 = ms[0]
+            ctx = capi.Context(0)
+            b = capi.WakeBatch(ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K,
+                               use_graph=use_graph)
+            for i, m in enumerate(ms):
+                y, z, w, py, pz = m.particles()
+                b.set_particles(i, y, z, w, py, pz)
+                b.set_laser_gaussian(i, *m.laser)
+            self.batches[n] = (ctx, b, sum(m.n_particles for m in ms))
+            log(f"[bench] C4 class {n}x{n}: {len(ms)} members, {self.batches[n][2]} particles")
+        self.pushes = sum(m.n_particles * (m.nx - 1) for m in members)
+
+    def describe(self):
+        return {"members": {f"{n}x{n}": len(self.classes[n]) for n in self.sizes},
+                "particles": sum(m.n_particles for m in self.members), "slices": self.members[0].nx}
+
+    def run(self):
+        import threading
         errs = []
 
         def one(b):
@@ -302,7 +475,7 @@ def bench_c4(args, capi, workloads):
             except Exception as e:  # surfaced below
                 errs.append(e)
 
-        ths = [threading.Thread(target=one, args=(b,)) for (_, b, _) in batches.values()]
+        ths = [threading.Thread(target=one, args=(b,)) for (_, b, _) in self.batches.values()]
         for t in ths:
             t.start()
         for t in ths:
@@ -310,63 +483,82 @@ def bench_c4(args, capi, workloads):
         if errs:
             raise errs[0]
 
+    def synchronize(self):
+        for (ctx, _, _) in self.batches.values():
+            ctx.synchronize()
+
+    def close(self):
+        for (ctx, b, _) in self.batches.values():
+            b.close()
+            ctx.close()
+
+
+def c4_pass(capi, workloads, steps, warmup=1):
+    job = C4Job(capi, workloads)
+    for _ in range(warmup):
+        job.run()
+    job.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        job.run()
+    job.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    desc = job.describe()
+    rate = job.pushes / (ms * 1e-3)
+    job.close()
+    return ms, rate, desc
+
+
+def bench_c4(args, capi, workloads):
+    """value = all members' particle-slice pushes of a pass / wall time."""
+    job = C4Job(capi, workloads, use_graph=0 if args.no_graph else 1, only_class=args.c4_class)
     for _ in range(args.warmup):
-        run_all()
+        job.run()
+    job.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        run_all()
-    t1 = time.perf_counter()
-    ms_per_step = 1e3 * (t1 - t0) / args.steps
-    value = pushes / (ms_per_step * 1e-3)
-    log(f"[bench] C4: {ms_per_step:.1f} ms per pass of 64 wakes = {value:.3e} pushes/s")
-    # each class alone (device time of one pass)
+        job.run()
+    job.synchronize()
+    ms_per_step = 1e3 * (time.perf_counter() - t0) / args.steps
+    value = job.pushes / (ms_per_step * 1e-3)
+    log(f"[bench] C4: {ms_per_step:.1f} ms per pass of {len(job.members)} wakes = {value:.3e} pushes/s")
     alone = {}
-    for n in (sizes if not args.no_profile else []):
-        ctx, b, npart = batches[n]
+    for n in (job.sizes if not args.no_profile else []):
+        ctx, b, npart = job.batches[n]
         b.run()
-        alone[f"{n}x{n}"] = {"members": len(classes[n]), "particles": npart, "ms": b.last_time_ms(),
-                             "pushes_per_s": npart * classes[n][0].nx / (b.last_time_ms() * 1e-3)}
+        alone[f"{n}x{n}"] = {"members": len(job.classes[n]), "particles": npart, "ms": b.last_time_ms(),
+                             "pushes_per_s": npart * (job.classes[n][0].nx - 1) / (b.last_time_ms() * 1e-3)}
     # roofline of the dominant push kernel in the largest class
-    ctx, b, npart = batches[sizes[-1]]
+    ctx, b, npart = job.batches[job.sizes[-1]]
     kc = {}
-    for cls in (("push_A", "push_B", "push_C", "solve") if not args.no_profile else ()):
-        nl, us = b.profile_kernel(cls)
-        kc[cls] = {"launches": nl, "mean_us": us, "total_ms": nl * us * 1e-3}
     roofline = None
-    if kc:
-        dom = max(("push_A", "push_B", "push_C"), key=lambda c: kc[c]["total_ms"])
-        dom_bytes = bytes_per_particle(dom, members[0].K) * npart
-        achieved = dom_bytes / (kc[dom]["mean_us"] * 1e-6) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_mx_{dom[-1]}",
-                    "algorithmic_bytes_per_launch": dom_bytes, "class": f"{sizes[-1]}x{sizes[-1]}"}
-    # parity: the first member of the smallest class against the FP64 oracle, whole pass
-    err = None
-    cpu = None
+    if not args.no_profile:
+        kc = class_profile(b, ("push_A", "push_B", "push_C", "solve"))
+        roofline = roofline_of(kc, job.members[0].K, npart, f"C4_{job.sizes[-1]}")
+        roofline["class"] = f"{job.sizes[-1]}x{job.sizes[-1]}"
+    err = cpu = None
     if not args.no_check:
         from oracle import oracle_ctypes as oc
-        m = classes[sizes[0]][0]
-        ctx, b, _ = batches[sizes[0]]
-        outs, _, _ = b.run(fields=("psi", "ex"))
+        m = job.classes[job.sizes[0]][0]
+        ctx, b, _ = job.batches[job.sizes[0]]
+        outs, _, _ = b.run(fields=("psi", "ex"), members=[0])
         g = oc.grid_from_box(m.lx, m.L, m.L, m.dx0, m.L / m.n0, m.L / m.n0, K=m.K)
         y, z, w, py, pz = m.particles()
         a = oc.gaussian_a_sqr(g, *m.laser)
+        os.environ.setdefault("OMP_PROC_BIND", "close")
         t0 = time.perf_counter()
         want, _, _ = oc.wake_run(g, y, z, w, a, py=py, pz=pz, fields=("psi", "ex"), variant="omp")
         tc = time.perf_counter() - t0
-        err = {}
-        for f in ("psi", "ex"):
-            d = outs[0][f] - want[f]
-            err[f] = [float(np.abs(d).max() / np.abs(want[f]).max()),
-                      float(np.linalg.norm(d) / np.linalg.norm(want[f]))]
+        err = {f: err_pair(outs[0][f], want[f]) for f in ("psi", "ex")}
         lib = oc.load("omp")
-        cpu = {"value": len(y) * g.nx / tc, "unit": "particle-slice pushes/s", "cores": int(lib.qpo_num_threads()),
-               "kind": "port", "sample": f"one whole C4 member ({m.ny}x{m.nz}, {len(y)} particles, {g.nx} slices), "
-                                         f"FP64, OpenMP atomics, -Ofast; {tc:.1f} s"}
+        model, ncpu = cpu_info()
+        cpu = {"value": len(y) * (g.nx - 1) / tc, "unit": "particle-slice pushes/s",
+               "cores": int(lib.qpo_num_threads()), "kind": "port", "cpu_model": model, "host_logical_cpus": ncpu,
+               "sample": f"one whole C4 member ({m.ny}x{m.nz}, {len(y)} particles, {g.nx} slices), "
+                         f"FP64, OpenMP atomics, -Ofast; {tc:.1f} s"}
         log(f"[bench] C4 member vs oracle: {err}")
-    for (ctx, b, _) in batches.values():
-        b.close()
-        ctx.close()
+    desc = job.describe()
+    job.close()
     line = {
         "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, <=1e-5 rel err in psi and Ez vs FP64)",
         "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -374,11 +566,68 @@ def bench_c4(args, capi, workloads):
         "data": "synthetic",
         "config": {"workload": "C4: 64 wakes over [-12,12]^2, grids from {64,96,128,192,256} (one batch per size, "
                                "batches on concurrent streams), 4 ppc, 800 slices dxi=0.025, a0~U[0.5,3], "
-                               "w0~U[2,5], sigma_xi=2, transverse density up-ramp slope~U[0,0.1]",
-                   "members": {f"{n}x{n}": len(classes[n]) for n in sizes}, "particles": sum(
-                       m.n_particles for m in members), "slices": members[0].nx},
+                               "w0~U[2,5], sigma_xi=2, transverse density up-ramp slope~U[0,0.1]", **desc},
         "classes_alone": alone, "kernel_classes_largest": kc, "roofline": roofline,
         "err_vs_oracle_member": err, "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_single(args, capi, workloads):
+    """C1-C3: one wake pass per step (hipGraph slice step)."""
+    wl = workloads.get(args.workload)
+    log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles}")
+    ctx = capi.Context(0)
+    if args.envelope_only:
+        res = bench_envelope(capi, ctx, wl, args.steps, use_graph=0 if args.no_graph else 1)
+        print(json.dumps({"envelope": res, "config": wl.describe()}), flush=True)
+        ctx.close()
+        return
+    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=0 if args.no_graph else 1,
+                   slice_limit=args.slice_limit)
+    for _ in range(args.warmup):
+        wk.run()
+    dev_ms = []
+
+    def step(k):
+        wk.run()
+        dev_ms.append(wk.last_time_ms())
+
+    t = timed_steps(ctx, args.steps, step)
+    ms_per_step = 1e3 * t / args.steps
+    slices = args.slice_limit or wl.nx
+    value = wl.n_particles * (slices - 1) / (ms_per_step * 1e-3)
+    log(f"[bench] mixed: {ms_per_step:.2f} ms/pass wall, device {np.median(dev_ms):.2f} ms -> {value:.3e} pushes/s")
+    tuning = json.loads(wk.describe())
+    classes, roofline, sroof = {}, None, None
+    if not args.no_profile:
+        classes = class_profile(wk, ("push_A", "push_B", "push_C", "solve"))
+        roofline = roofline_of(classes, wl.K, wl.n_particles, args.workload)
+        sroof = solve_roofline(classes, wl.ny, wl.nz)
+    wk.close()
+    err64 = fp64_rate = None
+    if not args.no_fp64:
+        S = min(wl.nx, 400)
+        m, order, _ = leading_slices(capi, ctx, wl, capi.QP_PREC_MIXED, S, 1)
+        f64, _, t64 = leading_slices(capi, ctx, wl, capi.QP_PREC_FP64, S, 1)
+        err64 = {f: err_pair(m[f], f64[f]) for f in ("psi", "ex")}
+        fp64_rate = wl.n_particles * S / t64
+        log(f"[bench] mixed vs fp64 GPU over the first {S} slices: {err64}")
+    cpu = None
+    if not args.no_cpu:
+        cpu, _ = cpu_baseline(wl, min(args.cpu_slices, wl.nx))
+        log(f"[bench] cpu baseline: {cpu['value']:.3e} pushes/s on {cpu['cores']} threads")
+    ctx.close()
+    line = {
+        "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, <=1e-5 rel err in psi and Ez vs FP64)",
+        "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic", "config": wl.describe(),
+        "solves_per_s": ((1 + 3 * wl.K) * (slices - 1) + 1) / (ms_per_step * 1e-3),
+        "device_ms_per_step": float(np.median(dev_ms)), "kernel_classes": classes, "launch_shapes": tuning,
+        "roofline": roofline, "solve_roofline": sroof, "fp64_gpu_pushes_per_s_leading": fp64_rate,
+        "err_vs_fp64_gpu_leading": err64, "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(line), flush=True)
@@ -387,20 +636,21 @@ def bench_c4(args, capi, workloads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed steps (C5: beam time steps; else passes)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="C2")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--workload", default="C5", choices=("C1", "C2", "C3", "C4", "C5"))
+    ap.add_argument("--cpu-slices", type=int, default=20, help="slices of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="C5: skip the C2 / C3 / C4 lines beside the headline")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the slice step directly instead of replaying hipGraphs (for tracers)")
-    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel-class event pass")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel-class event passes")
+    ap.add_argument("--slice-limit", type=int, default=0,
+                    help="run only the first N slices of every pass (profiling runs; value counts those)")
     ap.add_argument("--c4-class", type=int, default=0,
                     help="C4 only: run the members of this grid size alone (profiling runs)")
-    ap.add_argument("--beam-steps", type=int, default=0,
-                    help="C5: beam time steps to time (default: the configuration's 10; fewer for profiling)")
     ap.add_argument("--envelope-only", action="store_true",
                     help="time only the laser-envelope advance on the workload's grid (profiling runs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -416,145 +666,10 @@ def main():
         calib_copy(args.calib_copy_mib)
     if args.workload == "C4":
         bench_c4(args, capi, workloads)
-        return
-    if args.workload == "C5":
-        bench_bunch_job(args, capi, workloads)
-        return
-    wl = workloads.get(args.workload)
-    log(f"[bench] workload {wl.name}: nx={wl.nx} n={wl.ny} particles={wl.n_particles}")
-    ctx = capi.Context(0)
-    if args.envelope_only:
-        res = bench_envelope(capi, ctx, wl, args.steps, use_graph=0 if args.no_graph else 1)
-        print(json.dumps({"envelope": res, "config": wl.describe()}), flush=True)
-        ctx.close()
-        return
-
-    # ---- timed region: the mixed fused path, hipGraph slice step
-    wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, use_graph=0 if args.no_graph else 1)
-    for _ in range(args.warmup):
-        wk.run()
-    ctx.synchronize()
-    dev_ms = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        wk.run()
-        dev_ms.append(wk.last_time_ms())
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = 1e3 * (t1 - t0) / args.steps
-    pushes = wl.n_particles * (wl.nx - 1)
-    solves = (1 + 3 * wl.K) * (wl.nx - 1) + 1
-    value = pushes / (ms_per_step * 1e-3)
-    log(f"[bench] mixed: {ms_per_step:.2f} ms/pass wall, device {np.median(dev_ms):.2f} ms -> {value:.3e} pushes/s")
-    tuning = json.loads(wk.describe())
-    log(f"[bench] launch shapes: {tuning}")
-
-    # ---- per-kernel-class device times (separate instrumented pass, same work):
-    # hipEvents on the wake's stream around every launch of the class
-    classes = {}
-    roofline = None
-    if not args.no_profile:
-        for cls in ("push_A", "push_B", "push_C", "solve"):
-            n, us = wk.profile_kernel(cls)
-            classes[cls] = {"launches": n, "mean_us": us, "total_ms": n * us * 1e-3}
-            log(f"[bench]   {cls}: {n} launches x {us:.2f} us = {n * us * 1e-3:.2f} ms")
-        # push/deposit kernel with the largest total time (the particle streaming the metric counts)
-        dom = max(("push_A", "push_B", "push_C"), key=lambda c: classes[c]["total_ms"])
-        dom_bytes = bytes_per_particle(dom, wl.K) * wl.n_particles
-        achieved = dom_bytes / (classes[dom]["mean_us"] * 1e-6) / 1e9
-        traffic = read_traffic(dom, args.workload)
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": f"k_mx_{dom[-1]}",
-                    "algorithmic_bytes_per_launch": dom_bytes}
-        sb = solve_bytes(wl.ny, wl.nz) / 3.0
-        classes["solve"]["algorithmic_bytes_per_launch"] = sb
-        classes["solve"]["achieved_GBs"] = sb / (classes["solve"]["mean_us"] * 1e-6) / 1e9
-
-    # ---- plain FP64 GPU path (one kernel per reference loop) on the same workload
-    fp64 = None
-    fields64 = None
-    if not args.no_fp64:
-        wf = make_wake(capi, ctx, wl, capi.QP_PREC_FP64)
-        wf.run()
-        t0 = time.perf_counter()
-        wf.run()
-        t64 = time.perf_counter() - t0
-        fp64 = pushes / t64
-        log(f"[bench] fp64 plain: {t64 * 1e3:.1f} ms/pass -> {fp64:.3e} pushes/s")
-        if not args.no_check:
-            fields64, _, _ = wf.run(fields=("psi", "ex"))
-        wf.close()
-
-    # ---- error of the mixed path against the FP64 path over the whole pass
-    err64 = None
-    fieldsmx = None
-    if not args.no_check:
-        fieldsmx, _, _ = wk.run(fields=("psi", "ex"))
-        if fields64 is not None:
-            err64 = {}
-            for f in ("psi", "ex"):
-                d = fieldsmx[f] - fields64[f]
-                err64[f] = [float(np.abs(d).max() / np.abs(fields64[f]).max()),
-                            float(np.linalg.norm(d) / np.linalg.norm(fields64[f]))]
-            log(f"[bench] mixed vs fp64 GPU (max, rms rel): {err64}")
-    wk.close()
-
-    # ---- passes with every slice's eleven output planes handed to a host
-    # callback (outside the timed region): inline vs streamed
-    output_pass = None
-    if not args.no_check:
-        output_pass = {}
-        for mode in (0, 1):
-            wo = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, async_output=mode)
-            wo.run(fields=FIELD_NAMES_ALL)
-            t0 = time.perf_counter()
-            wo.run(fields=FIELD_NAMES_ALL)
-            output_pass["async" if mode else "inline"] = 1e3 * (time.perf_counter() - t0)
-            wo.close()
-        log(f"[bench] pass with all planes to a host callback: inline {output_pass['inline']:.1f} ms, "
-            f"streamed {output_pass['async']:.1f} ms")
-
-    # ---- laser-envelope advance on the same grid (outside the timed region)
-    envelope = bench_envelope(capi, ctx, wl, max(args.steps, 3), use_graph=0 if args.no_graph else 1)
-
-    cpu = None
-    err_oracle = None
-    if not args.no_cpu:
-        cpu, err_oracle = cpu_baseline(wl, args.cpu_budget, fieldsmx)
-        log(f"[bench] cpu baseline: {cpu['value']:.3e} pushes/s on {cpu['cores']} threads; err vs oracle "
-            f"{err_oracle}")
-    ctx.close()
-
-    line = {
-        "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, <=1e-5 rel err in psi and Ez vs FP64)",
-        "value": value,
-        "unit": "particle-slice pushes/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "storage": "particles: u32 cell + u32x2 offsets, f64x2 momenta, f32 weight, f32x2 p* increments; grids and solves f64",
-        "data": "synthetic",
-        "config": wl.describe(),
-        "solves_per_s": solves / (ms_per_step * 1e-3),
-        "device_ms_per_step": float(np.median(dev_ms)),
-        "kernel_classes": classes,
-        "launch_shapes": tuning,
-        "roofline": roofline,
-        "fp64_gpu_pushes_per_s": fp64,
-        "speedup_vs_fp64_gpu": (value / fp64) if fp64 else None,
-        "err_vs_fp64_gpu": err64,
-        "err_vs_oracle_sample": err_oracle,
-        "cpu_baseline": cpu,
-        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "envelope_advance": envelope,
-        "output_pass_ms": output_pass,
-    }
-    print(json.dumps(line), flush=True)
+    elif args.workload == "C5":
+        bench_c5(args, capi, workloads)
+    else:
+        bench_single(args, capi, workloads)
 
 
 if __name__ == "__main__":
