@@ -69,6 +69,7 @@ def load_path(variant, path):
     load deliberately broken copies of the oracle this way)."""
     lib = ctypes.CDLL(path)
     lib.qpo_fft.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+    lib.qpo_set_solve_f32.argtypes = [ctypes.c_int]
     lib.qpo_rfft2.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
     lib.qpo_irfft2.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
     lib.qpo_solve_poisson.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

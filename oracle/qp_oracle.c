@@ -303,11 +303,24 @@ void qpo_irfft2(int n1, int n2, const double *in, double *out) {
     plan2d_free(&p);
 }
 
+/* Precision study only (not a reference behaviour): with a non-zero value the
+ * solve rounds its input, its spectrum before and after the eigenvalue
+ * division, and its output to single precision, a lower bound on the
+ * rounding of any transform that stores or accumulates in FP32 (an FP32 FFT,
+ * or a split-BF16 dense product accumulated in FP32). */
+static int g_solve_f32 = 0;
+void qpo_set_solve_f32(int on) { g_solve_f32 = on; }
+static void round_f32(double *a, size_t n) {
+    for (size_t i = 0; i < n; ++i) a[i] = (double)(float)a[i];
+}
+
 /* solve_poisson_equation(D), src/System_3d.cpp:686-706. */
 static void solve_poisson_plan(const plan2d *p, double d1, double d2, double D, double *inout,
                                double *spec) {
     const int n1 = p->n1, n2 = p->n2, nh = p->nh;
+    if (g_solve_f32) round_f32(inout, (size_t)n1 * n2);
     rfft2_exec(p, inout, spec);
+    if (g_solve_f32) round_f32(spec, 2 * (size_t)n1 * nh);
     spec[0] = 0.0;
     spec[1] = 0.0;
 #pragma omp parallel for
@@ -320,7 +333,9 @@ static void solve_poisson_plan(const plan2d *p, double d1, double d2, double D, 
             spec[2 * ((size_t)nh * j + k) + 1] /= mult;
         }
     }
+    if (g_solve_f32) round_f32(spec, 2 * (size_t)n1 * nh);
     irfft2_exec(p, spec, inout);
+    if (g_solve_f32) round_f32(inout, (size_t)n1 * n2);
 }
 
 void qpo_solve_poisson(int n1, int n2, double d1, double d2, double D, double *inout) {

@@ -71,6 +71,9 @@ void qpo_irfft2(int n1, int n2, const double *in, double *out);
  * r2c, zero the (0,0) mode, divide by the 5-point eigenvalue, c2r.
  * Unnormalised: inout holds n1*n2 times the solution on return. */
 void qpo_solve_poisson(int n1, int n2, double d1, double d2, double D, double *inout);
+/* Precision study (test infrastructure only): non-zero rounds every solve's
+ * input, spectrum and output to single precision; 0 restores FP64. */
+void qpo_set_solve_f32(int on);
 
 /* ---- CIC deposit / gather (src/array_utils.cpp) ----------------------- */
 /* deposit(y,z,v,array2d&) with an origin, :85-118 (also :50-83 for 3D slices). */

@@ -122,6 +122,32 @@ def _check(st, what):
     return st
 
 
+def _f64(a, shape, what, dtype=np.float64):
+    """a as a C-contiguous array of `dtype` with exactly `shape` (an int: that
+    many elements in one dimension, any shape for a tuple match), else
+    ValueError: the C side reads exactly that many values through the pointer."""
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=dtype)
+    want = shape if isinstance(shape, tuple) else (int(shape),)
+    if a.shape != want:
+        raise ValueError(f"{what}: shape {a.shape}, expected {want}")
+    return a
+
+
+def _particles(y, z, py, pz, w, what):
+    """Particle SoA arrays, all of len(y)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    if y.ndim != 1:
+        raise ValueError(f"{what}: y must be one-dimensional, got shape {y.shape}")
+    n = len(y)
+    z = _f64(z, n, f"{what} z")
+    w = _f64(w, n, f"{what} weight")
+    py = _f64(py, n, f"{what} py")
+    pz = _f64(pz, n, f"{what} pz")
+    return y, z, py, pz, w
+
+
 def _ptr(a):
     if a is None:
         return None
@@ -169,20 +195,20 @@ class Fourier2d:
         ctx._adopt(self)
 
     def forward_transform(self, real):
-        real = np.ascontiguousarray(real, dtype=np.float64)
+        real = _f64(real, (self.n1, self.n2), "forward_transform")
         out = np.zeros((self.n1, self.n2 // 2 + 1), dtype=np.complex128)
         _check(load().qp_fourier2d_forward(self.h, _ptr(real), _ptr(out.view(np.float64))), "forward")
         return out
 
     def backward_transform(self, spec):
-        spec = np.ascontiguousarray(spec, dtype=np.complex128)
+        spec = _f64(spec, (self.n1, self.n2 // 2 + 1), "backward_transform", np.complex128)
         out = np.zeros((self.n1, self.n2))
         _check(load().qp_fourier2d_backward(self.h, _ptr(spec.view(np.float64)), _ptr(out)), "backward")
         return out
 
     def solve_poisson_equation(self, src, d1, d2, D=0.0):
         """System_3d::solve_poisson_equation (src/System_3d.cpp:686-706); unnormalised."""
-        src = np.ascontiguousarray(src, dtype=np.float64)
+        src = _f64(src, (self.n1, self.n2), "solve_poisson_equation")
         out = np.zeros_like(src)
         _check(load().qp_solve2d(self.h, d1, d2, D, _ptr(src), _ptr(out)), "qp_solve2d")
         return out
@@ -217,15 +243,13 @@ class Wake:
         ctx._adopt(self)
 
     def set_particles(self, y, z, w, py=None, pz=None):
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (y, z, py, pz, w)]
-        y, z, py, pz, w = arrs
+        y, z, py, pz, w = _particles(y, z, py, pz, w, "Wake.set_particles")
         self.np = len(y)
         _check(load().qp_wake_set_particles(self.h, len(y), _ptr(y), _ptr(z), _ptr(py), _ptr(pz), _ptr(w)),
                "qp_wake_set_particles")
 
     def set_a_sqr(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        assert a.shape == self.shape
+        a = _f64(a, self.shape, "set_a_sqr")
         _check(load().qp_wake_set_a_sqr(self.h, _ptr(a)), "qp_wake_set_a_sqr")
 
     def set_laser_gaussian(self, a0, xsigma, ysigma, zsigma, x0, y0, z0):
@@ -233,8 +257,8 @@ class Wake:
                "qp_wake_set_laser_gaussian")
 
     def set_bunch_sources(self, rho_b=None, jx_b=None):
-        rho_b = None if rho_b is None else np.ascontiguousarray(rho_b, dtype=np.float64)
-        jx_b = None if jx_b is None else np.ascontiguousarray(jx_b, dtype=np.float64)
+        rho_b = _f64(rho_b, self.shape, "set_bunch_sources rho_b")
+        jx_b = _f64(jx_b, self.shape, "set_bunch_sources jx_b")
         _check(load().qp_wake_set_bunch_sources(self.h, _ptr(rho_b), _ptr(jx_b)), "qp_wake_set_bunch_sources")
 
     def run(self, fields=None, callback=None, slices=None):
@@ -352,7 +376,8 @@ class Envelope:
 
     def set(self, a):
         a = np.asarray(a)
-        assert a.shape == self.shape
+        if a.shape != self.shape:
+            raise ValueError(f"Envelope.set: shape {a.shape}, expected {self.shape}")
         re = np.ascontiguousarray(a.real, dtype=np.float64)
         im = np.ascontiguousarray(a.imag, dtype=np.float64) if np.iscomplexobj(a) else None
         _check(load().qp_envelope_set(self.h, _ptr(re), _ptr(im)), "qp_envelope_set")
@@ -367,8 +392,7 @@ class Envelope:
         return re + 1j * im
 
     def set_chi(self, chi):
-        c = None if chi is None else np.ascontiguousarray(chi, dtype=np.float64)
-        assert c is None or c.shape == self.shape
+        c = _f64(chi, self.shape, "Envelope.set_chi")
         _check(load().qp_envelope_set_chi(self.h, _ptr(c)), "qp_envelope_set_chi")
 
     def get_chi(self):
@@ -420,9 +444,9 @@ class Bunch:
 
     def set_particles(self, b):
         """b: dict with x, y, z, px, py, pz, gamma, q, cmr (float) and photon (int) arrays."""
-        a = [np.ascontiguousarray(b[k], dtype=np.float64) for k in self.STATE + ("q", "cmr")]
-        ph = np.ascontiguousarray(b["photon"], dtype=np.int32)
-        n = len(a[0])
+        n = len(np.asarray(b["x"]))
+        a = [_f64(b[k], n, f"Bunch.set_particles {k}") for k in self.STATE + ("q", "cmr")]
+        ph = _f64(b["photon"], n, "Bunch.set_particles photon", np.int32)
         _check(load().qp_bunch_set_particles(self.h, n, *[_ptr(v) for v in a],
                                              ph.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
                "qp_bunch_set_particles")
@@ -481,8 +505,9 @@ class WakeBatch:
         ctx._adopt(self)
 
     def set_particles(self, member, y, z, w, py=None, pz=None):
-        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (y, z, py, pz, w)]
-        y, z, py, pz, w = arrs
+        if not 0 <= member < self.nwakes:
+            raise ValueError(f"WakeBatch.set_particles: member {member} outside [0, {self.nwakes})")
+        y, z, py, pz, w = _particles(y, z, py, pz, w, "WakeBatch.set_particles")
         self.nps[member] = len(y)
         _check(load().qp_wake_set_particles_at(self.h, member, len(y), _ptr(y), _ptr(z), _ptr(py), _ptr(pz),
                                                _ptr(w)), "qp_wake_set_particles_at")

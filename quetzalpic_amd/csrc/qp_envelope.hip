@@ -281,7 +281,10 @@ int envelope_step(Envelope *e, double dt, int m, int use_graph, hipStream_t s) {
         QP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const int st = env_sweep(e, dt, m, s);
         const hipError_t ce = hipStreamEndCapture(s, &gr);
-        if (st < 0) return st;
+        if (st < 0) {
+            if (ce == hipSuccess) (void)hipGraphDestroy(gr);
+            return st;
+        }
         if (ce != hipSuccess) {
             set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
             return QP_ERR_HIP;
@@ -474,7 +477,10 @@ int qp_envelope_advance(qp_envelope h, double dtau, int steps, int chi_iteration
             QP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             const int st = env_sweep(e, dtau, chi_iterations, s);
             const hipError_t ce = hipStreamEndCapture(s, &gr);
-            if (st < 0) return st;
+            if (st < 0) {
+                if (ce == hipSuccess) (void)hipGraphDestroy(gr);
+                return st;
+            }
             if (ce != hipSuccess) {
                 set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
                 return QP_ERR_HIP;

@@ -1448,7 +1448,10 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
                 QP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 int st = enqueue_slice(R, s, i);
                 hipError_t e = hipStreamEndCapture(s, &gr);
-                if (st < 0) return st;
+                if (st < 0) {
+                    if (e == hipSuccess) (void)hipGraphDestroy(gr);
+                    return st;
+                }
                 if (e != hipSuccess) {
                     set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
                     return QP_ERR_HIP;
