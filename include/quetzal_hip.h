@@ -71,6 +71,17 @@ int qp_solve2d(qp_fourier2d f, double d1, double d2, double D, const double *src
 int qp_solve2d_device(qp_fourier2d f, double d1, double d2, double D, const double *src_dev,
                       double *dst_dev);
 
+/* Allocation-free variant of qp_solve2d_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Same solve, enqueued on ctx's
+ * stream; src_dev/dst_dev are device buffers of n1*n2 doubles (may alias).
+ * The half spectrum, twiddle tables and eigenvalue terms live in one fixed
+ * device array per GPU (8.07 MiB, held until the process ends), so n1 and n2
+ * must lie in [1, 1024] (QP_ERR_ARG beyond).  It calls no hipMalloc/hipFree.
+ * Calls that overlap in time on one GPU share that array and race. */
+int qp_solve2d_device_noalloc(qp_ctx ctx, int n1, int n2, double d1, double d2, double D,
+                              const double *src_dev, double *dst_dev);
+
 /* ---- One wake: System_3d::solve_wakefield (src/System_3d.cpp:238-627) ---- */
 typedef struct {
     int nx, ny, nz;          /* slices and transverse nodes, n = (int)(l/d) (src/System_3d.cpp:87-89) */

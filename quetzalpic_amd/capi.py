@@ -69,6 +69,8 @@ def load():
         "qp_fourier2d_backward": ([vp, _dp, _dp], ctypes.c_int),
         "qp_solve2d": ([vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp], ctypes.c_int),
         "qp_solve2d_device": ([vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp], ctypes.c_int),
+        "qp_solve2d_device_noalloc": ([vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 3 + [vp, vp],
+                                      ctypes.c_int),
         "qp_wake_create": ([vp, ctypes.POINTER(WakeParams), ctypes.POINTER(vp)], ctypes.c_int),
         "qp_wake_destroy": ([vp], ctypes.c_int),
         "qp_wake_set_particles": ([vp, ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),

@@ -43,6 +43,9 @@ int qp_ctx_create(int device, qp_ctx *out) {
     QP_HIP_CHECK(hipGetDeviceCount(&n));
     ARG_CHECK(device >= 0 && device < n, "qp_ctx_create: no such HIP device");
     QP_HIP_CHECK(hipSetDevice(device));
+    // once per device, with it current: the address of the allocation-free
+    // solve's scratch array on this GPU
+    QP_TRY(noalloc_setup(device));
     auto *c = new (std::nothrow) qp_ctx_s();
     if (!c) return QP_ERR_ALLOC;
     c->device = device;

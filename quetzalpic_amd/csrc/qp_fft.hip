@@ -23,7 +23,7 @@
 
 namespace qp {
 
-int FftAxis::init(int n) {
+int FftAxis::plan(int n) {
     if (n < 1 || n > kMaxN) {
         set_error("transform length " + std::to_string(n) + " outside [1, " + std::to_string(kMaxN) + "]");
         return QP_ERR_ARG;
@@ -42,6 +42,12 @@ int FftAxis::init(int n) {
         while (m % f == 0) { if (!push(f)) return QP_ERR_ARG; m /= f; }
         if ((long)f * f > m && m > 1) { if (!push(m)) return QP_ERR_ARG; m = 1; }
     }
+    dev.tw = nullptr;
+    return QP_OK;
+}
+
+int FftAxis::init(int n) {
+    QP_TRY(plan(n));
     std::vector<double2> tw(n);
     const long double two_pi = 6.283185307179586476925286766559005768L;
     for (int t = 0; t < n; ++t) {
@@ -82,6 +88,12 @@ int Solver2d::init(int n1_, int n2_, double d1_, double d2_, int nbatch_) {
     QP_HIP_CHECK(hipMalloc(&d_lam2, sizeof(double) * nh));
     QP_TRY(set_steps(d1_, d2_));
     for (int i = 0; i < 2; ++i) QP_HIP_CHECK(hipMalloc(&spec[i], sizeof(double2) * (size_t)n1 * nh * nbatch));
+    default_shape();
+    return QP_OK;
+}
+
+// Kernel family and launch shape of an n1 x n2 solver (no device work).
+void Solver2d::default_shape() {
     // QP_FFT=generic keeps the generic kernels for every size (A/B comparisons).
     const char *sel = getenv("QP_FFT");
     wave = (wf_supported(n1) && wf_supported(n2) && !(sel && strcmp(sel, "generic") == 0)) ? 1 : 0;
@@ -94,7 +106,6 @@ int Solver2d::init(int n1_, int n2_, double d1_, double d2_, int nbatch_) {
     if (c < 1) c = 1;
     if (c > kLdsComplex / n1) c = kLdsComplex / n1;
     cols_per_block = c < 1 ? 1 : c;
-    return QP_OK;
 }
 
 // Pick rows_per_block / cols_per_block for this grid by timing the row and

@@ -43,7 +43,8 @@ struct FftAxisDev {
 struct FftAxis {
     FftAxisDev dev{};
     double2 *d_tw = nullptr;
-    int init(int n);
+    int plan(int n);   // radix plan only (host); dev.tw left null
+    int init(int n);   // plan + twiddle table in device memory
     void destroy();
 };
 
@@ -63,6 +64,7 @@ struct Solver2d {
     // spec[j * nh + l] (FFTW's r2c layout).  Fourier2d always uses the latter.
     int wave = 0;
     int init(int n1, int n2, double d1, double d2, int nbatch = 1);
+    void default_shape();  // wave / rows_per_block / cols_per_block from n1, n2
     int set_steps(double d1, double d2);
     int autotune(hipStream_t s, double *scratch_real);
     void destroy();
@@ -226,6 +228,12 @@ int solve_rows_fwd_plain_ctl(const Solver2d &S, const double *src, double2 *spec
 int solve_rows_inv_plain_ctl(const Solver2d &S, const double2 *spec, double *dst, hipStream_t s,
                              const Ctl *ctl);
 
+}  // namespace qp
+
+namespace qp {
+// Per-GPU set-up of the allocation-free solve (qp_noalloc.hip): resolves the
+// device address of its file-scope scratch array on `device` (current).
+int noalloc_setup(int device);
 }  // namespace qp
 
 struct qp_ctx_s {
