@@ -33,22 +33,22 @@ def log(*a):
 
 def bytes_per_particle(cls, K):
     """Algorithmic HBM bytes per particle and launch of a particle phase in the
-    build's own layout (persistent state S = 32 B: u32 cell, 2 x u32 in-cell
-    offsets, 2 x f64 momenta, f32 weight; scratch T = 8 B: 2 x f32 increments
-    p* - p):
-      A: read S + T, write cell, offsets, momenta (28 B)      = 68
-      B: read S (+ T after the first iteration), write T      = 40 | 48
-      C: read S + T                                           = 40
-      S: read S                                               = 32
-    500 B per particle-slice at K = 5."""
+    build's own layout (persistent state S = 36 B: u32 cell, 2 x u32 in-cell
+    offsets, 2 x f64 momenta, f64 weight; scratch T = 16 B: 2 x f64 predicted
+    momenta p*):
+      A: read cell, offsets, weight, p* (36), write cell, offsets, momenta (28) = 64
+      B: read S (+ T after the first iteration), write T                      = 52 | 68
+      C: read S + T                                                           = 52
+      S: read S                                                               = 36
+    648 B per particle-slice at K = 5."""
     if cls == "push_A":
-        return 68.0
+        return 64.0
     if cls == "push_B":
-        return (40.0 + 48.0 * (K - 1)) / K if K > 0 else 40.0
+        return (52.0 + 68.0 * (K - 1)) / K if K > 0 else 52.0
     if cls == "push_C":
-        return 40.0
+        return 52.0
     if cls == "push_S":
-        return 32.0
+        return 36.0
     raise KeyError(cls)
 
 
@@ -354,6 +354,7 @@ def bench_c5(args, capi, workloads):
             err_cpu = {f: err_pair(mixed_lead[f][:k], oref[f][::3][:k]) for f in ("psi", "ex")}
         log(f"[bench] cpu baseline: {cpu['value']:.3e} pushes/s on {cpu['cores']} threads ({cpu['cpu_model']}); "
             f"err vs oracle {err_cpu}")
+    windows = None if args.no_profile else window_report(capi, ctx, wl, bunch=db, bunch_particles=b0)
     db.close()
     extra = {} if args.no_extra else bench_extras(args, capi, workloads, ctx)
     ctx.close()
@@ -362,7 +363,7 @@ def bench_c5(args, capi, workloads):
         "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * t / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "storage": "plasma particles: u32 cell + u32x2 offsets, f64x2 momenta, f32 weight, f32x2 p* increments; "
+        "storage": "plasma particles: u32 cell + u32x2 offsets, f64x2 momenta, f64 weight, f64x2 predicted momenta; "
                    "grids and solves f64; bunch f64 SoA",
         "config": {**wl.describe(), "bunch_particles": nb,
                    "step": "one beam time step: mixed wake pass with the bunch streamed through it + bunch push"},
@@ -373,6 +374,7 @@ def bench_c5(args, capi, workloads):
         "fp64_gpu_pushes_per_s_leading": fp64_rate,
         "err_vs_fp64_gpu_leading": err64, "err_vs_oracle_sample": err_cpu,
         "cpu_baseline": cpu, "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "deposit_windows": windows,
         "other_workloads": extra,
         "note": "3D arrays of the reference's layout (281 GiB at this size) are never formed: the bunch "
                 "sources and field gathers are streamed slice by slice",
@@ -399,9 +401,14 @@ def bench_extras(args, capi, workloads, ctx):
         ms, rate = pass_rate(capi, ctx, wl, steps)
         out[name] = {"ms_per_pass": ms, "pushes_per_s": rate, "particles": wl.n_particles, "slices": wl.nx}
         log(f"[bench] {name}: {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
+        if name == "C3" and not args.no_profile:
+            out[name]["deposit_windows"] = window_report(capi, ctx, wl)
     ms, rate, desc = c4_pass(capi, workloads, 2)
     out["C4"] = {"ms_per_pass": ms, "pushes_per_s": rate, **desc}
-    log(f"[bench] C4: {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
+    log(f"[bench] C4 (one group): {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
+    ms, rate, desc = c4_pass(capi, workloads, 2, mode="streams")
+    out["C4_streams"] = {"ms_per_pass": ms, "pushes_per_s": rate, **desc}
+    log(f"[bench] C4 (batches on streams): {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
     return out
 
 
@@ -422,22 +429,48 @@ def calib_copy(mib):
     hip.hipFree(b)
 
 
-def make_wake(capi, ctx, wl, precision, use_graph=1, async_output=0, slice_limit=0):
+def make_wake(capi, ctx, wl, precision, use_graph=1, async_output=0, slice_limit=0, resort_every=0):
     w = capi.Wake(ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, wl.L, wl.L, K=wl.K, precision=precision,
-                  use_graph=use_graph, resort_every=0, async_output=async_output, slice_limit=slice_limit)
+                  use_graph=use_graph, resort_every=resort_every, async_output=async_output, slice_limit=slice_limit)
     y, z, wt, py, pz = wl.particles()
     w.set_particles(y, z, wt, py, pz)
     w.set_laser_gaussian(*wl.laser)
     return w
 
 
+def window_report(capi, ctx, wl, resort_values=(0, 50), bunch=None, bunch_particles=None):
+    """Deposit-window overflow (blocks that fall back to global atomics) over one
+    whole pass, with the particles left in their initial cell order (0) and
+    re-binned by tile every N slices; with the pass time."""
+    out = []
+    for r in resort_values:
+        wk = make_wake(capi, ctx, wl, capi.QP_PREC_MIXED, resort_every=r)
+        if bunch is not None:
+            bunch.set_particles(bunch_particles)
+            wk.attach_bunch(bunch)
+        wk.run()
+        ovf, tot = wk.window_stats()
+        ms = wk.last_time_ms()
+        wk.close()
+        out.append({"resort_every": r, "overflow_blocks": ovf, "deposit_blocks": tot,
+                    "overflow_fraction": ovf / tot if tot else 0.0, "pass_ms": ms})
+        log(f"[bench] {wl.name[:2]} resort_every={r}: {ovf} of {tot} deposit blocks overflow the LDS window "
+            f"({100.0 * ovf / max(tot, 1):.3f} %), pass {ms:.1f} ms")
+    return out
+
+
 class C4Job:
     """C4: 64 independent wakes.  The members of one grid size form a batch
-    (qp_wake_create_batch: every launch of a batch carries all its members); the
-    batches run concurrently, each on its own context (stream) from its own host
-    thread."""
+    (qp_wake_create_batch: every launch of a batch carries all its members).
+    mode "group": the five batches form one group (qp_group_create) on one
+    context, so every phase of the slice step is one launch over all 64 wakes;
+    mode "streams": the batches run concurrently, each on its own context
+    (stream) from its own host thread."""
 
-    def __init__(self, capi, workloads, use_graph=1, only_class=0):
+    def __init__(self, capi, workloads, use_graph=1, only_class=0, mode="group"):
+        self.mode = mode
+        self.group = None
+        shared = capi.Context(0) if mode == "group" else None
         members = workloads.c4_members()
         if only_class:  # one grid-size class only (profiling runs)
             members = [m for m in members if m.ny == only_class]
@@ -450,7 +483,7 @@ class C4Job:
         for n in self.sizes:
             ms = self.classes[n]
             m0 = ms[0]
-            ctx = capi.Context(0)
+            ctx = shared if shared is not None else capi.Context(0)
             b = capi.WakeBatch(ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K,
                                use_graph=use_graph)
             for i, m in enumerate(ms):
@@ -460,12 +493,18 @@ class C4Job:
             self.batches[n] = (ctx, b, sum(m.n_particles for m in ms))
             log(f"[bench] C4 class {n}x{n}: {len(ms)} members, {self.batches[n][2]} particles")
         self.pushes = sum(m.n_particles * (m.nx - 1) for m in members)
+        if mode == "group":
+            self.group = capi.WakeGroup(shared, [self.batches[n][1] for n in self.sizes])
 
     def describe(self):
         return {"members": {f"{n}x{n}": len(self.classes[n]) for n in self.sizes},
-                "particles": sum(m.n_particles for m in self.members), "slices": self.members[0].nx}
+                "particles": sum(m.n_particles for m in self.members), "slices": self.members[0].nx,
+                "mode": self.mode}
 
     def run(self):
+        if self.group is not None:
+            self.group.run()
+            return
         import threading
         errs = []
 
@@ -488,13 +527,19 @@ class C4Job:
             ctx.synchronize()
 
     def close(self):
+        if self.group is not None:
+            self.group.close()
+        ctxs = []
         for (ctx, b, _) in self.batches.values():
             b.close()
+            if ctx not in ctxs:
+                ctxs.append(ctx)
+        for ctx in ctxs:
             ctx.close()
 
 
-def c4_pass(capi, workloads, steps, warmup=1):
-    job = C4Job(capi, workloads)
+def c4_pass(capi, workloads, steps, warmup=1, mode="group"):
+    job = C4Job(capi, workloads, mode=mode)
     for _ in range(warmup):
         job.run()
     job.synchronize()
@@ -511,7 +556,7 @@ def c4_pass(capi, workloads, steps, warmup=1):
 
 def bench_c4(args, capi, workloads):
     """value = all members' particle-slice pushes of a pass / wall time."""
-    job = C4Job(capi, workloads, use_graph=0 if args.no_graph else 1, only_class=args.c4_class)
+    job = C4Job(capi, workloads, use_graph=0 if args.no_graph else 1, only_class=args.c4_class, mode=args.c4_mode)
     for _ in range(args.warmup):
         job.run()
     job.synchronize()
@@ -649,6 +694,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel-class event passes")
     ap.add_argument("--slice-limit", type=int, default=0,
                     help="run only the first N slices of every pass (profiling runs; value counts those)")
+    ap.add_argument("--c4-mode", default="group", choices=("group", "streams"),
+                    help="C4: one launch per phase over all wakes (group) or one batch per stream")
     ap.add_argument("--c4-class", type=int, default=0,
                     help="C4 only: run the members of this grid size alone (profiling runs)")
     ap.add_argument("--envelope-only", action="store_true",

@@ -34,7 +34,7 @@ extern "C" {
 /* Arithmetic of the slice loop. */
 #define QP_PREC_FP64 0   /* plain path: FP64 SoA particles, one kernel per reference loop */
 #define QP_PREC_MIXED 1  /* fused path: u32 cell + u32 fixed-point in-cell offsets, FP64
-                            momenta, FP32 weights and FP32 predictor increments p* - p;
+                            momenta, weights and predicted momenta p*;
                             FP64 arithmetic, grids and solves; hipGraph slice step */
 
 typedef struct qp_ctx_s *qp_ctx;
@@ -154,8 +154,23 @@ int qp_wake_last_time_ms(qp_wake w, double *ms);
  * device duration in microseconds.  Classes: "push" (particle phases),
  * "solve" (all transform launches), "deposit_push_B" etc. (see docs). */
 int qp_wake_profile_kernel(qp_wake w, const char *kernel_class, int *launches, double *mean_us);
+/* Mixed path, last pass, one member (member 0 for a single wake): particle
+ * blocks of the deposit launches (A, B, C, S phases) and those whose deposit
+ * window exceeded the block's LDS capacity and deposited with global atomics
+ * instead.  Particles are stored cell-sorted at the start of every pass;
+ * resort_every re-bins them during it.  No reference counterpart. */
+int qp_wake_window_stats(qp_wake w, int member, long long *overflow_blocks, long long *deposit_blocks);
+/* Particle tile of the mixed path: threads (= particles) per block of the
+ * push/deposit launches and LDS window nodes per deposited grid.  By default
+ * the tile autotuner picks one per wake on its first pass (the first slices
+ * with each instantiated tile, fastest kept; cached per grid and particle
+ * count; QP_TILE=pt,cap in the environment forces one); this call forces a
+ * tile, and (0, 0) lets the tuner pick again on the next pass.  The list of
+ * instantiated tiles is in qp_wake_describe.  No reference counterpart. */
+int qp_wake_set_tile(qp_wake w, int threads_per_block, int window_nodes);
 /* JSON description of the launch shapes the library picked for this wake
- * (FFT rows/columns per block from the autotuner). */
+ * (particle tile and the list of instantiated tiles; FFT rows/columns per
+ * block of the generic kernels). */
 int qp_wake_describe(qp_wake w, char *buf, size_t len);
 /* Copy the current particle state back (FP64 SoA; any pointer may be NULL). */
 int qp_wake_get_particles(qp_wake w, size_t np, double *y, double *z, double *py, double *pz);
@@ -181,6 +196,33 @@ typedef int (*qp_batch_cb)(int member, const qp_slice_view *view, void *user);
 int qp_wake_run_batch(qp_wake w, qp_batch_cb cb, void *user, int *stop_slices);
 int qp_wake_get_particles_at(qp_wake w, int member, size_t np, double *y, double *z, double *py,
                              double *pz);
+
+/* ---- Groups of batches with uneven grids (no reference counterpart) --------
+ * A group advances several batches (each from qp_wake_create_batch, each with
+ * its own transverse grid and members) in lockstep: every phase of the slice
+ * step -- commit/advance, predictor, current deposits, and each pass of every
+ * periodic solve -- is one launch over all members of all batches, and the
+ * generic slice step is captured in a hipGraph.  Members give the same result
+ * as their batch run alone.  The batches must be mixed-precision handles of
+ * the group's context with equal nx, K and slice_limit, Gaussian lasers only
+ * (no envelope, bunch, 3D a_sqr or bunch sources), no async_output,
+ * resort_every or longitudinal_b, and grid sides 2^p in [16, 256] or 48, 96,
+ * 192.  The group borrows the batch handles: destroy it before them.  One
+ * group per host thread. */
+typedef struct qp_group_s *qp_group;
+int qp_group_create(qp_ctx ctx, int nbatches, const qp_wake *batches, qp_group *out);
+int qp_group_destroy(qp_group g);
+/* Per-slice callback of a group: batch index, member index, then the view. */
+typedef int (*qp_group_cb)(int batch, int member, const qp_slice_view *view, void *user);
+/* One pass of every member of every batch; stop_slices (sum of the batch sizes
+ * entries, batch by batch; may be NULL) gets each member's stop slice or -1.
+ * Returns QP_STOPPED if any member stopped. */
+int qp_group_run(qp_group g, qp_group_cb cb, void *user, int *stop_slices);
+/* Device time of the last qp_group_run in milliseconds. */
+int qp_group_last_time_ms(qp_group g, double *ms);
+/* Particle tile of every launch of the group (as qp_wake_set_tile; by default
+ * the group's first run tunes one for all its batches together). */
+int qp_group_set_tile(qp_group g, int threads_per_block, int window_nodes);
 
 /* ---- Laser envelope (no reference counterpart) ------------------------------
  * The reference keeps <a^2> fixed for the whole run (src/System_3d.cpp:805-816

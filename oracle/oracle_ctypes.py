@@ -70,6 +70,7 @@ def load_path(variant, path):
     lib = ctypes.CDLL(path)
     lib.qpo_fft.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
     lib.qpo_set_solve_f32.argtypes = [ctypes.c_int]
+    lib.qpo_set_pstar_f32.argtypes = [ctypes.c_int]
     lib.qpo_rfft2.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
     lib.qpo_irfft2.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
     lib.qpo_solve_poisson.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

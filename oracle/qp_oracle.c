@@ -310,6 +310,10 @@ void qpo_irfft2(int n1, int n2, const double *in, double *out) {
  * or a split-BF16 dense product accumulated in FP32). */
 static int g_solve_f32 = 0;
 void qpo_set_solve_f32(int on) { g_solve_f32 = on; }
+/* Precision study only: with a non-zero value the predictor's increment
+ * p* - p is rounded to single precision (the mixed path's stored increment). */
+static int g_pstar_f32 = 0;
+void qpo_set_pstar_f32(int on) { g_pstar_f32 = on; }
 static void round_f32(double *a, size_t n) {
     for (size_t i = 0; i < n; ++i) a[i] = (double)(float)a[i];
 }
@@ -726,6 +730,10 @@ static int wake_run_impl(const qpo_grid *g, size_t np_in, const double *y0, cons
                 q->pz_next = q->pz - dx * 0.5 * daz / (1 + ps);
                 q->pz_next += dx * q->gamma * dpz / (1 + ps);
                 q->pz_next += dx * byp;
+                if (g_pstar_f32) {  /* precision study: the increment p* - p rounded to single */
+                    q->py_next = q->py + (double)(float)(q->py_next - q->py);
+                    q->pz_next = q->pz + (double)(float)(q->pz_next - q->pz);
+                }
             }
             /* P2 advance half coordinate (:435-443) */
 #pragma omp parallel for

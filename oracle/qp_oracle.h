@@ -74,6 +74,9 @@ void qpo_solve_poisson(int n1, int n2, double d1, double d2, double D, double *i
 /* Precision study (test infrastructure only): non-zero rounds every solve's
  * input, spectrum and output to single precision; 0 restores FP64. */
 void qpo_set_solve_f32(int on);
+/* Precision study (test infrastructure only): non-zero rounds the predicted
+ * momentum increment p* - p to single precision; 0 restores FP64. */
+void qpo_set_pstar_f32(int on);
 
 /* ---- CIC deposit / gather (src/array_utils.cpp) ----------------------- */
 /* deposit(y,z,v,array2d&) with an origin, :85-118 (also :50-83 for 3D slices). */

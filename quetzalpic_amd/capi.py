@@ -44,6 +44,7 @@ class SliceView(ctypes.Structure):
 
 SLICE_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(SliceView), ctypes.c_void_p)
 BATCH_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.POINTER(SliceView), ctypes.c_void_p)
+GROUP_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(SliceView), ctypes.c_void_p)
 
 _lib = None
 
@@ -82,12 +83,18 @@ def load():
         "qp_wake_profile_kernel": ([vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
         "qp_wake_get_particles": ([vp, ctypes.c_size_t, _dp, _dp, _dp, _dp], ctypes.c_int),
         "qp_wake_describe": ([vp, ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
+        "qp_wake_window_stats": ([vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
         "qp_wake_create_batch": ([vp, ctypes.POINTER(WakeParams), ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
         "qp_wake_batch_size": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "qp_wake_set_particles_at": ([vp, ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
         "qp_wake_set_laser_gaussian_at": ([vp, ctypes.c_int] + [ctypes.c_double] * 7, ctypes.c_int),
         "qp_wake_run_batch": ([vp, BATCH_CB, vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "qp_wake_get_particles_at": ([vp, ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "qp_group_create": ([vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp)], ctypes.c_int),
+        "qp_group_destroy": ([vp], ctypes.c_int),
+        "qp_group_run": ([vp, GROUP_CB, vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "qp_group_last_time_ms": ([vp, _dp], ctypes.c_int),
         "qp_envelope_create": ([vp] + [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.POINTER(vp)],
                                ctypes.c_int),
         "qp_envelope_destroy": ([vp], ctypes.c_int),
@@ -315,6 +322,12 @@ class Wake:
         buf = ctypes.create_string_buffer(512)
         _check(load().qp_wake_describe(self.h, buf, 512), "qp_wake_describe")
         return buf.value.decode()
+
+    def window_stats(self, member=0):
+        """(overflow_blocks, deposit_blocks) of the last mixed pass."""
+        o, t = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(load().qp_wake_window_stats(self.h, member, ctypes.byref(o), ctypes.byref(t)), "qp_wake_window_stats")
+        return o.value, t.value
 
     def get_particles(self):
         y, z, py, pz = (np.zeros(self.np) for _ in range(4))
@@ -577,6 +590,74 @@ class WakeBatch:
     def close(self):
         if self.h:
             load().qp_wake_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WakeGroup:
+    """Several WakeBatch objects (each its own transverse grid) advanced in
+    lockstep, one launch per phase of the slice step over all of them
+    (qp_group_create).  The group borrows the batches: close it first."""
+
+    def __init__(self, ctx, batches):
+        self.ctx = ctx
+        self.batches = list(batches)
+        arr = (ctypes.c_void_p * len(self.batches))(*[b.h for b in self.batches])
+        self.h = ctypes.c_void_p()
+        _check(load().qp_group_create(ctx.h, len(self.batches), arr, ctypes.byref(self.h)), "qp_group_create")
+        ctx._adopt(self)
+
+    def run(self, fields=None, callback=None, members=None):
+        """One pass of every member of every batch.  Returns (outs, status, stops):
+        outs[b][m] is a field dict as WakeBatch.run (empty unless (b, m) is in
+        `members`, default all), stops[b][m] the stop slice or -1."""
+        keep = None if members is None else set(members)
+        outs = []
+        for bi, b in enumerate(self.batches):
+            outs.append([{k: np.zeros(b.shape) for k in fields} if fields and (keep is None or (bi, m) in keep)
+                         else {} for m in range(b.nwakes)])
+        ntot = sum(b.nwakes for b in self.batches)
+        stops = (ctypes.c_int * ntot)()
+        if fields or callback:
+            def _cb(bi, member, vp, _user):
+                v = vp.contents
+                out = outs[bi][member]
+                ny, nz = self.batches[bi].shape[1:]
+                for k in out:
+                    if v.kind == QP_SLICE_STOPPED and k not in ("by", "bz", "chi"):
+                        continue
+                    ptr = getattr(v, k)
+                    if not ptr:
+                        continue
+                    out[k][v.slice] = np.ctypeslib.as_array(ptr, shape=(ny * nz,)).reshape(ny, nz)
+                if callback:
+                    return int(callback(bi, member, v) or 0)
+                return 0
+
+            cb = GROUP_CB(_cb)
+        else:
+            cb = GROUP_CB()
+        st = _check(load().qp_group_run(self.h, cb, None, stops), "qp_group_run")
+        flat = list(stops)
+        res, off = [], 0
+        for b in self.batches:
+            res.append(flat[off:off + b.nwakes])
+            off += b.nwakes
+        return outs, st, res
+
+    def last_time_ms(self):
+        ms = ctypes.c_double()
+        _check(load().qp_group_last_time_ms(self.h, ctypes.byref(ms)), "qp_group_last_time_ms")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            load().qp_group_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

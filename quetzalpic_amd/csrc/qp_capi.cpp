@@ -400,6 +400,67 @@ int qp_wake_run_batch(qp_wake h, qp_batch_cb cb, void *user, int *stop_slices) {
     return wake_run_mixed(w, cb, user, stop_slices);
 }
 
+// ------------------------------------------------------------- groups
+int qp_group_create(qp_ctx ctx, int nbatches, const qp_wake *batches, qp_group *out) {
+    ARG_CHECK(ctx && batches && out, "qp_group_create: null argument");
+    ARG_CHECK(nbatches >= 1 && nbatches <= 16, "qp_group_create: 1 to 16 batches");
+    for (int b = 0; b < nbatches; ++b) {
+        ARG_CHECK(batches[b], "qp_group_create: null batch");
+        ARG_CHECK(batches[b]->w.ctx == ctx, "qp_group_create: every batch must be on the group's context");
+        ARG_CHECK(batches[b]->w.p.precision == QP_PREC_MIXED, "qp_group_create: mixed-precision batches only");
+        for (int c = 0; c < b; ++c) ARG_CHECK(batches[c] != batches[b], "qp_group_create: a batch appears twice");
+    }
+    QP_HIP_CHECK(hipSetDevice(ctx->device));
+    auto *h = new (std::nothrow) qp_group_s();
+    if (!h) return QP_ERR_ALLOC;
+    Group &g = h->g;
+    g.ctx = ctx;
+    for (int b = 0; b < nbatches; ++b) g.b.push_back(&batches[b]->w);
+    if (hipEventCreate(&g.ev0) != hipSuccess || hipEventCreate(&g.ev1) != hipSuccess) {
+        if (g.ev0) (void)hipEventDestroy(g.ev0);
+        if (g.ev1) (void)hipEventDestroy(g.ev1);
+        delete h;
+        set_error("qp_group_create: hipEventCreate failed");
+        return QP_ERR_HIP;
+    }
+    *out = h;
+    return QP_OK;
+}
+
+int qp_group_destroy(qp_group h) {
+    if (!h) return QP_OK;
+    (void)hipStreamSynchronize(h->g.ctx->stream);
+    group_destroy_mixed(&h->g);
+    (void)hipEventDestroy(h->g.ev0);
+    (void)hipEventDestroy(h->g.ev1);
+    delete h;
+    return QP_OK;
+}
+
+int qp_group_run(qp_group h, qp_group_cb cb, void *user, int *stop_slices) {
+    ARG_CHECK(h, "null group");
+    for (Wake *w : h->g.b) ARG_CHECK(all_particles_set(w), "qp_group_run: particles not set for every member");
+    QP_HIP_CHECK(hipSetDevice(h->g.ctx->device));
+    return group_run_mixed(&h->g, cb, user, stop_slices);
+}
+
+int qp_group_last_time_ms(qp_group h, double *ms) {
+    ARG_CHECK(h && ms, "null argument");
+    *ms = h->g.last_ms;
+    return QP_OK;
+}
+
+int qp_wake_window_stats(qp_wake h, int member, long long *overflow_blocks, long long *deposit_blocks) {
+    ARG_CHECK(h && overflow_blocks && deposit_blocks, "null argument");
+    Wake *w = &h->w;
+    ARG_CHECK(w->p.precision == QP_PREC_MIXED, "qp_wake_window_stats: mixed-precision handles only");
+    ARG_CHECK(member >= 0 && member < w->nw, "qp_wake_window_stats: member out of range");
+    ARG_CHECK((int)w->win_blocks.size() == w->nw, "qp_wake_window_stats: no pass has run");
+    *overflow_blocks = w->win_ovf[member];
+    *deposit_blocks = w->win_blocks[member];
+    return QP_OK;
+}
+
 int qp_wake_batch_size(qp_wake h, int *nwakes) {
     ARG_CHECK(h && nwakes, "null argument");
     *nwakes = h->w.nw;
@@ -438,10 +499,43 @@ int qp_wake_profile_kernel(qp_wake h, const char *kernel_class, int *launches, d
     return QP_OK;
 }
 
+int qp_wake_set_tile(qp_wake h, int threads_per_block, int window_nodes) {
+    ARG_CHECK(h, "null wake");
+    Wake *w = &h->w;
+    ARG_CHECK(w->p.precision == QP_PREC_MIXED, "qp_wake_set_tile: mixed-precision handles only");
+    if (threads_per_block == 0 && window_nodes == 0) {
+        w->tile_tuned = false;   // the autotuner picks again on the next pass
+        return QP_OK;
+    }
+    ARG_CHECK(mixed_tile_valid(threads_per_block, window_nodes),
+              "qp_wake_set_tile: tile not instantiated (see qp_wake_describe for the list)");
+    w->tile_pt = threads_per_block;
+    w->tile_cap = window_nodes;
+    w->tile_tuned = true;
+    return QP_OK;
+}
+
+int qp_group_set_tile(qp_group h, int threads_per_block, int window_nodes) {
+    ARG_CHECK(h, "null group");
+    Group *g = &h->g;
+    if (threads_per_block == 0 && window_nodes == 0) {
+        g->tile_tuned = false;
+        return QP_OK;
+    }
+    ARG_CHECK(mixed_tile_valid(threads_per_block, window_nodes),
+              "qp_group_set_tile: tile not instantiated (see qp_wake_describe for the list)");
+    g->tile_pt = threads_per_block;
+    g->tile_cap = window_nodes;
+    g->tile_tuned = true;
+    return QP_OK;
+}
+
 int qp_wake_describe(qp_wake h, char *buf, size_t len) {
     ARG_CHECK(h && buf && len > 0, "null argument");
     const Wake *w = &h->w;
     std::string s = "{\"members\": " + std::to_string(w->nw) +
+                    ", \"tile\": [" + std::to_string(w->tile_pt) + ", " + std::to_string(w->tile_cap) + "]" +
+                    ", \"tile_tuned\": " + (w->tile_tuned ? "1" : "0") + ", \"tiles\": " + mixed_tiles_json() +
                     ", \"rows_per_block\": " + std::to_string(w->S.rows_per_block) +
                     ", \"cols_per_block\": " + std::to_string(w->S.cols_per_block) +
                     ", \"precision\": " + std::to_string(w->p.precision) +

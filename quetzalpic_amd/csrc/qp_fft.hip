@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -112,7 +113,11 @@ void Solver2d::default_shape() {
 // column passes (20 launches each) on scratch data; cached per (n1, n2).
 int Solver2d::autotune(hipStream_t s, double *scratch_real) {
     if (wave) return QP_OK;  // the register-resident kernels have a fixed shape
+    // shared by every solver of the process; wakes may be created from several
+    // host threads, so lookups and the tuning that fills an entry hold the lock
+    static std::mutex mu;
     static std::map<std::pair<int, int>, std::pair<int, int>> cache;
+    std::lock_guard<std::mutex> lk(mu);
     auto key = std::make_pair(n1, n2);
     auto it = cache.find(key);
     if (it != cache.end()) {

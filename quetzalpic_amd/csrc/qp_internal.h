@@ -82,7 +82,8 @@ struct Ctl {
     int stop_now;
     int stop_slice;
     int slice_of[2];   // mixed path: slice index for the even / odd slice step
-    int pad[3];
+    unsigned ovf;      // mixed path: particle blocks whose deposit window exceeded the LDS capacity
+    int pad[2];
     double border[4];  // plain path: border mean of the last psi solve
 };
 

@@ -1,17 +1,22 @@
 // qp_mixed_kernels.h -- device code of the fused mixed-precision slice step.
 //
-// Particle layout (structure of arrays, 36 B persistent + 8 B scratch):
+// Particle layout (structure of arrays, 36 B persistent + 16 B scratch):
 //   cell  uint32   (j << 16) | k          node-origin cell of the particle
 //   off   uint2    in-cell offsets (y, z) in units of 2^-32 cell (Y = j + off.x 2^-32)
 //   p     double2  transverse momenta (y, z) (they accumulate over every slice)
-//   w     float    macro-particle weight (wake_particle_2d::n)
-//   dp    float2   increment p* - p of the current iteration's predicted
-//                  momenta (scratch): its rounding is relative to the small
-//                  increment, not to p
-// Arithmetic is FP64 throughout (FP64 is cheap next to the memory traffic);
-// only the stored state is narrowed.  Positions are advanced in FP64 cell
-// units and re-split with round-to-nearest into cell + offset, so the
-// stored position keeps ~2^-32 cell resolution over thousands of slices.
+//   w     double   macro-particle weight (wake_particle_2d::n); in single precision
+//                  its rounding alone (up to 6e-8 relative for a density ramp) moved
+//                  e_x of a 256^2 x 800-slice wake at a0 = 2.3 by 1.1e-5 of its
+//                  maximum against the FP64 oracle
+//   ps    double2  the current iteration's predicted momenta p* (scratch).  As
+//                  an FP32 increment p* - p its rounding alone moved e_x of a
+//                  256^2 x 800-slice wake at a0 = 2.3 by 1.1e-5 of its maximum
+//                  against the FP64 oracle (the rounding enters every commit)
+// Arithmetic is FP64 throughout (without packed FP32, FP32 and FP64 add, mul
+// and FMA issue at the same rate on gfx950); only the position is narrowed.
+// Positions are advanced in FP64 cell units and re-split with round-to-nearest
+// into cell + offset, so the stored position keeps ~2^-32 cell resolution
+// over thousands of slices.
 //
 // The 35 particle sweeps of a reference slice (src/System_3d.cpp:328-619)
 // become 2K+1 launches: A (commit + advance + psi source deposit), then per
@@ -25,8 +30,13 @@
 
 namespace qp {
 
-constexpr int kPT = 256;          // threads (= particles) per block
-constexpr int kWinCap = 640;      // LDS window nodes per deposited grid
+// Particle tiles: PT threads (= particles) per block and CAP LDS window nodes
+// per deposited grid are template parameters of the particle kernels, picked
+// per wake by the tile autotuner among the instantiated pairs (qp_wake_mixed.hip);
+// kPT / kWinCap are the defaults, and kPT is the block of the re-binning and
+// rho_ion kernels.
+constexpr int kPT = 256;
+constexpr int kWinCap = 640;
 constexpr double kTwo32 = 4294967296.0;
 constexpr double kInvTwo32 = 1.0 / 4294967296.0;
 
@@ -45,8 +55,8 @@ struct MxPart {
     uint32_t *cell;
     uint2 *off;    // in-cell offsets (y, z) in units of 2^-32 of a cell
     double2 *p;    // p_y, p_z
-    float *w;
-    float2 *dp;    // p* - p (y, z)
+    double *w;
+    double2 *ps;   // p* (y, z)
     int np;
     int pstride;
     const int *npw;
@@ -243,7 +253,7 @@ __device__ __forceinline__ ValGrad mx_gather_vg(const double *a, const Pos &q, i
 // Each block deposits into an LDS window covering the nodes its particles
 // touch, then flushes the window with one global atomic per non-zero node.
 // The window is found from the block's own particles (periodic-aware, relative
-// to the first particle's cell).  If it exceeds kWinCap nodes the block falls
+// to the first particle's cell).  If it exceeds CAP nodes the block falls
 // back to direct global atomics.
 struct Win {
     int j0, k0, wj, wk;
@@ -301,6 +311,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // they lie in [-n/2, n - n/2].
 // `sh` is 4*(blockDim/64)+2 ints of LDS.  The window is the same in every lane
 // (kept in scalar registers).
+template <int CAP>
 __device__ inline Win block_window(int rmin, int rmax, int cmin, int cmax, int jref, int kref, int n1, int n2,
                                    int *sh) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -343,8 +354,14 @@ __device__ inline Win block_window(int rmin, int rmax, int cmin, int cmax, int j
     w.k0 = __builtin_amdgcn_readfirstlane(kref) + c;
     if (w.k0 < 0) w.k0 += n2;
     else if (w.k0 >= n2) w.k0 -= n2;
-    w.lds = (w.wj <= n1 && w.wk <= n2 && w.wj * w.wk <= kWinCap);
+    w.lds = (w.wj <= n1 && w.wk <= n2 && w.wj * w.wk <= CAP);
     return w;
+}
+
+// Count a block that deposits but whose window exceeds its capacity (it falls back
+// to global atomics): one atomic per such block, none otherwise.
+__device__ __forceinline__ void win_note_overflow(const Win &w, Ctl *ctl) {
+    if (!w.lds && w.wj > 0 && threadIdx.x == 0) atomicAdd(&ctl->ovf, 1u);
 }
 
 // Window slot of node (j, k) for a CIC stencil whose other nodes are (j + 1, k),
@@ -440,8 +457,8 @@ __device__ __forceinline__ void dep4w(double *lds, const Win &w, double *glob, i
 }
 
 // G grids sharing one stencil (weights a[4 g .. 4 g + 3] for grid g, LDS
-// windows G apart by kWinCap): one slot computation and one branch for all.
-template <int G>
+// windows G apart by CAP): one slot computation and one branch for all.
+template <int G, int CAP>
 __device__ __forceinline__ void dep4w_multi(double *lds, const Win &w, double *const *glob, int n1, int n2, int j1,
                                             int j2, int k1, int k2, const double *a) {
     if (w.lds) {
@@ -450,12 +467,12 @@ __device__ __forceinline__ void dep4w_multi(double *lds, const Win &w, double *c
             lds_f64 *l = (lds_f64 *)lds + s;
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                __hip_atomic_fetch_add(l + g * kWinCap, a[4 * g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(l + g * kWinCap + w.wk, a[4 * g + 1], __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(l + g * CAP, a[4 * g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(l + g * CAP + w.wk, a[4 * g + 1], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(l + g * kWinCap + 1, a[4 * g + 2], __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(l + g * CAP + 1, a[4 * g + 2], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(l + g * kWinCap + w.wk + 1, a[4 * g + 3], __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(l + g * CAP + w.wk + 1, a[4 * g + 3], __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             return;
@@ -471,17 +488,20 @@ __device__ __forceinline__ void dep4w_multi(double *lds, const Win &w, double *c
     }
 }
 
+template <int CAP>
 __device__ inline void win_zero(double *lds, int ngrid, const Win &w) {
     if (!w.lds) return;
     const int m = w.wj * w.wk;
     for (int g = 0; g < ngrid; ++g)
-        for (int i = threadIdx.x; i < m; i += blockDim.x) lds[g * kWinCap + i] = 0.0;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) lds[g * CAP + i] = 0.0;
 }
 
 // Row of window slot i = dj * wk + dk: (i + 1/2) / wk sits at least 1/(2 wk)
 // from an integer, and the single-precision product errs by less than
-// 2^-22 kWinCap < 1/(2 kWinCap) for i < kWinCap, wk <= kWinCap.
+// 2^-22 CAP < 1/(2 CAP) for i < CAP, wk <= CAP (CAP < 1448).
+template <int CAP>
 __device__ inline void win_flush(const double *lds, double *const *glob, int ngrid, const Win &w, int n1, int n2) {
+    static_assert(CAP < 1448, "window slot rows come from a single-precision reciprocal");
     if (!w.lds) return;
     const int m = w.wj * w.wk;
     const float inv_wk = 1.0f / (float)w.wk;
@@ -493,7 +513,7 @@ __device__ inline void win_flush(const double *lds, double *const *glob, int ngr
         if (k >= n2) k -= n2;
         const unsigned t = nidx(j, k, n2);
         for (int g = 0; g < ngrid; ++g) {
-            const double v = lds[g * kWinCap + i];
+            const double v = lds[g * CAP + i];
             if (v != 0.0) unsafeAtomicAdd(at(glob[g], t), v);
         }
     }

@@ -60,6 +60,13 @@ struct Wake {
     double *h_out = nullptr;             // pinned host staging, 12 planes
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    // mixed path: particle tile (threads per block, LDS window nodes per grid),
+    // from the tile autotuner on the first pass unless set by QP_TILE
+    int tile_pt = 256, tile_cap = 640;
+    bool tile_tuned = false;
+    // mixed path, last pass, per member: particle blocks of deposit launches and
+    // those whose deposit window overflowed the LDS capacity
+    std::vector<long long> win_blocks, win_ovf;
 
     // mixed-path state (owned by qp_wake_mixed.hip)
     void *mixed = nullptr;
@@ -129,8 +136,32 @@ int deliver_slice(Wake *w, qp_slice_cb cb, void *user, int slice, int kind, cons
                   const double *ey, const double *ez, const double *by, const double *bz, const double *chi,
                   const double *bx = nullptr);
 
+// A group of batches with possibly different transverse grids, advanced in
+// lockstep with one launch per phase of the slice step over all of them.
+// The batches stay owned by their handles.
+struct Group {
+    qp_ctx_s *ctx = nullptr;
+    std::vector<Wake *> b;
+    void *dev = nullptr;   // launch tables and graphs (owned by qp_wake_mixed.hip)
+    int tile_pt = 256, tile_cap = 640;   // particle tile of every batch in the group's launches
+    bool tile_tuned = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+};
+// Particle tiles the mixed kernels are instantiated for.
+bool mixed_tile_valid(int pt, int cap);
+std::string mixed_tiles_json();
+// Group slice callback: batch index, member index, then the slice view.
+using group_cb = int (*)(int batch, int member, const qp_slice_view *v, void *user);
+int group_run_mixed(Group *g, group_cb cb, void *user, int *stop_slices);
+void group_destroy_mixed(Group *g);
+
 }  // namespace qp
 
 struct qp_wake_s {
     qp::Wake w;
+};
+
+struct qp_group_s {
+    qp::Group g;
 };

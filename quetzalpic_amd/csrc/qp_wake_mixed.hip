@@ -24,7 +24,9 @@
 #include <climits>
 #include <condition_variable>
 #include <cstring>
+#include <cstdio>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -49,14 +51,14 @@ namespace qp {
 // rho/jx/chi (:381-388).
 // Batches: blockIdx.y is the member; its grids sit at + member * nn, its
 // particles at + member * pstride, its control block at ctl + member.
-__global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, const double *psim, double *src_acc,
+template <int PT, int CAP>
+__device__ __forceinline__ void mx_A(MxPart P, MxGeom g, MxSliceIn si, const double *psim, double *src_acc,
                                               double *asq2, const LaserGauss *Ls, int laser, double *rho, double *jx,
-                                              double *chi, int init_rho, int advance, Ctl *ctl) {
-    const int wk = blockIdx.y;
+                                              double *chi, int init_rho, int advance, Ctl *ctl, int bx, int wk, int gx) {
     ctl += wk;
     MX_SKIP(ctl);
-    __shared__ double lds[kWinCap];
-    __shared__ int sh[4 * (kPT / 64) + 2];
+    __shared__ double lds[CAP];
+    __shared__ int sh[4 * (PT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
     const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
     psim += go;
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
     chi += go;
     const LaserGauss L = Ls[wk];
     const int slice = mx_slice(ctl, si);
-    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const int lp = bx * PT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
     const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos q{0, 0, 0.0, 0.0};
@@ -79,10 +81,9 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
             // commit p <- p* (advance 2: K = 0, where p* keeps its initial value 0)
             double pyv = 0.0, pzv = 0.0;
             if (advance != 2) {
-                const double2 pp = P.p[p];
-                const float2 dd = P.dp[p];
-                pyv = pp.x + (double)dd.x;
-                pzv = pp.y + (double)dd.y;
+                const double2 pst = P.ps[p];
+                pyv = pst.x;
+                pzv = pst.y;
             }
             P.p[p] = make_double2(pyv, pzv);
             const double ps = mx_gather(psim, q, n1, n2);
@@ -104,15 +105,16 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
         }
     }
     if (threadIdx.x == 0) {
-        sh[4 * (kPT / 64)] = q.j;
-        sh[4 * (kPT / 64) + 1] = q.k;
+        sh[4 * (PT / 64)] = q.j;
+        sh[4 * (PT / 64) + 1] = q.k;
     }
     __syncthreads();
-    const int jref = sh[4 * (kPT / 64)], kref = sh[4 * (kPT / 64) + 1];
+    const int jref = sh[4 * (PT / 64)], kref = sh[4 * (PT / 64) + 1];
     int rmin = INT_MAX, rmax = INT_MIN, cmin = INT_MAX, cmax = INT_MIN;
     if (valid) range_add(q.j, q.k, jref, kref, n1, n2, rmin, rmax, cmin, cmax);
-    const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
-    win_zero(lds, 1, win);
+    const Win win = block_window<CAP>(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
+    win_note_overflow(win, ctl);
+    win_zero<CAP>(lds, 1, win);
     __syncthreads();
     {
         int j1, j2, k1, k2;
@@ -127,14 +129,14 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
     }
     __syncthreads();
     double *gl[1] = {src_acc};
-    win_flush(lds, gl, 1, win, n1, n2);
+    win_flush<CAP>(lds, gl, 1, win, n1, n2);
     // The slice's laser plane and the S-phase initial rho / jx / chi last, so that
     // these stores do not queue ahead of the particle loads (nothing in this kernel
     // reads them).
     if (laser || init_rho) {
         const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
         const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
-        for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
+        for (size_t t = (size_t)bx * PT + threadIdx.x; t < nn; t += (size_t)gx * PT) {
             if (laser) {
                 const int j = (int)(t / n2), k = (int)(t % n2);
                 const double x = slice * g.dx, y = j * g.d1, z = k * g.d2;
@@ -149,6 +151,13 @@ __global__ __launch_bounds__(kPT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, 
         }
     }
 }
+template <int PT, int CAP>
+__global__ __launch_bounds__(PT) void k_mx_A(MxPart P, MxGeom g, MxSliceIn si, const double *psim, double *src_acc,
+                                              double *asq2, const LaserGauss *Ls, int laser, double *rho, double *jx,
+                                              double *chi, int init_rho, int advance, Ctl *ctl) {
+    mx_A<PT, CAP>(P, g, si, psim, src_acc, asq2, Ls, laser, rho, jx, chi, init_rho, advance, ctl, blockIdx.x, blockIdx.y,
+         gridDim.x);
+}
 
 // ------------------------------------------------------------------ B phase
 // gamma = (1 + p^2 + a + (1+psi)^2) / (2 (1+psi)) (src/System_3d.cpp:373-374), with 1/(1+psi) given.
@@ -156,15 +165,15 @@ __device__ __forceinline__ double gamma_of(double p2, double a, double ps, doubl
     return 0.5 * (1 + p2 + a + (1 + ps) * (1 + ps)) * inv1ps;
 }
 
-__global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, const double *psi, const double *by,
+template <int PT, int CAP>
+__device__ __forceinline__ void mx_B(MxPart P, MxGeom g, MxSliceIn si, const double *psi, const double *by,
                                               const double *bz, double *src_acc, double *rho, double *jx,
                                               double *chi, double *jyn, double *jzn, int first_iter, int init_rho,
-                                              Ctl *ctl) {
-    const int wk = blockIdx.y;
+                                              Ctl *ctl, int bx, int wk, int gx) {
     ctl += wk;
     MX_SKIP(ctl);
-    __shared__ double lds[kWinCap];
-    __shared__ int sh[4 * (kPT / 64) + 2];
+    __shared__ double lds[CAP];
+    __shared__ int sh[4 * (PT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
     const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
     psi += go;
@@ -178,7 +187,7 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     jzn += go;
     const int slice = mx_slice(ctl, si);
     const double *asq = mx_asq(ctl, si, nn, go);
-    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const int lp = bx * PT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
     const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos qm{0, 0, 0.0, 0.0};
@@ -196,8 +205,8 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         if (first_iter) {
             gm = gamma_of(py * py + pz * pz, a, ps, inv1ps);  // S4 (:373-374)
         } else {
-            const float2 dd = P.dp[p];
-            const double pys = py + (double)dd.x, pzs = pz + (double)dd.y;  // P6 of the previous iteration (:503-505)
+            const double2 pst = P.ps[p];
+            const double pys = pst.x, pzs = pst.y;  // P6 of the previous iteration (:503-505)
             gm = gamma_of(0.25 * (py + pys) * (py + pys) + 0.25 * (pz + pzs) * (pz + pzs), a, ps, inv1ps);
         }
         const double day = ga.dy, daz = ga.dz, dpy = gp.dy, dpz = gp.dz;
@@ -211,24 +220,23 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
         double pzn = pz - s * 0.5 * daz;
         pzn += s * gm * dpz;
         pzn += dx * byp;
-        const float dyf = (float)(pyn - py), dzf = (float)(pzn - pz);
-        P.dp[p] = make_float2(dyf, dzf);
-        // P2 (:439-442) with the stored p* = p + increment
-        const double pyst = py + (double)dyf, pzst = pz + (double)dzf;
-        double Ym = (q.j + q.fy) + (0.5 * s * pyst) * g.inv_d1;
-        double Zm = (q.k + q.fz) + (0.5 * s * pzst) * g.inv_d2;
+        P.ps[p] = make_double2(pyn, pzn);
+        // P2 (:439-442)
+        double Ym = (q.j + q.fy) + (0.5 * s * pyn) * g.inv_d1;
+        double Zm = (q.k + q.fz) + (0.5 * s * pzn) * g.inv_d2;
         qm = pos_from_cells(wrap_cells(Ym, n1), wrap_cells(Zm, n2), n1, n2);
     }
     if (threadIdx.x == 0) {
-        sh[4 * (kPT / 64)] = qm.j;
-        sh[4 * (kPT / 64) + 1] = qm.k;
+        sh[4 * (PT / 64)] = qm.j;
+        sh[4 * (PT / 64) + 1] = qm.k;
     }
     __syncthreads();
-    const int jref = sh[4 * (kPT / 64)], kref = sh[4 * (kPT / 64) + 1];
+    const int jref = sh[4 * (PT / 64)], kref = sh[4 * (PT / 64) + 1];
     int rmin = INT_MAX, rmax = INT_MIN, cmin = INT_MAX, cmax = INT_MIN;
     if (valid) range_add(qm.j, qm.k, jref, kref, n1, n2, rmin, rmax, cmin, cmax);
-    const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
-    win_zero(lds, 1, win);
+    const Win win = block_window<CAP>(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
+    win_note_overflow(win, ctl);
+    win_zero<CAP>(lds, 1, win);
     __syncthreads();
     {  // P3 (:453-457)
         int j1, j2, k1, k2;
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
     }
     __syncthreads();
     double *gl[1] = {src_acc};
-    win_flush(lds, gl, 1, win, n1, n2);
+    win_flush<CAP>(lds, gl, 1, win, n1, n2);
     // Last, so that these stores do not queue ahead of the particle loads (nothing
     // in this kernel reads the grids).
     {  // jx = jx_bunch(i), jy* = jz* = 0 before the C phase (:473-478, :511-518); rho =
@@ -251,7 +259,7 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
        // one that deposits chi (the first)
         const double *rb = si.rho_b3 ? si.rho_b3 + (size_t)slice * si.bstride : nullptr;
         const double *jb = si.jx_b3 ? si.jx_b3 + (size_t)slice * si.bstride : nullptr;
-        for (size_t t = (size_t)blockIdx.x * kPT + threadIdx.x; t < nn; t += (size_t)gridDim.x * kPT) {
+        for (size_t t = (size_t)bx * PT + threadIdx.x; t < nn; t += (size_t)gx * PT) {
             if (init_rho) rho[t] = rb ? rb[t] : 0.0;
             jx[t] = jb ? jb[t] : 0.0;
             if (first_iter) chi[t] = 0.0;
@@ -259,6 +267,14 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
             jzn[t] = 0.0;
         }
     }
+}
+template <int PT, int CAP>
+__global__ __launch_bounds__(PT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, const double *psi, const double *by,
+                                              const double *bz, double *src_acc, double *rho, double *jx,
+                                              double *chi, double *jyn, double *jzn, int first_iter, int init_rho,
+                                              Ctl *ctl) {
+    mx_B<PT, CAP>(P, g, si, psi, by, bz, src_acc, rho, jx, chi, jyn, jzn, first_iter, init_rho, ctl, blockIdx.x,
+         blockIdx.y, gridDim.x);
 }
 
 // ------------------------------------------------------------------ C phase
@@ -272,17 +288,14 @@ __global__ __launch_bounds__(kPT) void k_mx_B(MxPart P, MxGeom g, MxSliceIn si, 
 // spills (5 otherwise), which hides more of its gather -> deposit -> flush chain
 // in batches (256^2 x 10 members: 69.8 -> 65.9 us).  The same bound on B
 // (56 VGPRs, 8 waves) made it slower (77.9 -> 83.2 us) and is not used.
-template <bool RHO, bool CHI>
-__global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, const double *psi,
+template <int PT, int CAP, bool RHO, bool CHI>
+__device__ __forceinline__ void mx_C(MxPart P, MxGeom g, MxSliceIn si, const double *psi,
                                               const double *psim, double *rho, double *jx, double *chi,
-                                              double *jyn, double *jzn, Ctl *ctl) {
+                                              double *jyn, double *jzn, Ctl *ctl, double *lds, int bx, int wk) {
     constexpr int G = 1 + (RHO ? 1 : 0) + (CHI ? 1 : 0);  // node grids
-    const int wk = blockIdx.y;
     ctl += wk;
     MX_SKIP(ctl);
-    extern __shared__ double ldsd[];
-    double *lds = ldsd;
-    __shared__ int sh[4 * (kPT / 64) + 2];
+    __shared__ int sh[4 * (PT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
     const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
     psi += go;
@@ -293,7 +306,7 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
     jyn += go;
     jzn += go;
     const double *asq = mx_asq(ctl, si, nn, go);
-    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const int lp = bx * PT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
     const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos q{0, 0, 0.0, 0.0}, qm{0, 0, 0.0, 0.0};
@@ -302,9 +315,9 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         q = load_pos(P, p);
         const double w = P.w[p];
         const double2 pp = P.p[p];
-        const float2 dd = P.dp[p];
+        const double2 pst = P.ps[p];
         const double py = pp.x, pz = pp.y;
-        const double pys = py + (double)dd.x, pzs = pz + (double)dd.y;
+        const double pys = pst.x, pzs = pst.y;
         const double ps = mx_gather(psi, q, n1, n2);
         const double inv1ps = 1.0 / (1 + ps);
         const double s = g.dx * inv1ps;
@@ -327,11 +340,11 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         if (CHI) vchi = -w * inv1ps;
     }
     if (threadIdx.x == 0) {
-        sh[4 * (kPT / 64)] = q.j;
-        sh[4 * (kPT / 64) + 1] = q.k;
+        sh[4 * (PT / 64)] = q.j;
+        sh[4 * (PT / 64) + 1] = q.k;
     }
     __syncthreads();
-    const int jref = sh[4 * (kPT / 64)], kref = sh[4 * (kPT / 64) + 1];
+    const int jref = sh[4 * (PT / 64)], kref = sh[4 * (PT / 64) + 1];
     int ya1, ya2, za1, za2, yb1, yb2, zb1, zb2, yc1, yc2, zc1, zc2;
     double yaf, zaf, ybf, zbf, ycf, zcf;
     ax_half(qm.j, qm.fy, n1, ya1, ya2, yaf);  // jy* on the (dy/2, 0) grid
@@ -346,8 +359,9 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         range_add(yb1, zb1, jref, kref, n1, n2, rmin, rmax, cmin, cmax);
         range_add(yc1, zc1, jref, kref, n1, n2, rmin, rmax, cmin, cmax);
     }
-    const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
-    win_zero(lds, 2 + G, win);
+    const Win win = block_window<CAP>(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
+    win_note_overflow(win, ctl);
+    win_zero<CAP>(lds, 2 + G, win);
     __syncthreads();
     double *gc[3];  // node grids in deposit order
     {
@@ -371,25 +385,30 @@ __global__ __launch_bounds__(kPT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn s
         quad_fold<4>(yb1 * n2 + zb1, fb, ab);
         quad_fold<4 * G>(yc1 * n2 + zc1, fc, ac);
         if (fa) dep4w(lds, win, jyn, n1, n2, ya1, ya2, za1, za2, aa[0], aa[1], aa[2], aa[3]);
-        if (fb) dep4w(lds + kWinCap, win, jzn, n1, n2, yb1, yb2, zb1, zb2, ab[0], ab[1], ab[2], ab[3]);
-        if (fc) dep4w_multi<G>(lds + 2 * kWinCap, win, gc, n1, n2, yc1, yc2, zc1, zc2, ac);
+        if (fb) dep4w(lds + CAP, win, jzn, n1, n2, yb1, yb2, zb1, zb2, ab[0], ab[1], ab[2], ab[3]);
+        if (fc) dep4w_multi<G, CAP>(lds + 2 * CAP, win, gc, n1, n2, yc1, yc2, zc1, zc2, ac);
     }
     __syncthreads();
     double *gl[2 + G] = {jyn, jzn};
     for (int i = 0; i < G; ++i) gl[2 + i] = gc[i];
-    win_flush(lds, gl, 2 + G, win, n1, n2);
+    win_flush<CAP>(lds, gl, 2 + G, win, n1, n2);
+}
+template <int PT, int CAP, bool RHO, bool CHI>
+__global__ __launch_bounds__(PT, 6) void k_mx_C(MxPart P, MxGeom g, MxSliceIn si, const double *psi,
+                                              const double *psim, double *rho, double *jx, double *chi,
+                                              double *jyn, double *jzn, Ctl *ctl) {
+    extern __shared__ double ldsd[];
+    mx_C<PT, CAP, RHO, CHI>(P, g, si, psi, psim, rho, jx, chi, jyn, jzn, ctl, ldsd, blockIdx.x, blockIdx.y);
 }
 
 // -------------------------------------------------- S phase (slice 0, K = 0)
 // gamma/px (S4, :368-378) and the rho/jx/chi deposits (S5, :390-397).
-__global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, const double *psi, double *rho,
-                                              double *jx, double *chi, Ctl *ctl) {
-    const int wk = blockIdx.y;
+template <int PT, int CAP>
+__device__ __forceinline__ void mx_S(MxPart P, MxGeom g, MxSliceIn si, const double *psi, double *rho,
+                                              double *jx, double *chi, Ctl *ctl, double *lds, int bx, int wk) {
     ctl += wk;
     MX_SKIP(ctl);
-    extern __shared__ double ldsd[];
-    double *lds = ldsd;
-    __shared__ int sh[4 * (kPT / 64) + 2];
+    __shared__ int sh[4 * (PT / 64) + 2];
     const int n1 = g.n1, n2 = g.n2;
     const size_t nn = (size_t)n1 * n2, go = (size_t)wk * nn;
     psi += go;
@@ -397,7 +416,7 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
     jx += go;
     chi += go;
     const double *asq = mx_asq(ctl, si, nn, go);
-    const int lp = blockIdx.x * kPT + threadIdx.x;
+    const int lp = bx * PT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
     const unsigned p = (unsigned)(wk * P.pstride + lp);
     Pos q{0, 0, 0.0, 0.0};
@@ -418,19 +437,20 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
         vchi = -w * inv1ps;
     }
     if (threadIdx.x == 0) {
-        sh[4 * (kPT / 64)] = q.j;
-        sh[4 * (kPT / 64) + 1] = q.k;
+        sh[4 * (PT / 64)] = q.j;
+        sh[4 * (PT / 64) + 1] = q.k;
     }
     __syncthreads();
-    const int jref = sh[4 * (kPT / 64)], kref = sh[4 * (kPT / 64) + 1];
+    const int jref = sh[4 * (PT / 64)], kref = sh[4 * (PT / 64) + 1];
     int y1, y2, z1, z2;
     double yf, zf;
     ax_node(q.j, q.fy, n1, y1, y2, yf);
     ax_node(q.k, q.fz, n2, z1, z2, zf);
     int rmin = INT_MAX, rmax = INT_MIN, cmin = INT_MAX, cmax = INT_MIN;
     if (valid) range_add(y1, z1, jref, kref, n1, n2, rmin, rmax, cmin, cmax);
-    const Win win = block_window(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
-    win_zero(lds, 3, win);
+    const Win win = block_window<CAP>(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
+    win_note_overflow(win, ctl);
+    win_zero<CAP>(lds, 3, win);
     __syncthreads();
     {
         double ac[12];
@@ -441,12 +461,18 @@ __global__ __launch_bounds__(kPT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, 
         quad_fold<12>(y1 * n2 + z1, act, ac);
         if (act) {
             double *const gc[3] = {rho, jx, chi};
-            dep4w_multi<3>(lds, win, gc, n1, n2, y1, y2, z1, z2, ac);
+            dep4w_multi<3, CAP>(lds, win, gc, n1, n2, y1, y2, z1, z2, ac);
         }
     }
     __syncthreads();
     double *gl[3] = {rho, jx, chi};
-    win_flush(lds, gl, 3, win, n1, n2);
+    win_flush<CAP>(lds, gl, 3, win, n1, n2);
+}
+template <int PT, int CAP>
+__global__ __launch_bounds__(PT) void k_mx_S(MxPart P, MxGeom g, MxSliceIn si, const double *psi, double *rho,
+                                              double *jx, double *chi, Ctl *ctl) {
+    extern __shared__ double ldsd[];
+    mx_S<PT, CAP>(P, g, si, psi, rho, jx, chi, ctl, ldsd, blockIdx.x, blockIdx.y);
 }
 
 // rho_ion = S(y,z)(-n) (src/System_3d.cpp:318-323), direct atomics (once per pass).
@@ -461,7 +487,7 @@ __global__ __launch_bounds__(kPT) void k_mx_ion(MxPart P, MxGeom g, double *rho_
     double yf, zf;
     ax_node(q.j, q.fy, g.n1, j1, j2, yf);
     ax_node(q.k, q.fz, g.n2, k1, k2, zf);
-    scatter4_global(rho_ion, g.n2, j1, j2, k1, k2, yf, zf, -(double)P.w[p]);
+    scatter4_global(rho_ion, g.n2, j1, j2, k1, k2, yf, zf, -P.w[p]);
 }
 
 // ------------------------------------------------------------- output step
@@ -469,10 +495,9 @@ __global__ __launch_bounds__(kPT) void k_mx_ion(MxPart P, MxGeom g, double *rho_
 // counter to the next step.
 // psi of the slice in psi_b[slot], of the slice before in psi_b[1 - slot]; for
 // the final output_step(0) the slot follows each member's own stop slice.
-__global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi_b0, const double *psi_b1,
+__device__ __forceinline__ void mx_out(double *ex, double *ey, double *ez, const double *psi_b0, const double *psi_b1,
                          const double *by, const double *bz, int n1, int n2, double dx, double dy, double dz,
-                         int with_ex, Ctl *ctl, int slot, int final_step) {
-    const int wk = blockIdx.y;
+                         int with_ex, Ctl *ctl, int slot, int final_step, int bx, int wk) {
     ctl += wk;
     if (!final_step) {
         MX_SKIP(ctl);
@@ -491,8 +516,8 @@ __global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi_b
     ez += go;
     if (by) by += go;
     if (bz) bz += go;
-    if (!final_step && blockIdx.x == 0 && threadIdx.x == 0) ctl->slice_of[1 - slot] = ctl->slice_of[slot] - 1;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!final_step && bx == 0 && threadIdx.x == 0) ctl->slice_of[1 - slot] = ctl->slice_of[slot] - 1;
+    const size_t t = (size_t)bx * blockDim.x + threadIdx.x;
     if (t >= (size_t)n1 * n2) return;
     const int j = (int)(t / n2), k = (int)(t % n2);
     if (with_ex) ex[t] = (psi[t] - psi_prev[t]) / dx;
@@ -500,6 +525,12 @@ __global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi_b
     const size_t tk = k < n2 - 1 ? t + 1 : (size_t)n2 * j;
     ey[t] = -(psi[tj] - psi[t]) / dy + (bz ? bz[t] : 0.0);
     ez[t] = -(psi[tk] - psi[t]) / dz - (by ? by[t] : 0.0);
+}
+__global__ void k_mx_out(double *ex, double *ey, double *ez, const double *psi_b0, const double *psi_b1,
+                         const double *by, const double *bz, int n1, int n2, double dx, double dy, double dz,
+                         int with_ex, Ctl *ctl, int slot, int final_step) {
+    mx_out(ex, ey, ez, psi_b0, psi_b1, by, bz, n1, n2, dx, dy, dz, with_ex, ctl, slot, final_step, blockIdx.x,
+           blockIdx.y);
 }
 
 // chi of the slice into the attached envelope (single wake; the slice index
@@ -747,7 +778,7 @@ __global__ void k_sort_scatter(MxPart P, MxPart Q, const unsigned *offset, unsig
     Q.off[d] = P.off[p];
     Q.p[d] = P.p[p];
     Q.w[d] = P.w[p];
-    Q.dp[d] = P.dp[p];
+    Q.ps[d] = P.ps[p];
     perm2[d] = perm[p];
 }
 
@@ -758,8 +789,223 @@ __global__ void k_sort_copyback(MxPart P, MxPart Q, int *perm, const int *perm2)
     P.off[p] = Q.off[p];
     P.p[p] = Q.p[p];
     P.w[p] = Q.w[p];
-    P.dp[p] = Q.dp[p];
+    P.ps[p] = Q.ps[p];
     perm[p] = perm2[p];
+}
+
+// ----------------------------------------------------------- group launch
+// A group advances several batches (each with its own transverse grid) in
+// lockstep, and every phase of the slice step is one launch over all of them:
+// the flat block index walks the batches' own launch grids one after another
+// (batch b owns blocks [start[b], start[b+1]), laid out as gx[b] blocks per
+// member row), and each block reads its batch's arguments from a device table.
+// The solver phases run the register-resident FFT of the batch's grid side
+// (a uniform switch) with one launch shape for all sides, 64 threads per
+// block, so the LDS buffers are one dynamic allocation sized for the largest
+// transform group (sides 3 * 2^p: eight 96-point or four 192-point transforms).
+constexpr int kGroupMax = 16;
+constexpr int kGrpTB = 64;
+// LDS of the two transform families per 64-thread block: T * PADN = 64 E / N *
+// 17 N / 16 = 68 E complex values per exchange buffer (E = 4 for sides 2^p,
+// 12 for 3 * 2^p).  The families launch separately: one kernel holding both
+// would give every block the registers and LDS of the 3 * 2^p transforms.
+constexpr int grp_lds(int fam) { return 68 * (fam ? 12 : 4); }
+
+struct GrpMap {
+    int nb;                      // entries
+    int start[kGroupMax + 1];    // first block of entry i
+    int gx[kGroupMax];           // x extent of entry i's own launch grid
+    int bi[kGroupMax];           // batch (table index) of entry i
+};
+__device__ __forceinline__ void grp_locate(const GrpMap &m, int &b, int &bx, int &by) {
+    const int f = (int)blockIdx.x;
+    int i = 0;
+    while (i + 1 < m.nb && f >= m.start[i + 1]) ++i;
+    b = m.bi[i];
+    const int r = f - m.start[i];
+    by = r / m.gx[i];
+    bx = r - by * m.gx[i];
+}
+
+struct GArgsA {
+    MxPart P;
+    MxGeom g;
+    MxSliceIn si;
+    const double *psim;
+    double *src_acc, *asq2;
+    const LaserGauss *Ls;
+    int laser;
+    double *rho, *jx, *chi;
+    Ctl *ctl;
+};
+struct GArgsB {
+    MxPart P;
+    MxGeom g;
+    MxSliceIn si;
+    const double *psi, *by, *bz;
+    double *src_acc, *rho, *jx, *chi, *jyn, *jzn;
+    Ctl *ctl;
+};
+struct GArgsC {
+    MxPart P;
+    MxGeom g;
+    MxSliceIn si;
+    const double *psi, *psim;
+    double *rho, *jx, *chi, *jyn, *jzn;
+    Ctl *ctl;
+};
+struct GArgsOut {
+    double *ex, *ey, *ez;
+    const double *psi_b0, *psi_b1, *by, *bz;
+    int n1, n2;
+    double dx, dy, dz;
+    Ctl *ctl;
+};
+template <class Src>
+struct GArgsRowsF {
+    Src src;
+    double2 *spec0, *spec1;
+    int n1, n2;
+    const double2 *tw2;
+    const Ctl *ctl;
+    size_t sstride;
+};
+struct GArgsCols {
+    double2 *spec0, *spec1;
+    int n1, nh, n2;
+    const double2 *tw1;
+    const double *lam1, *lam2;
+    double D;
+    double *bpart;
+    const Ctl *ctl;
+    size_t sstride, bstride;
+};
+template <class Sink>
+struct GArgsRowsI {
+    Sink sink;
+    const double2 *spec0, *spec1;
+    int n1, n2;
+    const double2 *tw2;
+    const Ctl *ctl;
+    size_t sstride;
+};
+
+template <int PT, int CAP>
+__global__ __launch_bounds__(PT) void kg_mx_A(GrpMap m, const GArgsA *__restrict__ t, int init_rho, int advance) {
+    int b, bx, wk;
+    grp_locate(m, b, bx, wk);
+    const GArgsA &a = t[b];
+    mx_A<PT, CAP>(a.P, a.g, a.si, a.psim, a.src_acc, a.asq2, a.Ls, a.laser, a.rho, a.jx, a.chi, init_rho, advance, a.ctl, bx, wk,
+         m.gx[b]);
+}
+template <int PT, int CAP>
+__global__ __launch_bounds__(PT) void kg_mx_B(GrpMap m, const GArgsB *__restrict__ t, int first_iter, int init_rho) {
+    int b, bx, wk;
+    grp_locate(m, b, bx, wk);
+    const GArgsB &a = t[b];
+    mx_B<PT, CAP>(a.P, a.g, a.si, a.psi, a.by, a.bz, a.src_acc, a.rho, a.jx, a.chi, a.jyn, a.jzn, first_iter, init_rho, a.ctl,
+         bx, wk, m.gx[b]);
+}
+template <int PT, int CAP, bool RHO, bool CHI>
+__global__ __launch_bounds__(PT, 6) void kg_mx_C(GrpMap m, const GArgsC *__restrict__ t) {
+    extern __shared__ double ldsd[];
+    int b, bx, wk;
+    grp_locate(m, b, bx, wk);
+    const GArgsC &a = t[b];
+    mx_C<PT, CAP, RHO, CHI>(a.P, a.g, a.si, a.psi, a.psim, a.rho, a.jx, a.chi, a.jyn, a.jzn, a.ctl, ldsd, bx, wk);
+}
+// S phase: the C table carries the same pointers (psi, rho, jx, chi).
+template <int PT, int CAP>
+__global__ __launch_bounds__(PT) void kg_mx_S(GrpMap m, const GArgsC *__restrict__ t) {
+    extern __shared__ double ldsd[];
+    int b, bx, wk;
+    grp_locate(m, b, bx, wk);
+    const GArgsC &a = t[b];
+    mx_S<PT, CAP>(a.P, a.g, a.si, a.psi, a.rho, a.jx, a.chi, a.ctl, ldsd, bx, wk);
+}
+__global__ void kg_mx_out(GrpMap m, const GArgsOut *__restrict__ t, int with_ex, int slot, int final_step) {
+    int b, bx, wk;
+    grp_locate(m, b, bx, wk);
+    const GArgsOut &a = t[b];
+    mx_out(a.ex, a.ey, a.ez, a.psi_b0, a.psi_b1, a.by, a.bz, a.n1, a.n2, a.dx, a.dy, a.dz, with_ex, a.ctl, slot,
+           final_step, bx, wk);
+}
+
+// Grid sides a group launch takes: the register-resident lengths whose
+// transforms fit a 64-thread block with register twiddles.
+inline bool grp_side_ok(int n) {
+    switch (n) {
+        case 16: case 32: case 48: case 64: case 96: case 128: case 192: case 256: return true;
+        default: return false;
+    }
+}
+#define QP_GRP_SIDES_P2(F) F(16) F(32) F(64) F(128) F(256)
+#define QP_GRP_SIDES_P3(F) F(48) F(96) F(192)
+// Elements per lane of a side's transform (WfCfg<N>::E at run time).
+inline int grp_elems(int n) { return n % 3 == 0 ? 12 : 4; }
+
+template <class Src, int FAM>
+__global__ __launch_bounds__(kGrpTB) void kg_rows_fwd(GrpMap m, const GArgsRowsF<Src> *__restrict__ t, int skip_mode,
+                                                      int nspec) {
+    extern __shared__ double2 gl2[];
+    int b, bx, by;
+    grp_locate(m, b, bx, by);
+    const GArgsRowsF<Src> &a = t[b];
+    double2 *sb0 = gl2, *sb1 = gl2 + grp_lds(FAM);
+#define QP_CASE(N)                                                                                                 \
+    case N:                                                                                                        \
+        wrows_fwd_body<N, WfCfg<N>::E, kGrpTB, Src>(a.src, a.spec0, a.spec1, a.n1, a.tw2, a.ctl, skip_mode, nspec, \
+                                                    a.sstride, bx, by, sb0, sb1, nullptr);                         \
+        break;
+    if constexpr (FAM == 0) {
+        switch (a.n2) { QP_GRP_SIDES_P2(QP_CASE) default: break; }
+    } else {
+        switch (a.n2) { QP_GRP_SIDES_P3(QP_CASE) default: break; }
+    }
+#undef QP_CASE
+}
+// The border partial sums are indexed by the block's position in its batch's
+// own column grid (bx) and that grid's size.
+template <int FAM>
+__global__ __launch_bounds__(kGrpTB) void kg_cols(GrpMap m, const GArgsCols *__restrict__ t, int skip_mode, int nspec) {
+    extern __shared__ double2 gl2[];
+    int b, bx, by;
+    grp_locate(m, b, bx, by);
+    const GArgsCols &a = t[b];
+    const int tpb = kGrpTB * (FAM ? 12 : 4) / a.n1;   // transforms per block
+    const int gx = (a.nh + tpb - 1) / tpb;
+    double2 *sb0 = gl2, *sb1 = gl2 + grp_lds(FAM);
+#define QP_CASE(N)                                                                                                  \
+    case N:                                                                                                         \
+        wcols_body<N, WfCfg<N>::E, kGrpTB>(a.spec0, a.spec1, a.nh, a.n2, a.tw1, a.lam1, a.lam2, a.D, a.bpart, a.ctl, \
+                                           skip_mode, nspec, a.sstride, a.bstride, bx, by, gx, sb0, sb1, nullptr);  \
+        break;
+    if constexpr (FAM == 0) {
+        switch (a.n1) { QP_GRP_SIDES_P2(QP_CASE) default: break; }
+    } else {
+        switch (a.n1) { QP_GRP_SIDES_P3(QP_CASE) default: break; }
+    }
+#undef QP_CASE
+}
+template <class Sink, int FAM>
+__global__ __launch_bounds__(kGrpTB) void kg_rows_inv(GrpMap m, const GArgsRowsI<Sink> *__restrict__ t, int skip_mode,
+                                                      int nspec) {
+    extern __shared__ double2 gl2[];
+    int b, bx, by;
+    grp_locate(m, b, bx, by);
+    const GArgsRowsI<Sink> &a = t[b];
+    double2 *sb0 = gl2, *sb1 = gl2 + grp_lds(FAM);
+#define QP_CASE(N)                                                                                                 \
+    case N:                                                                                                        \
+        wrows_inv_body<N, WfCfg<N>::E, kGrpTB, Sink, true>(a.spec0, a.spec1, a.sink, a.n1, a.tw2, a.ctl, skip_mode, \
+                                                           nspec, a.sstride, bx, by, sb0, sb1, nullptr);           \
+        break;
+    if constexpr (FAM == 0) {
+        switch (a.n2) { QP_GRP_SIDES_P2(QP_CASE) default: break; }
+    } else {
+        switch (a.n2) { QP_GRP_SIDES_P3(QP_CASE) default: break; }
+    }
+#undef QP_CASE
 }
 
 // ------------------------------------------------------------------ host
@@ -779,6 +1025,7 @@ struct MixedState {
     std::vector<int> h_perm;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     unsigned graph_epoch = 0;  // Wake::inputs_epoch the graphs were captured at
+    int graph_pt = 0, graph_cap = 0;  // particle tile the graphs were captured with
     // streamed output (async_output): per-slice snapshots of the output planes
     // and control blocks in two device slots (slice parity), copied on a copy
     // stream into a ring of pinned host slots that a writer thread hands to the
@@ -796,7 +1043,7 @@ struct MixedState {
 static MixedState *mstate(Wake *w) { return static_cast<MixedState *>(w->mixed); }
 
 static void free_part(MxPart &P) {
-    void *arrs[] = {P.cell, P.off, P.p, P.w, P.dp};
+    void *arrs[] = {P.cell, P.off, P.p, P.w, P.ps};
     for (void *a : arrs)
         if (a) (void)hipFree(a);
     P = MxPart{};
@@ -807,8 +1054,8 @@ static int alloc_part(MxPart &P, size_t n) {
     QP_HIP_CHECK(hipMalloc(&P.cell, 4 * n));
     QP_HIP_CHECK(hipMalloc(&P.off, 8 * n));
     QP_HIP_CHECK(hipMalloc(&P.p, 16 * n));
-    QP_HIP_CHECK(hipMalloc(&P.w, 4 * n));
-    QP_HIP_CHECK(hipMalloc(&P.dp, 8 * n));
+    QP_HIP_CHECK(hipMalloc(&P.w, 8 * n));
+    QP_HIP_CHECK(hipMalloc(&P.ps, 16 * n));
     return QP_OK;
 }
 
@@ -923,7 +1170,7 @@ static int build_initial_state(Wake *w, MixedState *m) {
     const size_t total = (size_t)w->nw * m->pstride;
     std::vector<uint32_t> c2(total, 0), oyz(2 * total, 0);
     std::vector<double> pyz(2 * total, 0.0);
-    std::vector<float> wt(total, 0.0f);
+    std::vector<double> wt(total, 0.0);
     m->h_perm.assign(total, -1);
     for (int mi = 0; mi < w->nw; ++mi) {
         const WakeMember &mb = w->mem[mi];
@@ -948,14 +1195,14 @@ static int build_initial_state(Wake *w, MixedState *m) {
             oyz[2 * (base + s) + 1] = oz[i];
             pyz[2 * (base + s)] = mb.h_py[i];
             pyz[2 * (base + s) + 1] = mb.h_pz[i];
-            wt[base + s] = (float)mb.h_w[i];
+            wt[base + s] = mb.h_w[i];
             m->h_perm[base + s] = i;
         }
     }
     QP_HIP_CHECK(hipMemcpy(m->I.cell, c2.data(), 4 * total, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->I.off, oyz.data(), 8 * total, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->I.p, pyz.data(), 16 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 4 * total, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 8 * total, hipMemcpyHostToDevice));
     QP_HIP_CHECK(hipMemcpy(m->perm0, m->h_perm.data(), sizeof(int) * total, hipMemcpyHostToDevice));
     m->init_valid = true;
     return QP_OK;
@@ -973,10 +1220,10 @@ int wake_upload_mixed(Wake *w) {
     QP_HIP_CHECK(hipMemcpyAsync(m->P.cell, m->I.cell, 4 * n, hipMemcpyDeviceToDevice, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->P.off, m->I.off, 8 * n, hipMemcpyDeviceToDevice, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->P.p, m->I.p, 16 * n, hipMemcpyDeviceToDevice, s));
-    QP_HIP_CHECK(hipMemcpyAsync(m->P.w, m->I.w, 4 * n, hipMemcpyDeviceToDevice, s));
+    QP_HIP_CHECK(hipMemcpyAsync(m->P.w, m->I.w, 8 * n, hipMemcpyDeviceToDevice, s));
     // py_next = pz_next = 0 (src/System_3d.cpp:740-741): the first slice reads no
     // increment, and with K = 0 the commit writes p = 0 directly
-    QP_HIP_CHECK(hipMemsetAsync(m->P.dp, 0, 8 * n, s));
+    QP_HIP_CHECK(hipMemsetAsync(m->P.ps, 0, 16 * n, s));
     QP_HIP_CHECK(hipMemcpyAsync(m->perm, m->perm0, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
     return QP_OK;
 }
@@ -1018,7 +1265,31 @@ struct MxRun {
     int nw, pb, gb;
     size_t sstride;
     size_t ldsS;
+    int pt, cap;   // particle tile: threads per block, LDS window nodes per grid
 };
+
+// The instantiated particle tiles (threads per block, window nodes per grid).
+// The LDS of the C phase is (2 + grids) x CAP doubles: 25.6 KB at (256, 640),
+// 40 KB at (256, 1024).
+constexpr int kTiles[][2] = {{256, 640}, {128, 640}, {256, 1024}, {128, 384}};
+constexpr int kNumTiles = 4;
+
+inline bool tile_valid(int pt, int cap) {
+    for (const auto &t : kTiles)
+        if (t[0] == pt && t[1] == cap) return true;
+    return false;
+}
+
+// f(integral_constant<PT>, integral_constant<CAP>) for the tile (pt, cap).
+template <class F>
+int with_tile(int pt, int cap, F &&f) {
+    if (pt == 256 && cap == 640) return f(std::integral_constant<int, 256>{}, std::integral_constant<int, 640>{});
+    if (pt == 128 && cap == 640) return f(std::integral_constant<int, 128>{}, std::integral_constant<int, 640>{});
+    if (pt == 256 && cap == 1024) return f(std::integral_constant<int, 256>{}, std::integral_constant<int, 1024>{});
+    if (pt == 128 && cap == 384) return f(std::integral_constant<int, 128>{}, std::integral_constant<int, 384>{});
+    set_error("particle tile (" + std::to_string(pt) + ", " + std::to_string(cap) + ") not instantiated");
+    return QP_ERR_ARG;
+}
 
 #define MLAUNCH(kern, grid, block, lds, ...)                                            \
     do {                                                                                \
@@ -1128,7 +1399,8 @@ int bx_solve(MxRun &R, hipStream_t s, int slot) {
 }
 
 // Enqueue the work of slice i (everything between two output steps) for every member.
-int enqueue_slice(MxRun &R, hipStream_t s, int i) {
+template <int PT, int CAP>
+int enqueue_slice_t(MxRun &R, hipStream_t s, int i) {
     Wake *w = R.w;
     MixedState *m = R.m;
     const qp_wake_params &p = w->p;
@@ -1141,14 +1413,14 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     const dim3 pg(R.pb, R.nw);
     if (w->bunch) QP_TRY(bunch_slice_sources(w->bunch, (const Ctl *)w->ctl, slot, 0, s));
     prof_begin(w, "push_A");
-    MLAUNCH(k_mx_A, pg, kPT, 0, m->P, R.g, si, (const double *)w->psim, w->src, w->asq2,
+    MLAUNCH((k_mx_A<PT, CAP>), pg, PT, 0, m->P, R.g, si, (const double *)w->psim, w->src, w->asq2,
             (const LaserGauss *)w->d_laser, laser, w->rho, w->jx, w->chi, s_phase ? 1 : 0,
             first ? 0 : (p.K == 0 ? 2 : 1), w->ctl);
     prof_end(w, "push_A");
     QP_TRY(psi_solve(R, s, slot, psi));
     if (s_phase) {
         prof_begin(w, "push_S");
-        MLAUNCH(k_mx_S, pg, kPT, R.ldsS, m->P, R.g, si, (const double *)psi, w->rho, w->jx, w->chi, w->ctl);
+        MLAUNCH((k_mx_S<PT, CAP>), pg, PT, R.ldsS, m->P, R.g, si, (const double *)psi, w->rho, w->jx, w->chi, w->ctl);
         prof_end(w, "push_S");
         if (p.longitudinal_b) QP_TRY(bx_solve(R, s, slot));
         if (w->env)
@@ -1158,24 +1430,24 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
         for (int it = 0; it < p.K; ++it) {
             prof_begin(w, "push_B");
             const bool first_it = it == 0, last_it = it == p.K - 1;
-            MLAUNCH(k_mx_B, pg, kPT, 0, m->P, R.g, si, (const double *)psi, (const double *)w->by,
+            MLAUNCH((k_mx_B<PT, CAP>), pg, PT, 0, m->P, R.g, si, (const double *)psi, (const double *)w->by,
                     (const double *)w->bz, w->src, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1],
                     first_it ? 1 : 0, last_it ? 1 : 0, w->ctl);
             prof_end(w, "push_B");
             QP_TRY(psi_solve(R, s, slot, w->psim));
             prof_begin(w, "push_C");
-            const size_t ldsc = sizeof(double) * kWinCap * (3 + (first_it ? 1 : 0) + (last_it ? 1 : 0));
+            const size_t ldsc = sizeof(double) * CAP * (3 + (first_it ? 1 : 0) + (last_it ? 1 : 0));
             if (first_it && last_it)
-                MLAUNCH((k_mx_C<true, true>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                MLAUNCH((k_mx_C<PT, CAP, true, true>), pg, PT, ldsc, m->P, R.g, si, (const double *)psi,
                         (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
             else if (first_it)
-                MLAUNCH((k_mx_C<false, true>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                MLAUNCH((k_mx_C<PT, CAP, false, true>), pg, PT, ldsc, m->P, R.g, si, (const double *)psi,
                         (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
             else if (last_it)
-                MLAUNCH((k_mx_C<true, false>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                MLAUNCH((k_mx_C<PT, CAP, true, false>), pg, PT, ldsc, m->P, R.g, si, (const double *)psi,
                         (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
             else
-                MLAUNCH((k_mx_C<false, false>), pg, kPT, ldsc, m->P, R.g, si, (const double *)psi,
+                MLAUNCH((k_mx_C<PT, CAP, false, false>), pg, PT, ldsc, m->P, R.g, si, (const double *)psi,
                         (const double *)w->psim, w->rho, w->jx, w->chi, m->jbuf[slot][0], m->jbuf[slot][1], w->ctl);
             prof_end(w, "push_C");
             QP_TRY(b_solve(R, s, slot));
@@ -1192,12 +1464,16 @@ int enqueue_slice(MxRun &R, hipStream_t s, int i) {
     return QP_OK;
 }
 
+int enqueue_slice(MxRun &R, hipStream_t s, int i) {
+    return with_tile(R.pt, R.cap, [&](auto P, auto C) { return enqueue_slice_t<decltype(P)::value, decltype(C)::value>(R, s, i); });
+}
+
 int resort(MxRun &R, hipStream_t s) {
     Wake *w = R.w;
     MixedState *m = R.m;
     const int ntk = (w->p.nz + kTile - 1) / kTile, ntj = (w->p.ny + kTile - 1) / kTile;
     const int ntiles = ntk * ntj;
-    const dim3 pg(R.pb, 1);
+    const dim3 pg(m->pstride / kPT, 1);
     QP_HIP_CHECK(hipMemsetAsync(m->count, 0, sizeof(unsigned) * ntiles, s));
     MLAUNCH(k_sort_hist, pg, kPT, 0, m->P, m->count, ntk);
     MLAUNCH(k_sort_scan, dim3(1), 1024, 0, m->count, m->cursor, ntiles);
@@ -1357,18 +1633,21 @@ void writer_loop(Wake *w, MixedState *m, OutQueue *q, batch_cb cb, void *user, s
 
 }  // namespace
 
-int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
+// Pass set-up of one wake or batch: device initial state, launch geometry,
+// lasers, zeroed fields and control blocks (src/System_3d.cpp:262-316).
+int pass_setup(Wake *w, MxRun &R) {
     const qp_wake_params &p = w->p;
     hipStream_t s = w->stream();
     QP_TRY(wake_upload_mixed(w));
     MixedState *m = mstate(w);
     const int nw = w->nw;
-    MxRun R;
     R.w = w;
     R.m = m;
     R.nn = w->nn();
     R.nw = nw;
-    R.pb = m->pstride / kPT;
+    R.pt = w->tile_pt;
+    R.cap = w->tile_cap;
+    R.pb = m->pstride / R.pt;
     R.gb = (int)((R.nn + 255) / 256);
     R.sstride = (size_t)w->S.n1 * w->S.nh;
     R.g = MxGeom{p.ny, p.nz, p.dy, p.dz, p.ly, p.lz, p.dx, 1.0 / p.dy, 1.0 / p.dz};
@@ -1379,13 +1658,15 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
     for (int sl = 0; sl < 2; ++sl)
         R.si[sl] = MxSliceIn{w->asq3(), w->asq2, w->rho_b_src(), w->jx_b_src(), w->bsrc_stride(), sl};
     // captured slice steps hold the driver pointers and flags of their inputs
-    if (m->graph_epoch != w->inputs_epoch) {
+    if (m->graph_epoch != w->inputs_epoch || m->graph_pt != R.pt || m->graph_cap != R.cap) {
         for (auto &g : m->gexec)
             if (g) (void)hipGraphExecDestroy(g);
         m->gexec[0] = m->gexec[1] = nullptr;
         m->graph_epoch = w->inputs_epoch;
+        m->graph_pt = R.pt;
+        m->graph_cap = R.cap;
     }
-    R.ldsS = sizeof(double) * 3 * kWinCap;
+    R.ldsS = sizeof(double) * 3 * R.cap;
 
     // members' lasers (a member without one gets amplitude 0)
     std::vector<LaserGauss> hl(nw);
@@ -1406,9 +1687,218 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
         c.slice_of[p.nx & 1] = p.nx;
     }
     QP_HIP_CHECK(hipMemcpyAsync(w->ctl, hc0.data(), sizeof(Ctl) * nw, hipMemcpyHostToDevice, s));
-    QP_HIP_CHECK(hipEventRecord(w->ev0, s));
-    MLAUNCH(k_mx_ion, dim3(R.pb, nw), kPT, 0, m->P, R.g, w->rho_ion);
+    return QP_OK;
+}
+
+// First launches of a pass: rho_ion (src/System_3d.cpp:318-323) and the bunch's
+// field buffers.
+int pass_start(Wake *w, MxRun &R) {
+    hipStream_t s = w->stream();
+    MLAUNCH(k_mx_ion, dim3(R.m->pstride / kPT, R.nw), kPT, 0, R.m->P, R.g, w->rho_ion);
     if (w->bunch) QP_TRY(bunch_clear_fields(w->bunch, s));
+    return QP_OK;
+}
+
+// Inline per-slice output of slice i (after its step): every member still
+// running gets output_step(i); a member whose stop rule fired gets its stopped
+// slice once.  `running` counts the members still running.
+int deliver_inline(Wake *w, MxRun &R, batch_cb cb, void *user, int i, std::vector<char> &delivered_stop,
+                   std::vector<Ctl> &hc, int &running) {
+    const qp_wake_params &p = w->p;
+    MixedState *m = R.m;
+    const int nw = R.nw, slot = i & 1;
+    QP_HIP_CHECK(hipStreamSynchronize(w->stream()));
+    QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
+    running = 0;
+    for (int mb = 0; mb < nw; ++mb) {
+        if (delivered_stop[mb]) continue;
+        if (hc[mb].stop_acc) {
+            QP_TRY(deliver_member(w, cb, user, mb, hc[mb].stop_slice, QP_SLICE_STOPPED, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz, w->chi));
+            delivered_stop[mb] = 1;
+            continue;
+        }
+        ++running;
+        QP_TRY(deliver_member(w, cb, user, mb, i, QP_SLICE_NORMAL, w->psi_b[slot], w->rho, w->jx,
+                              m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], w->ex, w->ey, w->ez, w->by, w->bz, w->chi,
+                              p.longitudinal_b ? w->bx : nullptr));
+    }
+    return QP_OK;
+}
+
+// Launches after the last slice of a whole pass: output_step(0), the bunch's
+// slice-0 gather and an enqueued envelope step.
+int pass_final_launch(Wake *w, MxRun &R) {
+    const qp_wake_params &p = w->p;
+    hipStream_t s = w->stream();
+    const int nw = R.nw;
+    // output_step(0) after the loop (src/System_3d.cpp:622); the kernel picks each
+    // member's psi slot from its own stop slice
+    hipLaunchKernelGGL(k_mx_out, dim3(R.gb, nw), dim3(256), 0, s, w->ex, w->ey, w->ez,
+                       (const double *)w->psi_b[0], (const double *)w->psi_b[1], (const double *)w->by,
+                       (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz, p.nx > 1 ? 1 : 0, w->ctl, 0, 1);
+    QP_HIP_CHECK(hipGetLastError());
+    // bunch bucket 0: slice 0's fields are final only after output_step(0)
+    if (w->bunch)
+        QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, s));
+    // the envelope step after the pass (chi of slices the pass did not reach is
+    // the last one stored)
+    if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
+    return QP_OK;
+}
+
+// After the pass has drained: stop slices, the stopped-slice and final
+// (output_step(0)) deliveries.  any_stop gets 1 if a member stopped.
+int pass_finish(Wake *w, MxRun &R, batch_cb cb, void *user, int i_end, std::vector<char> &delivered_stop,
+                int *stop_slices, int &any_stop) {
+    const qp_wake_params &p = w->p;
+    MixedState *m = R.m;
+    const int nw = R.nw;
+    std::vector<Ctl> hc(nw);
+    QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
+    any_stop = 0;
+    w->win_blocks.assign(nw, 0);
+    w->win_ovf.assign(nw, 0);
+    for (int mb = 0; mb < nw; ++mb) {
+        const bool stopped = hc[mb].stop_acc != 0;
+        any_stop |= stopped ? 1 : 0;
+        // deposit launches of the slices this member ran: A, then S or K x (B, C)
+        long long launches = 0;
+        for (int i = p.nx - 1; i >= (stopped ? hc[mb].stop_slice : i_end); --i)
+            launches += 1 + ((i == 0 || p.K == 0) ? 1 : 2 * p.K);
+        w->win_blocks[mb] = launches * (long long)((w->mem[mb].np + R.pt - 1) / R.pt);
+        w->win_ovf[mb] = hc[mb].ovf;
+        if (stopped && cb && !delivered_stop[mb]) {
+            QP_TRY(deliver_member(w, cb, user, mb, hc[mb].stop_slice, QP_SLICE_STOPPED, nullptr, nullptr, nullptr,
+                                  nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz, w->chi));
+        }
+        if (cb && i_end == 0) {
+            const int cslot = (stopped ? hc[mb].stop_slice : 0) & 1;
+            // a stopped member's B and chi planes hold the stopped slice: zero planes
+            // instead (zero2 has a zero plane for every member)
+            const double *by0 = stopped ? w->zero2 : w->by;
+            const double *bz0 = stopped ? w->zero2 : w->bz;
+            const double *chi0 = stopped ? w->zero2 : w->chi;
+            const double *bx0 = p.longitudinal_b ? (stopped ? w->zero2 : w->bx) : nullptr;
+            QP_TRY(deliver_member(w, cb, user, mb, 0, QP_SLICE_FINAL, w->psi_b[cslot], w->rho, w->jx,
+                                  m->jbuf[1 - cslot][0], m->jbuf[1 - cslot][1], w->ex, w->ey, w->ez, by0, bz0,
+                                  chi0, bx0));
+        }
+        if (stop_slices) stop_slices[mb] = stopped ? hc[mb].stop_slice : -1;
+    }
+    return QP_OK;
+}
+
+bool mixed_tile_valid(int pt, int cap) { return tile_valid(pt, cap); }
+std::string mixed_tiles_json() {
+    std::string s = "[";
+    for (int i = 0; i < kNumTiles; ++i)
+        s += std::string(i ? ", " : "") + "[" + std::to_string(kTiles[i][0]) + ", " + std::to_string(kTiles[i][1]) + "]";
+    return s + "]";
+}
+
+// ------------------------------------------------------------ tile autotuner
+// Picks the particle tile (threads per block, LDS window nodes per grid) of a
+// wake or group: each instantiated tile runs the first slices of the pass
+// twice (the first run captures the slice step) and the fastest second run
+// wins.  Results are cached per (grid sides, members, largest member, K) for
+// the process.  QP_TILE=pt,cap forces a tile; a wake with an attached
+// envelope keeps the default (a tuning pass would store chi into it).
+struct TileKey {
+    std::vector<int> k;
+    bool operator<(const TileKey &o) const { return k < o.k; }
+};
+static std::mutex g_tile_mu;
+static std::map<TileKey, std::pair<int, int>> g_tile_cache;
+constexpr int kTuneSlices = 12;
+
+static bool tile_from_env(int &pt, int &cap) {
+    const char *e = getenv("QP_TILE");
+    if (!e) return false;
+    int a = 0, b = 0;
+    if (sscanf(e, "%d,%d", &a, &b) != 2 || !tile_valid(a, b)) return false;
+    pt = a;
+    cap = b;
+    return true;
+}
+
+static TileKey tile_key(const std::vector<Wake *> &ws) {
+    TileKey key;
+    for (Wake *w : ws) {
+        size_t mx = 0;
+        for (const WakeMember &mb : w->mem) mx = std::max(mx, mb.np);
+        key.k.insert(key.k.end(), {w->p.ny, w->p.nz, w->nw, (int)mx, w->p.K});
+    }
+    return key;
+}
+
+// run(pt, cap) runs one short pass with that tile and returns its status;
+// last_ms() reads that pass's device time.
+template <class Run, class Ms>
+int tune_tiles(const TileKey &key, int &pt, int &cap, Run &&run, Ms &&last_ms) {
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        auto it = g_tile_cache.find(key);
+        if (it != g_tile_cache.end()) {
+            pt = it->second.first;
+            cap = it->second.second;
+            return QP_OK;
+        }
+    }
+    double best = 1e300;
+    int bpt = pt, bcap = cap;
+    for (const auto &t : kTiles) {
+        for (int rep = 0; rep < 2; ++rep) QP_TRY(run(t[0], t[1]));
+        const double ms = last_ms();
+        if (ms < best) {
+            best = ms;
+            bpt = t[0];
+            bcap = t[1];
+        }
+    }
+    pt = bpt;
+    cap = bcap;
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile_cache[key] = std::make_pair(pt, cap);
+    return QP_OK;
+}
+
+static int tune_wake(Wake *w) {
+    w->tile_tuned = true;   // the tuning passes below run with the tile they set
+    if (tile_from_env(w->tile_pt, w->tile_cap)) return QP_OK;
+    if (w->env || w->p.nx < 3 || w->p.K == 0) return QP_OK;
+    const qp_wake_params saved = w->p;
+    const int lim = std::min(w->p.nx, kTuneSlices);
+    const TileKey key = tile_key({w});
+    int pt = w->tile_pt, cap = w->tile_cap;
+    int st = tune_tiles(
+        key, pt, cap,
+        [&](int a, int b) {
+            w->tile_pt = a;
+            w->tile_cap = b;
+            w->p.slice_limit = (saved.slice_limit > 0 && saved.slice_limit < lim) ? saved.slice_limit : lim;
+            w->p.async_output = 0;
+            const int r = wake_run_mixed(w, nullptr, nullptr, nullptr);
+            w->p = saved;
+            return r < 0 ? r : QP_OK;
+        },
+        [&] { return w->last_ms; });
+    w->p = saved;
+    w->tile_pt = pt;
+    w->tile_cap = cap;
+    return st;
+}
+
+int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
+    if (!w->tile_tuned && !w->prof_class) QP_TRY(tune_wake(w));
+    const qp_wake_params &p = w->p;
+    hipStream_t s = w->stream();
+    MxRun R;
+    QP_TRY(pass_setup(w, R));
+    MixedState *m = R.m;
+    const int nw = R.nw;
+    QP_HIP_CHECK(hipEventRecord(w->ev0, s));
+    QP_TRY(pass_start(w, R));
 
     const bool async = cb && p.async_output && !w->prof_class;
     const bool graph_ok = p.use_graph && (!cb || async) && !w->prof_class;
@@ -1493,23 +1983,8 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
             q.cv.notify_all();
             if (q.failed || q.all_stopped) break;
         } else if (cb && i > 0) {
-            QP_HIP_CHECK(hipStreamSynchronize(s));
-            QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
             int running = 0;
-            for (int mb = 0; mb < nw; ++mb) {
-                if (delivered_stop[mb]) continue;
-                if (hc[mb].stop_acc) {
-                    QP_TRY(deliver_member(w, cb, user, mb, hc[mb].stop_slice, QP_SLICE_STOPPED, nullptr, nullptr,
-                                          nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz,
-                                          w->chi));
-                    delivered_stop[mb] = 1;
-                    continue;
-                }
-                ++running;
-                QP_TRY(deliver_member(w, cb, user, mb, i, QP_SLICE_NORMAL, w->psi_b[slot], w->rho, w->jx,
-                                      m->jbuf[1 - slot][0], m->jbuf[1 - slot][1], w->ex, w->ey, w->ez, w->by,
-                                      w->bz, w->chi, p.longitudinal_b ? w->bx : nullptr));
-            }
+            QP_TRY(deliver_inline(w, R, cb, user, i, delivered_stop, hc, running));
             if (!running) break;
         }
     }
@@ -1526,49 +2001,368 @@ int wake_run_mixed(Wake *w, batch_cb cb, void *user, int *stop_slices) {
         }
     }
     QP_HIP_CHECK(hipStreamSynchronize(s));
-    if (i_end == 0) {
-        // output_step(0) after the loop (src/System_3d.cpp:622); the kernel picks each
-        // member's psi slot from its own stop slice
-        hipLaunchKernelGGL(k_mx_out, dim3(R.gb, nw), dim3(256), 0, s, w->ex, w->ey, w->ez,
-                           (const double *)w->psi_b[0], (const double *)w->psi_b[1], (const double *)w->by,
-                           (const double *)w->bz, p.ny, p.nz, p.dx, p.dy, p.dz, p.nx > 1 ? 1 : 0, w->ctl, 0, 1);
-        QP_HIP_CHECK(hipGetLastError());
-        // bunch bucket 0: slice 0's fields are final only after output_step(0)
-        if (w->bunch)
-            QP_TRY(bunch_slice_gather(w->bunch, w->ex, w->ey, w->ez, w->by, w->bz, (const Ctl *)w->ctl, -1, 0, s));
-        // the envelope step after the pass (chi of slices the pass did not reach is
-        // the last one stored)
-        if (w->env_step()) QP_TRY(envelope_step(w->env, w->env_dtau, w->env_chi_it, w->p.use_graph, s));
-    }
+    if (i_end == 0) QP_TRY(pass_final_launch(w, R));
     QP_HIP_CHECK(hipEventRecord(w->ev1, s));
     QP_HIP_CHECK(hipStreamSynchronize(s));
-    QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
     float ms = 0.f;
     QP_HIP_CHECK(hipEventElapsedTime(&ms, w->ev0, w->ev1));
     w->last_ms = ms;
     int any_stop = 0;
-    for (int mb = 0; mb < nw; ++mb) {
-        const bool stopped = hc[mb].stop_acc != 0;
-        any_stop |= stopped ? 1 : 0;
-        if (stopped && cb && !delivered_stop[mb]) {
-            QP_TRY(deliver_member(w, cb, user, mb, hc[mb].stop_slice, QP_SLICE_STOPPED, nullptr, nullptr, nullptr,
-                                  nullptr, nullptr, nullptr, nullptr, nullptr, w->by, w->bz, w->chi));
-        }
-        if (cb && i_end == 0) {
-            const int cslot = (stopped ? hc[mb].stop_slice : 0) & 1;
-            // a stopped member's B and chi planes hold the stopped slice: zero planes
-            // instead (zero2 has a zero plane for every member)
-            const double *by0 = stopped ? w->zero2 : w->by;
-            const double *bz0 = stopped ? w->zero2 : w->bz;
-            const double *chi0 = stopped ? w->zero2 : w->chi;
-            const double *bx0 = p.longitudinal_b ? (stopped ? w->zero2 : w->bx) : nullptr;
-            QP_TRY(deliver_member(w, cb, user, mb, 0, QP_SLICE_FINAL, w->psi_b[cslot], w->rho, w->jx,
-                                  m->jbuf[1 - cslot][0], m->jbuf[1 - cslot][1], w->ex, w->ey, w->ez, by0, bz0,
-                                  chi0, bx0));
-        }
-        if (stop_slices) stop_slices[mb] = stopped ? hc[mb].stop_slice : -1;
-    }
+    QP_TRY(pass_finish(w, R, cb, user, i_end, delivered_stop, stop_slices, any_stop));
     return any_stop ? QP_STOPPED : QP_OK;
+}
+
+
+// ------------------------------------------------------------------ groups
+namespace {
+
+// Launch tables of a group: per phase and slice parity one argument struct per
+// batch, in one device buffer whose layout is fixed by the batch count (the
+// captured slice steps hold pointers into it; each run rewrites the contents).
+struct GroupTabs {
+    GArgsA A[2][kGroupMax];
+    GArgsB B[2][kGroupMax];
+    GArgsC C[2][kGroupMax];
+    GArgsOut O[kGroupMax];
+    GArgsRowsF<SrcPsi> PF[2][kGroupMax];
+    GArgsCols PC[kGroupMax];
+    GArgsRowsI<SinkPsiFinal> PI[2][kGroupMax];   // psi of the slice
+    GArgsRowsI<SinkPsiFinal> MI[kGroupMax];      // psi_middle
+    GArgsRowsF<SrcB> BF[2][kGroupMax];
+    GArgsCols BC[kGroupMax];
+    GArgsRowsI<SinkBPair> BI[2][kGroupMax];
+};
+
+struct GroupMaps {
+    GrpMap part, out;
+    GrpMap rows1[2], rows2[2], cols1[2], cols2[2];   // per transform family (sides 2^p, 3 * 2^p)
+};
+
+struct GroupState {
+    GroupTabs *d_tabs = nullptr;
+    GroupTabs h_tabs;
+    GroupMaps maps{}, graph_maps{};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<unsigned> graph_epochs;
+};
+
+int map_total(const GrpMap &m) { return m.start[m.nb]; }
+
+void map_add(GrpMap &m, int b, int gx, int gy) {
+    const int i = m.nb;
+    m.gx[i] = gx;
+    m.bi[i] = b;
+    m.start[i + 1] = m.start[i] + gx * gy;
+    m.nb = i + 1;
+}
+
+// Launch tables and block maps of every batch (R[b] from pass_setup).
+void group_tables(Group *G, std::vector<MxRun> &R, GroupTabs &T, GroupMaps &M) {
+    M = GroupMaps{};
+    for (size_t bi = 0; bi < G->b.size(); ++bi) {
+        const int b = (int)bi;
+        Wake *w = G->b[b];
+        const qp_wake_params &p = w->p;
+        MixedState *m = R[b].m;
+        const Solver2d &S = w->S;
+        const int nw = R[b].nw;
+        const int laser = (w->any_laser() && !w->asq3()) ? 1 : 0;
+        const double scale = 1.0 / ((double)S.n1 * S.n2);
+        for (int sl = 0; sl < 2; ++sl) {
+            const MxSliceIn &si = R[b].si[sl];
+            T.A[sl][b] = GArgsA{m->P, R[b].g, si, w->psim, w->src, w->asq2, (const LaserGauss *)w->d_laser,
+                                laser, w->rho, w->jx, w->chi, w->ctl};
+            T.B[sl][b] = GArgsB{m->P, R[b].g, si, w->psi_b[sl], w->by, w->bz, w->src, w->rho, w->jx, w->chi,
+                                m->jbuf[sl][0], m->jbuf[sl][1], w->ctl};
+            T.C[sl][b] = GArgsC{m->P, R[b].g, si, w->psi_b[sl], w->psim, w->rho, w->jx, w->chi,
+                                m->jbuf[sl][0], m->jbuf[sl][1], w->ctl};
+            SrcPsi sp{w->src, w->rho_ion, w->rho_b_src(), w->jx_b_src(), w->bsrc_stride(), w->ctl, sl, S.n2, R[b].nn};
+            T.PF[sl][b] = GArgsRowsF<SrcPsi>{sp, S.spec[0], S.spec[0], S.n1, S.n2, S.ax2.dev.tw, w->ctl,
+                                             R[b].sstride};
+            SinkPsiFinal kp{w->psi_b[sl], m->bpart, m->ncb, S.n1, S.n2, p.psi_threshold - 1, 0.0, m->bstride, scale};
+            T.PI[sl][b] = GArgsRowsI<SinkPsiFinal>{kp, S.spec[0], S.spec[0], S.n1, S.n2, S.ax2.dev.tw, w->ctl,
+                                                   R[b].sstride};
+            double *jy = m->jbuf[1 - sl][0], *jz = m->jbuf[1 - sl][1];
+            double *jyn = m->jbuf[sl][0], *jzn = m->jbuf[sl][1];
+            SrcB sb{w->by, w->bz, w->jx, jy, jz, jyn, jzn, S.n1, S.n2, p.D, 1.0 / p.dx, 1.0 / p.dy, 1.0 / p.dz};
+            T.BF[sl][b] = GArgsRowsF<SrcB>{sb, S.spec[0], S.spec[1], S.n1, S.n2, S.ax2.dev.tw, w->ctl, R[b].sstride};
+            SinkBPair kb{w->by, w->bz, S.n1, S.n2, p.b_threshold, w->ctl, sl, scale};
+            T.BI[sl][b] = GArgsRowsI<SinkBPair>{kb, S.spec[0], S.spec[1], S.n1, S.n2, S.ax2.dev.tw, w->ctl,
+                                                R[b].sstride};
+        }
+        T.O[b] = GArgsOut{w->ex, w->ey, w->ez, w->psi_b[0], w->psi_b[1], w->by, w->bz, p.ny, p.nz, p.dx, p.dy, p.dz,
+                          w->ctl};
+        T.PC[b] = GArgsCols{S.spec[0], S.spec[0], S.n1, S.nh, S.n2, S.ax1.dev.tw, S.d_lam1, S.d_lam2, 0.0, m->bpart,
+                            w->ctl, R[b].sstride, m->bstride};
+        T.BC[b] = GArgsCols{S.spec[0], S.spec[1], S.n1, S.nh, S.n2, S.ax1.dev.tw, S.d_lam1, S.d_lam2, p.D, nullptr,
+                            w->ctl, R[b].sstride, 0};
+        SinkPsiFinal km{w->psim, m->bpart, m->ncb, S.n1, S.n2, p.psi_threshold - 1, 0.0, m->bstride, scale};
+        T.MI[b] = GArgsRowsI<SinkPsiFinal>{km, S.spec[0], S.spec[0], S.n1, S.n2, S.ax2.dev.tw, w->ctl,
+                                           R[b].sstride};
+        // block maps: the launch grids of the single-batch path with 64-thread solver blocks
+        const int T2 = kGrpTB * grp_elems(S.n2) / S.n2, T1 = kGrpTB * grp_elems(S.n1) / S.n1;
+        const int f2 = S.n2 % 3 == 0 ? 1 : 0, f1 = S.n1 % 3 == 0 ? 1 : 0;   // transform families
+        map_add(M.part, b, R[b].pb, nw);
+        map_add(M.rows1[f2], b, wrows_blocks_t(S.n1, T2), nw);
+        map_add(M.rows2[f2], b, wrows_blocks_t(S.n1, T2), 2 * nw);
+        map_add(M.cols1[f1], b, wcols_blocks_t(S.nh, T1), nw);
+        map_add(M.cols2[f1], b, wcols_blocks_t(S.nh, T1), 2 * nw);
+        map_add(M.out, b, R[b].gb, nw);
+    }
+}
+
+#define GLAUNCH(kern, map, block, lds, ...)                                                    \
+    do {                                                                                       \
+        if ((map).nb > 0) {                                                                    \
+            hipLaunchKernelGGL(kern, dim3(map_total(map)), dim3(block), lds, s, map, __VA_ARGS__); \
+            QP_HIP_CHECK(hipGetLastError());                                                   \
+        }                                                                                      \
+    } while (0)
+// One solver pass of every batch: one launch per transform family present
+// (GSOLVE_T for the kernels templated on their source / sink type).
+#define GSOLVE(kern, maps, nspec, tab)                                                              \
+    do {                                                                                            \
+        GLAUNCH((kern<0>), maps[0], kGrpTB, sizeof(double2) * 2 * grp_lds(0), tab, nspec, nspec);   \
+        GLAUNCH((kern<1>), maps[1], kGrpTB, sizeof(double2) * 2 * grp_lds(1), tab, nspec, nspec);   \
+    } while (0)
+#define GSOLVE_T(kern, T, maps, nspec, tab)                                                           \
+    do {                                                                                              \
+        GLAUNCH((kern<T, 0>), maps[0], kGrpTB, sizeof(double2) * 2 * grp_lds(0), tab, nspec, nspec);  \
+        GLAUNCH((kern<T, 1>), maps[1], kGrpTB, sizeof(double2) * 2 * grp_lds(1), tab, nspec, nspec);  \
+    } while (0)
+
+// One slice step of every batch: enqueue_slice with one launch per phase.
+template <int PT, int CAP>
+int enqueue_group_t(Group *G, GroupState *st, hipStream_t s, int i) {
+    const qp_wake_params &p = G->b[0]->p;
+    const GroupMaps &M = st->maps;
+    GroupTabs *T = st->d_tabs;
+    const int slot = i & 1;
+    const bool first = (i == p.nx - 1);
+    const bool s_phase = (i == 0) || (p.K == 0);
+    GLAUNCH((kg_mx_A<PT, CAP>), M.part, PT, 0, T->A[slot], s_phase ? 1 : 0, first ? 0 : (p.K == 0 ? 2 : 1));
+    GSOLVE_T(kg_rows_fwd, SrcPsi, M.rows1, 1, T->PF[slot]);
+    GSOLVE(kg_cols, M.cols1, 1, T->PC);
+    GSOLVE_T(kg_rows_inv, SinkPsiFinal, M.rows1, 1, T->PI[slot]);
+    if (s_phase) {
+        GLAUNCH((kg_mx_S<PT, CAP>), M.part, PT, sizeof(double) * 3 * CAP, T->C[slot]);
+        if (i == 0) return QP_OK;
+    } else {
+        for (int it = 0; it < p.K; ++it) {
+            const bool first_it = it == 0, last_it = it == p.K - 1;
+            GLAUNCH((kg_mx_B<PT, CAP>), M.part, PT, 0, T->B[slot], first_it ? 1 : 0, last_it ? 1 : 0);
+            GSOLVE_T(kg_rows_fwd, SrcPsi, M.rows1, 1, T->PF[slot]);
+            GSOLVE(kg_cols, M.cols1, 1, T->PC);
+            GSOLVE_T(kg_rows_inv, SinkPsiFinal, M.rows1, 1, T->MI);
+            const size_t ldsc = sizeof(double) * CAP * (3 + (first_it ? 1 : 0) + (last_it ? 1 : 0));
+            if (first_it && last_it)
+                GLAUNCH((kg_mx_C<PT, CAP, true, true>), M.part, PT, ldsc, T->C[slot]);
+            else if (first_it)
+                GLAUNCH((kg_mx_C<PT, CAP, false, true>), M.part, PT, ldsc, T->C[slot]);
+            else if (last_it)
+                GLAUNCH((kg_mx_C<PT, CAP, true, false>), M.part, PT, ldsc, T->C[slot]);
+            else
+                GLAUNCH((kg_mx_C<PT, CAP, false, false>), M.part, PT, ldsc, T->C[slot]);
+            GSOLVE_T(kg_rows_fwd, SrcB, M.rows2, 2, T->BF[slot]);
+            GSOLVE(kg_cols, M.cols2, 2, T->BC);
+            GSOLVE_T(kg_rows_inv, SinkBPair, M.rows2, 2, T->BI[slot]);
+        }
+    }
+    GLAUNCH(kg_mx_out, M.out, 256, 0, T->O, first ? 0 : 1, slot, 0);
+    return QP_OK;
+}
+
+int enqueue_group(Group *G, GroupState *st, hipStream_t s, int i) {
+    return with_tile(G->b[0]->tile_pt, G->b[0]->tile_cap, [&](auto P, auto C) {
+        return enqueue_group_t<decltype(P)::value, decltype(C)::value>(G, st, s, i);
+    });
+}
+
+// What a group launch takes: mixed batches on one context with the same slice
+// count, iterations and pass length, Gaussian lasers only, inline output, and
+// grid sides the group's solver kernels carry.
+int group_check(Group *G) {
+    if (G->b.empty() || (int)G->b.size() > kGroupMax) {
+        set_error("group: 1 to " + std::to_string(kGroupMax) + " batches");
+        return QP_ERR_ARG;
+    }
+    const qp_wake_params &p0 = G->b[0]->p;
+    for (Wake *w : G->b) {
+        const qp_wake_params &p = w->p;
+        if (w->ctx != G->ctx || p.precision != QP_PREC_MIXED) {
+            set_error("group: every batch must be a mixed-precision handle of the group's context");
+            return QP_ERR_ARG;
+        }
+        if (p.nx != p0.nx || p.K != p0.K || p.slice_limit != p0.slice_limit) {
+            set_error("group: batches must share nx, K and slice_limit (they advance in lockstep)");
+            return QP_ERR_ARG;
+        }
+        if (p.async_output || p.resort_every || p.longitudinal_b) {
+            set_error("group: async_output, resort_every and longitudinal_b are single-handle options");
+            return QP_ERR_ARG;
+        }
+        if (w->env || w->bunch || w->a_sqr3 || w->rho_b3 || w->jx_b3) {
+            set_error("group: batches take Gaussian lasers only (no envelope, bunch, 3D a_sqr or bunch sources)");
+            return QP_ERR_STATE;
+        }
+        if (!w->S.wave || !grp_side_ok(p.ny) || !grp_side_ok(p.nz)) {
+            set_error("group: grid sides must be 2^p in [16, 256] or 48, 96, 192 (got " + std::to_string(p.ny) + " x " +
+                      std::to_string(p.nz) + ")");
+            return QP_ERR_ARG;
+        }
+    }
+    return QP_OK;
+}
+
+// Adapts the group callback to one batch's member callback.
+struct GroupCbCtx {
+    group_cb cb;
+    void *user;
+    int batch;
+};
+int group_member_cb(int member, const qp_slice_view *v, void *u) {
+    const GroupCbCtx *c = static_cast<const GroupCbCtx *>(u);
+    return c->cb(c->batch, member, v, c->user);
+}
+
+}  // namespace
+
+void group_destroy_mixed(Group *G) {
+    GroupState *st = static_cast<GroupState *>(G->dev);
+    if (!st) return;
+    for (auto &g : st->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (st->d_tabs) (void)hipFree(st->d_tabs);
+    delete st;
+    G->dev = nullptr;
+}
+
+// Tile autotuner of a group (see tune_tiles): one tile for all its launches.
+static int tune_group(Group *G) {
+    G->tile_tuned = true;
+    int pt = G->tile_pt, cap = G->tile_cap;
+    if (!tile_from_env(pt, cap) && G->b[0]->p.nx >= 3 && G->b[0]->p.K > 0) {
+        std::vector<qp_wake_params> saved;
+        for (Wake *w : G->b) saved.push_back(w->p);
+        const int lim = std::min(G->b[0]->p.nx, kTuneSlices);
+        QP_TRY(tune_tiles(
+            tile_key(G->b), pt, cap,
+            [&](int a, int b) {
+                G->tile_pt = a;
+                G->tile_cap = b;
+                for (Wake *w : G->b)
+                    w->p.slice_limit = (w->p.slice_limit > 0 && w->p.slice_limit < lim) ? w->p.slice_limit : lim;
+                const int r = group_run_mixed(G, nullptr, nullptr, nullptr);
+                for (size_t i = 0; i < G->b.size(); ++i) G->b[i]->p = saved[i];
+                return r < 0 ? r : QP_OK;
+            },
+            [&] { return G->last_ms; }));
+    }
+    G->tile_pt = pt;
+    G->tile_cap = cap;
+    return QP_OK;
+}
+
+int group_run_mixed(Group *G, group_cb cb, void *user, int *stop_slices) {
+    QP_TRY(group_check(G));
+    if (!G->tile_tuned) QP_TRY(tune_group(G));
+    for (Wake *w : G->b) {   // every batch launches with the group's tile
+        w->tile_pt = G->tile_pt;
+        w->tile_cap = G->tile_cap;
+        w->tile_tuned = true;
+    }
+    const int nb = (int)G->b.size();
+    hipStream_t s = G->ctx->stream;
+    GroupState *st = static_cast<GroupState *>(G->dev);
+    if (!st) {
+        st = new GroupState();
+        G->dev = st;
+        QP_HIP_CHECK(hipMalloc(&st->d_tabs, sizeof(GroupTabs)));
+    }
+    std::vector<MxRun> R(nb);
+    for (int b = 0; b < nb; ++b) QP_TRY(pass_setup(G->b[b], R[b]));
+    group_tables(G, R, st->h_tabs, st->maps);
+    QP_HIP_CHECK(hipMemcpyAsync(st->d_tabs, &st->h_tabs, sizeof(GroupTabs), hipMemcpyHostToDevice, s));
+    // captured steps hold the block maps and the batches' driver flags
+    std::vector<unsigned> epochs(nb);
+    for (int b = 0; b < nb; ++b) epochs[b] = G->b[b]->inputs_epoch;
+    epochs.push_back((unsigned)G->tile_cap);   // the maps carry the block counts of the tile, not its capacity
+    if (std::memcmp(&st->maps, &st->graph_maps, sizeof(GroupMaps)) != 0 || epochs != st->graph_epochs) {
+        for (auto &g : st->gexec)
+            if (g) (void)hipGraphExecDestroy(g);
+        st->gexec[0] = st->gexec[1] = nullptr;
+        st->graph_maps = st->maps;
+        st->graph_epochs = epochs;
+    }
+    QP_HIP_CHECK(hipEventRecord(G->ev0, s));
+    for (int b = 0; b < nb; ++b) QP_TRY(pass_start(G->b[b], R[b]));
+
+    const qp_wake_params &p = G->b[0]->p;
+    const bool graph_ok = p.use_graph && !cb;
+    std::vector<std::vector<char>> delivered_stop(nb);
+    std::vector<std::vector<Ctl>> hc(nb);
+    std::vector<GroupCbCtx> cctx(nb);
+    for (int b = 0; b < nb; ++b) {
+        delivered_stop[b].assign(R[b].nw, 0);
+        hc[b].resize(R[b].nw);
+        cctx[b] = GroupCbCtx{cb, user, b};
+    }
+    const int i_end = (p.slice_limit > 0 && p.slice_limit < p.nx) ? p.nx - p.slice_limit : 0;
+    for (int i = p.nx - 1; i >= i_end; --i) {
+        const int slot = i & 1;
+        const bool generic = (i < p.nx - 1) && (i > 0) && (p.K > 0);
+        if (graph_ok && generic) {
+            if (!st->gexec[slot]) {
+                hipGraph_t gr;
+                QP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int rc = enqueue_group(G, st, s, i);
+                hipError_t e = hipStreamEndCapture(s, &gr);
+                if (rc < 0) {
+                    if (e == hipSuccess) (void)hipGraphDestroy(gr);
+                    return rc;
+                }
+                if (e != hipSuccess) {
+                    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                    return QP_ERR_HIP;
+                }
+                e = hipGraphInstantiate(&st->gexec[slot], gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                if (e != hipSuccess) {
+                    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                    return QP_ERR_HIP;
+                }
+            }
+            QP_HIP_CHECK(hipGraphLaunch(st->gexec[slot], s));
+        } else {
+            QP_TRY(enqueue_group(G, st, s, i));
+        }
+        if (cb && i > 0) {
+            int running = 0;
+            for (int b = 0; b < nb; ++b) {
+                int rb = 0;
+                QP_TRY(deliver_inline(G->b[b], R[b], group_member_cb, &cctx[b], i, delivered_stop[b], hc[b], rb));
+                running += rb;
+            }
+            if (!running) break;
+        }
+    }
+    QP_HIP_CHECK(hipStreamSynchronize(s));
+    if (i_end == 0)
+        for (int b = 0; b < nb; ++b) QP_TRY(pass_final_launch(G->b[b], R[b]));
+    QP_HIP_CHECK(hipEventRecord(G->ev1, s));
+    QP_HIP_CHECK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    QP_HIP_CHECK(hipEventElapsedTime(&ms, G->ev0, G->ev1));
+    G->last_ms = ms;
+    int any = 0, off = 0;
+    for (int b = 0; b < nb; ++b) {
+        int a = 0;
+        QP_TRY(pass_finish(G->b[b], R[b], cb ? group_member_cb : nullptr, &cctx[b], i_end, delivered_stop[b],
+                           stop_slices ? stop_slices + off : nullptr, a));
+        any |= a;
+        off += R[b].nw;
+    }
+    return any ? QP_STOPPED : QP_OK;
 }
 
 }  // namespace qp

@@ -65,20 +65,24 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c) {
 
 // Rows forward: rows (2p, 2p+1) packed as z = x_a + i x_b, complex FFT along
 // n2, X_a[k] = (Z[k] + conj Z[-k])/2, X_b[k] = (Z[k] - conj Z[-k])/(2i).
+// The bodies of the three solver kernels take their block coordinates (bx,
+// by) and LDS buffers as arguments: the plain kernels pass blockIdx and their
+// own static LDS, the group launch (several batches of different grid sizes
+// in one launch) the coordinates inside one batch and a shared dynamic buffer.
 template <int N2, int E, int TB, class Src>
-__global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, double2 *spec1, int n1,
-                                                  const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
-                                                  size_t sstride) {
+__device__ __forceinline__ void wrows_fwd_body(const Src &src, double2 *spec0, double2 *spec1, int n1,
+                                               const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
+                                               size_t sstride, int bx, int by, double2 *sb0, double2 *sb1,
+                                               double2 *twl) {
     using W = WfShape<N2, E, TB>;
-    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    const int which = by % nspec, wk = by / nspec;
     if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
-    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N2, E, TB>::NL];
     double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     const int tid = threadIdx.x;
     const int tt = tid / W::L, j = tid % W::L;
     WfTw<N2, E, TB> tw;
     tw.init(tw2, twl, j);
-    const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
+    const int ra = 2 * (bx * W::T + tt), rb = ra + 1;
     double2 v[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) {
@@ -97,7 +101,7 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
     __syncthreads();
     // outputs: thread = t2 + T kk writes frequencies kk + L i of row pair t2
     const int t2 = tid % W::T, kk = tid / W::T;
-    const int qa = 2 * (blockIdx.x * W::T + t2), qb = qa + 1;
+    const int qa = 2 * (bx * W::T + t2), qb = qa + 1;
     if (qa >= n1) return;
     const double2 *z = zb + t2 * W::PADN;
 #pragma unroll
@@ -112,6 +116,16 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
     }
 }
 
+template <int N2, int E, int TB, class Src>
+__global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, double2 *spec1, int n1,
+                                                  const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
+                                                  size_t sstride) {
+    using W = WfShape<N2, E, TB>;
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N2, E, TB>::NL];
+    wrows_fwd_body<N2, E, TB, Src>(src, spec0, spec1, n1, tw2, ctl, skip_mode, nspec, sstride, blockIdx.x,
+                                   blockIdx.y, sb0, sb1, twl);
+}
+
 // Columns: forward along n1, (0,0) := 0, divide by -D + lam1[m] + lam2[l],
 // inverse.  With bpart != nullptr (psi solves) also the border-mean partial
 // sums of the later c2r output x (unnormalised):
@@ -123,21 +137,21 @@ __global__ __launch_bounds__(TB) void kw_rows_fwd(Src src, double2 *spec0, doubl
 // second, and block 0 writes Re G[0][0] to bpart[2 gridDim.x] (each wake of
 // a batch at its own offset bstride).
 template <int N1, int E, int TB>
-__global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, int nh, int n2, const double2 *tw1,
-                                              const double *lam1, const double *lam2, double D, double *bpart,
-                                              const Ctl *ctl, int skip_mode, int nspec, size_t sstride,
-                                              size_t bstride) {
+__device__ __forceinline__ void wcols_body(double2 *spec0, double2 *spec1, int nh, int n2, const double2 *tw1,
+                                           const double *lam1, const double *lam2, double D, double *bpart,
+                                           const Ctl *ctl, int skip_mode, int nspec, size_t sstride,
+                                           size_t bstride, int bx, int by, int gx, double2 *sb0, double2 *sb1,
+                                           double2 *twl) {
     using W = WfShape<N1, E, TB>;
-    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    const int which = by % nspec, wk = by / nspec;
     if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
-    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N1, E, TB>::NL];
     double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     if (bpart) bpart += wk * bstride;
     const int tid = threadIdx.x;
     const int tt = tid / W::L, j = tid % W::L;
     WfTw<N1, E, TB> tw;
     tw.init(tw1, twl, j);
-    const int l = blockIdx.x * W::T + tt;
+    const int l = bx * W::T + tt;
     const bool valid = l < nh;
     double2 v[E];
     double lm[E];
@@ -170,9 +184,9 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
         double g00 = (l == 0) ? g : 0.0;
         block_sum3<TB>(s1, s2, g00);
         if (tid == 0) {
-            bpart[2 * blockIdx.x] = s1;
-            bpart[2 * blockIdx.x + 1] = s2;
-            if (blockIdx.x == 0) bpart[2 * gridDim.x] = g00;
+            bpart[2 * bx] = s1;
+            bpart[2 * bx + 1] = s2;
+            if (bx == 0) bpart[2 * gx] = g00;
         }
     }
     wave_fft<N1, E, +1>(v, b0, b1, tw, j);
@@ -182,24 +196,35 @@ __global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, in
     for (int m = 0; m < E; ++m) out[j + W::L * m] = v[m];
 }
 
+template <int N1, int E, int TB>
+__global__ __launch_bounds__(TB) void kw_cols(double2 *spec0, double2 *spec1, int nh, int n2, const double2 *tw1,
+                                              const double *lam1, const double *lam2, double D, double *bpart,
+                                              const Ctl *ctl, int skip_mode, int nspec, size_t sstride,
+                                              size_t bstride) {
+    using W = WfShape<N1, E, TB>;
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N1, E, TB>::NL];
+    wcols_body<N1, E, TB>(spec0, spec1, nh, n2, tw1, lam1, lam2, D, bpart, ctl, skip_mode, nspec, sstride, bstride,
+                          blockIdx.x, blockIdx.y, gridDim.x, sb0, sb1, twl);
+}
+
 // Rows inverse (c2r): Z[k] = X[k] + i Y[k] with X[-k] = conj X[k]; the
 // imaginary parts of the k = 0 and k = n2/2 terms are ignored (FFTW c2r
 // semantics).  Re z -> row 2p, Im z -> row 2p+1, handed to the sink.
 template <int N2, int E, int TB, class Sink, bool STAGE>
-__global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const double2 *spec1, Sink sink, int n1,
-                                                  const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
-                                                  size_t sstride) {
+__device__ __forceinline__ void wrows_inv_body(const double2 *spec0, const double2 *spec1, Sink sink, int n1,
+                                               const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
+                                               size_t sstride, int bx, int by, double2 *sb0, double2 *sb1,
+                                               double2 *twl) {
     using W = WfShape<N2, E, TB>;
-    const int which = blockIdx.y % nspec, wk = blockIdx.y / nspec;
+    const int which = by % nspec, wk = by / nspec;
     if (solver_skip(ctl ? ctl + wk : nullptr, skip_mode)) return;
-    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N2, E, TB>::NL];
     const double2 *spec = (which == 0 ? spec0 : spec1) + wk * sstride;
     sink.begin(wk);
     const int tid = threadIdx.x;
     const int tt = tid / W::L, j = tid % W::L;
     WfTw<N2, E, TB> tw;
     tw.init(tw2, twl, j);
-    const int ra = 2 * (blockIdx.x * W::T + tt), rb = ra + 1;
+    const int ra = 2 * (bx * W::T + tt), rb = ra + 1;
     constexpr int NHF = N2 / 2 + 1;  // frequencies of the half spectrum
     // STAGE (lengths >= 32): the block first copies its 2T rows of the
     // column-major half spectrum to LDS, neighbouring lanes reading
@@ -212,7 +237,7 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
     constexpr bool staged = STAGE && 2 * NHF <= W::PADN;
     if constexpr (staged) {
         constexpr int R2 = 2 * W::T;
-        const int row0 = 2 * blockIdx.x * W::T;
+        const int row0 = 2 * bx * W::T;
         for (int e = tid; e < R2 * NHF; e += TB) {
             const int r2 = e % R2, f = e / R2;
             const int row = row0 + r2;
@@ -254,6 +279,16 @@ __global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const do
         sink(wk, which, ra, k, v[m].x);
         if (rb < n1) sink(wk, which, rb, k, v[m].y);
     }
+}
+
+template <int N2, int E, int TB, class Sink, bool STAGE>
+__global__ __launch_bounds__(TB) void kw_rows_inv(const double2 *spec0, const double2 *spec1, Sink sink, int n1,
+                                                  const double2 *tw2, const Ctl *ctl, int skip_mode, int nspec,
+                                                  size_t sstride) {
+    using W = WfShape<N2, E, TB>;
+    __shared__ double2 sb0[W::LDS], sb1[W::LDS], twl[WfTw<N2, E, TB>::NL];
+    wrows_inv_body<N2, E, TB, Sink, STAGE>(spec0, spec1, sink, n1, tw2, ctl, skip_mode, nspec, sstride, blockIdx.x,
+                                           blockIdx.y, sb0, sb1, twl);
 }
 
 // ------------------------------------------------------------ launchers

@@ -21,7 +21,7 @@ and the mixed path is compared with the plain FP64 path over a longer run.
   C4  64 wakes, 64..256^2, 800 slices:           one member of each grid class, whole
                                                  pass inside its class batch, and equal
                                                  to the member run alone
-  C5  1024^2, 4000 slices, bunch + laser:        first 100 slices with the streamed bunch,
+  C5  1024^2, 4000 slices, bunch + laser:        first 300 slices with the streamed bunch,
                                                  mixed vs FP64 over 700 slices (bunch peak)
 """
 import sys
@@ -194,7 +194,12 @@ def test_c4_one_member_per_class(qp_ctx, oracle):
         single, st1, _ = wk.run(fields=("psi", "ex"))
         wk.close()
         assert st1 == 0
-        compare(f"C4 {n}x{n} batch vs alone", outs[0], single, ("psi", "ex"), 1e-9)
+        # Same kernels, but the FP64 deposit sums are atomics whose order differs
+        # between the batch and the lone run: over 800 slices the rounding
+        # differences grow to 1e-8 .. 2e-7 in e_x (a slice difference of psi,
+        # measured at 64^2 and 96^2), the level of the mixed path's own distance
+        # from the oracle and an order of magnitude below the bar.
+        compare(f"C4 {n}x{n} batch vs alone", outs[0], single, ("psi", "ex"), 1e-6)
 
 
 def test_c5_streamed_bunch(qp_ctx, oracle):
@@ -209,7 +214,10 @@ def test_c5_streamed_bunch(qp_ctx, oracle):
     b0 = wl.bunch_particles()
     bunch = capi.Bunch(qp_ctx, wl.nx, wl.ny, wl.nz, wl.dx, wl.dy, wl.dz, FIELD_SCHWINGER)
     bunch.set_particles(b0)
-    S = 100
+    # The bunch's head enters at slice 3920; over the first 300 slices psi grows to
+    # 2e-3 (slice 3700).  A shorter window compares fields near zero, where the
+    # mixed path's ~1e-10 absolute rounding floor dominates the relative error.
+    S = 300
     rb, jb = oracle.bunch_deposit_window(g, b0, wl.nx - S, S)
     assert np.abs(rb).max() > 0   # the bunch's head reaches these slices
     fields = ("psi", "ex", "ey", "by", "bz", "jx")

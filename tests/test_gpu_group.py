@@ -1,0 +1,186 @@
+"""GPU tests of groups (qp_group_create): batches with different transverse
+grids advanced in lockstep, one launch per phase of the slice step.
+
+Every member of every batch must give the result of its batch run alone
+(which the batch tests hold to single wakes and the mixed-path tests to the
+FP64 oracle): small uneven grids of both FFT families, members that stop,
+the graph-replayed step (checked through the particles it leaves), and the
+64 wakes of the C4 configuration in one group against the oracle.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
+L, LX, DX = 20.0, 4.0, 0.1
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def grid(oracle, n, **kw):
+    return oracle.grid_from_box(LX, L, L, DX, L / n, L / n, K=5, **kw)
+
+
+def inputs(oracle, g, seed, a0=1.0):
+    """(y, z, w, py, pz, laser) of one member: lattice, optional thermal momenta, a shifted laser."""
+    rng = np.random.default_rng(seed)
+    y, z, w = oracle.init_lattice(g, 2, 2)
+    py = pz = None
+    if seed % 2:
+        py, pz = rng.normal(0.0, 0.01, len(y)), rng.normal(0.0, 0.01, len(y))
+    laser = (a0 * (0.8 + 0.4 * rng.random()), 4.0, 5.0 + rng.random(), 6.0, LX - 2.0,
+             L / 2 + rng.uniform(-1, 1), L / 2 + rng.uniform(-1, 1))
+    return y, z, w, py, pz, laser
+
+
+def make_batch(qp_ctx, g, inps, use_graph=1):
+    from quetzalpic_amd import capi
+    b = capi.WakeBatch(qp_ctx, len(inps), g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ly, g.lz, K=g.K,
+                       psi_threshold=g.psi_threshold, b_threshold=g.b_threshold, use_graph=use_graph)
+    for m, (y, z, w, py, pz, laser) in enumerate(inps):
+        b.set_particles(m, y, z, w, py, pz)
+        b.set_laser_gaussian(m, *laser)
+    return b
+
+
+# (grid side, members): both FFT families (2^p and 3 * 2^p), unequal member counts
+LAYOUT = [(32, 2), (48, 1), (64, 3), (96, 2)]
+
+
+def build(qp_ctx, oracle, layout, a0=1.0, use_graph=1, **kw):
+    grids, inps, batches = [], [], []
+    for bi, (n, nm) in enumerate(layout):
+        g = grid(oracle, n, **kw)
+        ins = [inputs(oracle, g, 100 * bi + m, a0) for m in range(nm)]
+        grids.append(g)
+        inps.append(ins)
+        batches.append(make_batch(qp_ctx, g, ins, use_graph))
+    return grids, inps, batches
+
+
+def test_group_members_equal_their_batches(qp_ctx, oracle):
+    from quetzalpic_amd import capi
+    grids, inps, batches = build(qp_ctx, oracle, LAYOUT)
+    grp = capi.WakeGroup(qp_ctx, batches)
+    outs, st, stops = grp.run(fields=FIELDS)
+    grp.close()
+    assert st == 0 and all(s == -1 for ss in stops for s in ss)
+    for bi, b in enumerate(batches):
+        alone, st1, _ = b.run(fields=FIELDS)
+        assert st1 == 0
+        for m in range(b.nwakes):
+            for f in FIELDS:
+                e = rel_err(outs[bi][m][f], alone[m][f])
+                assert e[0] < 1e-9, (LAYOUT[bi], m, f, e)
+        b.close()
+
+
+def test_group_stops_like_the_batch(qp_ctx, oracle):
+    """A low B threshold: members trip the stop rule on their own slices."""
+    from quetzalpic_amd import capi
+    layout = [(32, 2), (48, 2)]
+    grids, inps, batches = build(qp_ctx, oracle, layout, a0=2.0, b_threshold=0.3)
+    grp = capi.WakeGroup(qp_ctx, batches)
+    outs, st, stops = grp.run(fields=("psi", "by", "bz"))
+    grp.close()
+    assert st == capi.QP_STOPPED
+    log("[group stops]", stops)
+    for bi, b in enumerate(batches):
+        alone, _, stops1 = b.run(fields=("psi", "by", "bz"))
+        assert stops[bi] == stops1, (bi, stops[bi], stops1)
+        for m in range(b.nwakes):
+            for f in ("psi", "by", "bz"):
+                assert rel_err(outs[bi][m][f], alone[m][f])[0] < 1e-9, (bi, m, f)
+        b.close()
+
+
+def test_group_graph_step_matches(qp_ctx, oracle):
+    """No callback: the group replays its captured slice step; the particles it
+    leaves equal those of each batch run alone (also replayed)."""
+    from quetzalpic_amd import capi
+    grids, inps, batches = build(qp_ctx, oracle, LAYOUT)
+    grp = capi.WakeGroup(qp_ctx, batches)
+    for rep in range(2):   # first run captures, second replays the captured steps
+        _, st, _ = grp.run()
+        assert st == 0
+        got = [[b.get_particles(m) for m in range(b.nwakes)] for b in batches]
+    assert grp.last_time_ms() > 0
+    grp.close()
+    for bi, b in enumerate(batches):
+        _, st1, _ = b.run()
+        assert st1 == 0
+        for m in range(b.nwakes):
+            want = b.get_particles(m)
+            for k in range(4):
+                d = np.abs(got[bi][m][k] - want[k]).max()
+                assert d < 1e-9 * max(1.0, np.abs(want[k]).max()), (bi, m, k, d)
+        b.close()
+
+
+def test_group_argument_errors(qp_ctx, oracle):
+    from quetzalpic_amd import capi
+    g = grid(oracle, 40)   # generic FFT size: not a group side
+    b = make_batch(qp_ctx, g, [inputs(oracle, g, 1)])
+    grp = capi.WakeGroup(qp_ctx, [b])
+    with pytest.raises(capi.QpError, match="grid sides"):
+        grp.run()
+    grp.close()
+    with pytest.raises(capi.QpError, match="twice"):
+        capi.WakeGroup(qp_ctx, [b, b])
+    b.close()
+
+
+def test_c4_as_one_group(qp_ctx, oracle):
+    """The 64 wakes of C4 (five grid classes) as one group: the first member of
+    each class against the oracle over the whole pass, and against its class
+    batch run alone."""
+    import time
+    from quetzalpic_amd import capi, workloads
+    members = workloads.c4_members()
+    classes = {}
+    for m in members:
+        classes.setdefault(m.ny, []).append(m)
+    sizes = sorted(classes)
+    batches = []
+    for n in sizes:
+        ms = classes[n]
+        m0 = ms[0]
+        b = capi.WakeBatch(qp_ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K)
+        for i, m in enumerate(ms):
+            y, z, w, py, pz = m.particles()
+            b.set_particles(i, y, z, w, py, pz)
+            b.set_laser_gaussian(i, *m.laser)
+        batches.append(b)
+    grp = capi.WakeGroup(qp_ctx, batches)
+    t0 = time.perf_counter()
+    outs, st, stops = grp.run(fields=("psi", "ex"), members=[(bi, 0) for bi in range(len(sizes))])
+    log(f"[C4 group] {len(members)} wakes, one pass with output: {time.perf_counter() - t0:.1f} s")
+    grp.close()
+    assert st == 0
+    for bi, n in enumerate(sizes):
+        m0 = classes[n][0]
+        g = oracle.grid_from_box(m0.lx, m0.L, m0.L, m0.dx0, m0.L / m0.n0, m0.L / m0.n0, K=m0.K)
+        y, z, w, py, pz = m0.particles()
+        want, st0, _ = oracle.wake_run(g, y, z, w, oracle.gaussian_a_sqr(g, *m0.laser), py=py, pz=pz,
+                                       fields=("psi", "ex"), variant="omp")
+        assert st0 == 0
+        for f in ("psi", "ex"):
+            e = rel_err(outs[bi][0][f], want[f])
+            log(f"[C4 group {n}x{n}] {f} vs oracle {e[0]:.2e}/{e[1]:.2e}")
+            assert e[0] < 1e-5 and e[1] < 1e-5, (n, f, e)
+        alone, st1, _ = batches[bi].run(fields=("psi", "ex"), members=[0])
+        assert st1 == 0
+        for f in ("psi", "ex"):
+            e = rel_err(outs[bi][0][f], alone[0][f])
+            log(f"[C4 group {n}x{n}] {f} vs its batch {e[0]:.2e}/{e[1]:.2e}")
+            # FP64 atomics in another order: the same bound as batch against lone run
+            assert e[0] < 1e-6, (n, f, e)
+    for b in batches:
+        b.close()
