@@ -24,6 +24,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# C4: groups (streams) the five grid-size batches run in (measured, see bench_c4).
+C4_CHAINS = 3
 FIELD_NAMES_ALL = ("psi", "rho", "jx", "jy", "jz", "ex", "ey", "ez", "by", "bz", "chi")
 
 
@@ -356,8 +358,10 @@ def bench_c5(args, capi, workloads):
             f"err vs oracle {err_cpu}")
     windows = None if args.no_profile else window_report(capi, ctx, wl, bunch=db, bunch_particles=b0)
     db.close()
-    extra = {} if args.no_extra else bench_extras(args, capi, workloads, ctx)
     ctx.close()
+    # the extras open their own contexts: HIP spreads streams over a few hardware
+    # queues, and an idle C5 stream left open may share one with a C4 chain
+    extra = {} if args.no_extra else bench_extras(args, capi, workloads)
     line = {
         "metric": "particle-slice pushes/s (mixed precision, 1 MI355X, <=1e-5 rel err in psi and Ez vs FP64)",
         "value": value, "unit": "particle-slice pushes/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -392,10 +396,11 @@ def pass_rate(capi, ctx, wl, steps):
     return ms, wl.n_particles * (wl.nx - 1) / (ms * 1e-3)
 
 
-def bench_extras(args, capi, workloads, ctx):
+def bench_extras(args, capi, workloads):
     """The smaller configurations beside the headline (no checks: their parity
     is in tests/test_gpu_configs.py)."""
     out = {}
+    ctx = capi.Context(0)
     for name, steps in (("C2", 3), ("C3", 2)):
         wl = workloads.get(name)
         ms, rate = pass_rate(capi, ctx, wl, steps)
@@ -403,12 +408,12 @@ def bench_extras(args, capi, workloads, ctx):
         log(f"[bench] {name}: {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
         if name == "C3" and not args.no_profile:
             out[name]["deposit_windows"] = window_report(capi, ctx, wl)
-    ms, rate, desc = c4_pass(capi, workloads, 2)
-    out["C4"] = {"ms_per_pass": ms, "pushes_per_s": rate, **desc}
-    log(f"[bench] C4 (one group): {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
-    ms, rate, desc = c4_pass(capi, workloads, 2, mode="streams")
-    out["C4_streams"] = {"ms_per_pass": ms, "pushes_per_s": rate, **desc}
-    log(f"[bench] C4 (batches on streams): {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
+    ctx.close()
+    for chains in (C4_CHAINS, 1):
+        ms, rate, desc = c4_pass(capi, workloads, 2, chains=chains)
+        key = "C4" if chains == C4_CHAINS else f"C4_{chains}_chain"
+        out[key] = {"ms_per_pass": ms, "pushes_per_s": rate, **desc}
+        log(f"[bench] C4 ({chains} chains): {ms:.1f} ms per pass -> {rate:.3e} pushes/s")
     return out
 
 
@@ -462,15 +467,13 @@ def window_report(capi, ctx, wl, resort_values=(0, 50), bunch=None, bunch_partic
 class C4Job:
     """C4: 64 independent wakes.  The members of one grid size form a batch
     (qp_wake_create_batch: every launch of a batch carries all its members).
-    mode "group": the five batches form one group (qp_group_create) on one
-    context, so every phase of the slice step is one launch over all 64 wakes;
-    mode "streams": the batches run concurrently, each on its own context
-    (stream) from its own host thread."""
+    The five batches are split into `chains` groups (qp_group_create: one
+    launch per phase over all batches of the group), balanced by particle
+    count; each group runs on its own context (stream) from its own host thread,
+    so the groups' dependency chains overlap.  chains = 1: all 64 wakes in one
+    launch per phase; chains = 5: one batch per stream."""
 
-    def __init__(self, capi, workloads, use_graph=1, only_class=0, mode="group"):
-        self.mode = mode
-        self.group = None
-        shared = capi.Context(0) if mode == "group" else None
+    def __init__(self, capi, workloads, use_graph=1, only_class=0, chains=1):
         members = workloads.c4_members()
         if only_class:  # one grid-size class only (profiling runs)
             members = [m for m in members if m.ny == only_class]
@@ -479,42 +482,52 @@ class C4Job:
         for m in members:
             self.classes.setdefault(m.ny, []).append(m)
         self.sizes = sorted(self.classes)
+        load = {n: sum(m.n_particles for m in self.classes[n]) for n in self.sizes}
+        chains = max(1, min(chains, len(self.sizes)))
+        bins = [[] for _ in range(chains)]
+        for n in sorted(self.sizes, key=lambda n: -load[n]):   # largest first, into the lightest chain
+            min(bins, key=lambda b: sum(load[k] for k in b)).append(n)
+        self.chains = [sorted(b) for b in bins if b]
         self.batches = {}
-        for n in self.sizes:
-            ms = self.classes[n]
-            m0 = ms[0]
-            ctx = shared if shared is not None else capi.Context(0)
-            b = capi.WakeBatch(ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K,
-                               use_graph=use_graph)
-            for i, m in enumerate(ms):
-                y, z, w, py, pz = m.particles()
-                b.set_particles(i, y, z, w, py, pz)
-                b.set_laser_gaussian(i, *m.laser)
-            self.batches[n] = (ctx, b, sum(m.n_particles for m in ms))
-            log(f"[bench] C4 class {n}x{n}: {len(ms)} members, {self.batches[n][2]} particles")
+        self.groups = []
+        for chain in self.chains:
+            ctx = capi.Context(0)
+            for n in chain:
+                ms = self.classes[n]
+                m0 = ms[0]
+                b = capi.WakeBatch(ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K,
+                                   use_graph=use_graph)
+                for i, m in enumerate(ms):
+                    y, z, w, py, pz = m.particles()
+                    b.set_particles(i, y, z, w, py, pz)
+                    b.set_laser_gaussian(i, *m.laser)
+                self.batches[n] = (ctx, b, sum(m.n_particles for m in ms))
+                log(f"[bench] C4 class {n}x{n}: {len(ms)} members, {self.batches[n][2]} particles")
+            if len(chain) > 1:
+                self.groups.append(capi.WakeGroup(ctx, [self.batches[n][1] for n in chain]))
+            else:
+                self.groups.append(self.batches[chain[0]][1])
         self.pushes = sum(m.n_particles * (m.nx - 1) for m in members)
-        if mode == "group":
-            self.group = capi.WakeGroup(shared, [self.batches[n][1] for n in self.sizes])
 
     def describe(self):
         return {"members": {f"{n}x{n}": len(self.classes[n]) for n in self.sizes},
                 "particles": sum(m.n_particles for m in self.members), "slices": self.members[0].nx,
-                "mode": self.mode}
+                "chains": [[f"{n}x{n}" for n in c] for c in self.chains]}
 
     def run(self):
-        if self.group is not None:
-            self.group.run()
+        if len(self.groups) == 1:
+            self.groups[0].run()
             return
         import threading
         errs = []
 
-        def one(b):
+        def one(g):
             try:
-                b.run()
+                g.run()
             except Exception as e:  # surfaced below
                 errs.append(e)
 
-        ths = [threading.Thread(target=one, args=(b,)) for (_, b, _) in self.batches.values()]
+        ths = [threading.Thread(target=one, args=(g,)) for g in self.groups]
         for t in ths:
             t.start()
         for t in ths:
@@ -527,8 +540,8 @@ class C4Job:
             ctx.synchronize()
 
     def close(self):
-        if self.group is not None:
-            self.group.close()
+        for g in self.groups:
+            g.close()
         ctxs = []
         for (ctx, b, _) in self.batches.values():
             b.close()
@@ -538,8 +551,8 @@ class C4Job:
             ctx.close()
 
 
-def c4_pass(capi, workloads, steps, warmup=1, mode="group"):
-    job = C4Job(capi, workloads, mode=mode)
+def c4_pass(capi, workloads, steps, warmup=1, chains=C4_CHAINS):
+    job = C4Job(capi, workloads, chains=chains)
     for _ in range(warmup):
         job.run()
     job.synchronize()
@@ -556,7 +569,7 @@ def c4_pass(capi, workloads, steps, warmup=1, mode="group"):
 
 def bench_c4(args, capi, workloads):
     """value = all members' particle-slice pushes of a pass / wall time."""
-    job = C4Job(capi, workloads, use_graph=0 if args.no_graph else 1, only_class=args.c4_class, mode=args.c4_mode)
+    job = C4Job(capi, workloads, use_graph=0 if args.no_graph else 1, only_class=args.c4_class, chains=args.c4_chains)
     for _ in range(args.warmup):
         job.run()
     job.synchronize()
@@ -694,8 +707,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel-class event passes")
     ap.add_argument("--slice-limit", type=int, default=0,
                     help="run only the first N slices of every pass (profiling runs; value counts those)")
-    ap.add_argument("--c4-mode", default="group", choices=("group", "streams"),
-                    help="C4: one launch per phase over all wakes (group) or one batch per stream")
+    ap.add_argument("--c4-chains", type=int, default=C4_CHAINS,
+                    help="C4: groups (streams) the five grid-size batches are split into; 1 = one launch per "
+                         "phase over all 64 wakes, 5 = one batch per stream")
     ap.add_argument("--c4-class", type=int, default=0,
                     help="C4 only: run the members of this grid size alone (profiling runs)")
     ap.add_argument("--envelope-only", action="store_true",

@@ -95,6 +95,8 @@ def load():
         "qp_group_destroy": ([vp], ctypes.c_int),
         "qp_group_run": ([vp, GROUP_CB, vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "qp_group_last_time_ms": ([vp, _dp], ctypes.c_int),
+        "qp_group_set_tile": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "qp_wake_set_tile": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "qp_envelope_create": ([vp] + [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.POINTER(vp)],
                                ctypes.c_int),
         "qp_envelope_destroy": ([vp], ctypes.c_int),
@@ -322,6 +324,10 @@ class Wake:
         buf = ctypes.create_string_buffer(512)
         _check(load().qp_wake_describe(self.h, buf, 512), "qp_wake_describe")
         return buf.value.decode()
+
+    def set_tile(self, threads_per_block, window_nodes):
+        """Force the particle tile of the mixed launches ((0, 0): let the autotuner pick)."""
+        _check(load().qp_wake_set_tile(self.h, threads_per_block, window_nodes), "qp_wake_set_tile")
 
     def window_stats(self, member=0):
         """(overflow_blocks, deposit_blocks) of the last mixed pass."""
@@ -580,6 +586,16 @@ class WakeBatch:
         _check(load().qp_wake_describe(self.h, buf, 512), "qp_wake_describe")
         return buf.value.decode()
 
+    def set_tile(self, threads_per_block, window_nodes):
+        """Force the particle tile of the mixed launches ((0, 0): let the autotuner pick)."""
+        _check(load().qp_wake_set_tile(self.h, threads_per_block, window_nodes), "qp_wake_set_tile")
+
+    def window_stats(self, member=0):
+        """(overflow_blocks, deposit_blocks) of the last pass for one member."""
+        o, t = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(load().qp_wake_window_stats(self.h, member, ctypes.byref(o), ctypes.byref(t)), "qp_wake_window_stats")
+        return o.value, t.value
+
     def get_particles(self, member):
         n = self.nps[member]
         y, z, py, pz = (np.zeros(n) for _ in range(4))
@@ -654,6 +670,10 @@ class WakeGroup:
         ms = ctypes.c_double()
         _check(load().qp_group_last_time_ms(self.h, ctypes.byref(ms)), "qp_group_last_time_ms")
         return ms.value
+
+    def set_tile(self, threads_per_block, window_nodes):
+        """Force the particle tile of every group launch ((0, 0): tune again)."""
+        _check(load().qp_group_set_tile(self.h, threads_per_block, window_nodes), "qp_group_set_tile")
 
     def close(self):
         if self.h:

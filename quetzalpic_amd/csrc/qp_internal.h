@@ -89,10 +89,11 @@ struct Ctl {
 
 __device__ __forceinline__ bool ctl_skip(const Ctl *c) { return c->stop_acc != 0; }
 __device__ __forceinline__ bool ctl_skip_b(const Ctl *c) { return c->stop_now != 0; }
-// c is the control block of the launching block's wake; several blocks of one
-// wake may latch (same value).
+// c is the control block of the launching block's wake.  Every skipping block
+// latches (the same value): in a group launch blockIdx.x is a flat index over
+// all batches and members, so no single block index is present for each wake.
 __device__ __forceinline__ void ctl_latch(Ctl *c) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) c->stop_now = 1;
+    if (threadIdx.x == 0) c->stop_now = 1;
 }
 
 // ------------------------------------------------------------ complex math

@@ -1042,6 +1042,15 @@ struct MixedState {
 
 static MixedState *mstate(Wake *w) { return static_cast<MixedState *>(w->mixed); }
 
+// Host <-> device copy ordered on the wake's stream, waited for.  Plain
+// hipMemcpy would go through the legacy stream, which HIP refuses while another
+// host thread captures a graph on its own stream (batches run from several
+// threads, each capturing its slice steps).
+static hipError_t copy_wait(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 static void free_part(MxPart &P) {
     void *arrs[] = {P.cell, P.off, P.p, P.w, P.ps};
     for (void *a : arrs)
@@ -1136,7 +1145,7 @@ static int ensure_state(Wake *w) {
     m->pstride = pstride;
     std::vector<int> hn(nw);
     for (size_t i = 0; i < nw; ++i) hn[i] = (int)w->mem[i].np;
-    QP_HIP_CHECK(hipMemcpy(m->npw, hn.data(), sizeof(int) * nw, hipMemcpyHostToDevice));
+    QP_HIP_CHECK(copy_wait(m->npw, hn.data(), sizeof(int) * nw, hipMemcpyHostToDevice, w->stream()));
     for (MxPart *q : {&m->P, &m->Q, &m->I}) {
         q->np = (int)w->mem[0].np;
         q->pstride = pstride;
@@ -1199,11 +1208,12 @@ static int build_initial_state(Wake *w, MixedState *m) {
             m->h_perm[base + s] = i;
         }
     }
-    QP_HIP_CHECK(hipMemcpy(m->I.cell, c2.data(), 4 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.off, oyz.data(), 8 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.p, pyz.data(), 16 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->I.w, wt.data(), 8 * total, hipMemcpyHostToDevice));
-    QP_HIP_CHECK(hipMemcpy(m->perm0, m->h_perm.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+    hipStream_t s = w->stream();
+    QP_HIP_CHECK(copy_wait(m->I.cell, c2.data(), 4 * total, hipMemcpyHostToDevice, s));
+    QP_HIP_CHECK(copy_wait(m->I.off, oyz.data(), 8 * total, hipMemcpyHostToDevice, s));
+    QP_HIP_CHECK(copy_wait(m->I.p, pyz.data(), 16 * total, hipMemcpyHostToDevice, s));
+    QP_HIP_CHECK(copy_wait(m->I.w, wt.data(), 8 * total, hipMemcpyHostToDevice, s));
+    QP_HIP_CHECK(copy_wait(m->perm0, m->h_perm.data(), sizeof(int) * total, hipMemcpyHostToDevice, s));
     m->init_valid = true;
     return QP_OK;
 }
@@ -1239,10 +1249,11 @@ int wake_download_mixed(Wake *w, int member, double *y, double *z, double *py, d
     std::vector<uint32_t> c(n), oyz(2 * n);
     std::vector<double> pyz(2 * n);
     std::vector<int> perm(n);
-    QP_HIP_CHECK(hipMemcpy(c.data(), m->P.cell + base, 4 * n, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(oyz.data(), m->P.off + base, 8 * n, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(pyz.data(), m->P.p + base, 16 * n, hipMemcpyDeviceToHost));
-    QP_HIP_CHECK(hipMemcpy(perm.data(), m->perm + base, sizeof(int) * n, hipMemcpyDeviceToHost));
+    hipStream_t s = w->stream();
+    QP_HIP_CHECK(copy_wait(c.data(), m->P.cell + base, 4 * n, hipMemcpyDeviceToHost, s));
+    QP_HIP_CHECK(copy_wait(oyz.data(), m->P.off + base, 8 * n, hipMemcpyDeviceToHost, s));
+    QP_HIP_CHECK(copy_wait(pyz.data(), m->P.p + base, 16 * n, hipMemcpyDeviceToHost, s));
+    QP_HIP_CHECK(copy_wait(perm.data(), m->perm + base, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     for (size_t s = 0; s < n; ++s) {
         const int i = perm[s];
         if (i < 0 || (size_t)i >= np) continue;
@@ -1708,7 +1719,7 @@ int deliver_inline(Wake *w, MxRun &R, batch_cb cb, void *user, int i, std::vecto
     MixedState *m = R.m;
     const int nw = R.nw, slot = i & 1;
     QP_HIP_CHECK(hipStreamSynchronize(w->stream()));
-    QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(copy_wait(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost, w->stream()));
     running = 0;
     for (int mb = 0; mb < nw; ++mb) {
         if (delivered_stop[mb]) continue;
@@ -1755,7 +1766,7 @@ int pass_finish(Wake *w, MxRun &R, batch_cb cb, void *user, int i_end, std::vect
     MixedState *m = R.m;
     const int nw = R.nw;
     std::vector<Ctl> hc(nw);
-    QP_HIP_CHECK(hipMemcpy(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost));
+    QP_HIP_CHECK(copy_wait(hc.data(), w->ctl, sizeof(Ctl) * nw, hipMemcpyDeviceToHost, w->stream()));
     any_stop = 0;
     w->win_blocks.assign(nw, 0);
     w->win_ovf.assign(nw, 0);

@@ -83,10 +83,16 @@ def test_group_members_equal_their_batches(qp_ctx, oracle):
 
 
 def test_group_stops_like_the_batch(qp_ctx, oracle):
-    """A low B threshold: members trip the stop rule on their own slices."""
+    """THRESHOLD_B = 0.3 (the batch tests' stop case): the a0 = 8 member of each
+    batch trips the stop rule on its own slice, the others run through."""
     from quetzalpic_amd import capi
-    layout = [(32, 2), (48, 2)]
-    grids, inps, batches = build(qp_ctx, oracle, layout, a0=2.0, b_threshold=0.3)
+    lasers = [(1.0, 4.0, 6.0, 6.0, LX - 2.0, L / 2, L / 2), (8.0, 4.0, 6.0, 6.0, LX - 2.0, L / 2, L / 2),
+              (1.2, 4.0, 6.0, 6.0, LX - 2.0, L / 2 + 1.0, L / 2)]
+    batches = []
+    for n in (32, 48):
+        g = grid(oracle, n, b_threshold=0.3)
+        y, z, w = oracle.init_lattice(g, 2, 2)
+        batches.append(make_batch(qp_ctx, g, [(y, z, w, None, None, las) for las in lasers]))
     grp = capi.WakeGroup(qp_ctx, batches)
     outs, st, stops = grp.run(fields=("psi", "by", "bz"))
     grp.close()

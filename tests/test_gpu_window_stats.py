@@ -18,7 +18,9 @@ def test_window_stats_cold_plasma(qp_ctx, oracle):
     _, st, _ = wk.run()
     assert st == 0
     ovf, tot = wk.window_stats()
-    blocks = (len(y) + 255) // 256
+    import json
+    pt = json.loads(wk.describe())["tile"][0]   # threads (= particles) per block of the tuned tile
+    blocks = (len(y) + pt - 1) // pt
     # slices nx-1 .. 1: A + 5 B + 5 C; slice 0: A + S
     assert tot == ((g.nx - 1) * 11 + 2) * blocks
     assert ovf == 0
