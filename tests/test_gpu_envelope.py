@@ -28,11 +28,13 @@ def random_state(shape, seed):
     return a, chi
 
 
-@pytest.mark.parametrize("n1,n2", [(32, 32), (64, 32), (16, 128), (24, 24), (20, 36), (48, 96), (96, 64)])
+@pytest.mark.parametrize("n1,n2", [(32, 32), (64, 32), (16, 128), (24, 24), (20, 36), (48, 96), (96, 64),
+                                   (256, 256), (512, 512), (1024, 64)])
 @pytest.mark.parametrize("m", [0, 2])
 def test_advance_matches_oracle(qp_ctx, n1, n2, m):
     """2^p and 3 * 2^p sides (48, 96) take the register-resident FFT kernels, the
-    others the generic ones."""
+    others the generic ones.  n1 = 256, 512 and 1024 give the column kernel its
+    64-, 128- and 256-thread shapes (one transform per block)."""
     from quetzalpic_amd import capi
     nx, dx, dy, dz, k0, dt = 12, 0.3, 0.7, 0.5, 5.0, 0.8
     a, chi = random_state((nx, n1, n2), 3 + n1 + n2 + m)
