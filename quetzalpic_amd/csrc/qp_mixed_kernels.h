@@ -145,11 +145,15 @@ __device__ __forceinline__ double *at(double *a, unsigned i) {
 // Node index j * n2 + k with a 24-bit multiply (j, n2 <= 1024).
 __device__ __forceinline__ unsigned nidx(int j, int k, int n2) { return __umul24((unsigned)j, (unsigned)n2) + (unsigned)k; }
 
+// Bilinear weights of the four nodes (j1,k1), (j2,k1), (j1,k2), (j2,k2).
+__device__ __forceinline__ double bilin(double c11, double c21, double c12, double c22, double yf, double zf) {
+    return c11 * (1 - yf) * (1 - zf) + c21 * yf * (1 - zf) + c12 * (1 - yf) * zf + c22 * yf * zf;
+}
+
 __device__ __forceinline__ double gather4u(const double *a, int n2, int j1, int j2, int k1, int k2, double yf,
                                            double zf) {
     const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2);
-    return ld_at(a, r1 + k1) * (1 - yf) * (1 - zf) + ld_at(a, r2 + k1) * yf * (1 - zf) +
-           ld_at(a, r1 + k2) * (1 - yf) * zf + ld_at(a, r2 + k2) * yf * zf;
+    return bilin(ld_at(a, r1 + k1), ld_at(a, r2 + k1), ld_at(a, r1 + k2), ld_at(a, r2 + k2), yf, zf);
 }
 
 __device__ __forceinline__ Pos pos_from_cells(double Y, double Z, int n1, int n2) {
@@ -217,21 +221,13 @@ __device__ __forceinline__ double mx_gather_hy(const double *a, const Pos &q, in
 struct ValGrad {
     double v, dy, dz;
 };
-__device__ __forceinline__ ValGrad mx_gather_vg(const double *a, const Pos &q, int n1, int n2, double inv_d1,
-                                                double inv_d2) {
-    const int j1 = q.j, j2 = q.j + 1 == n1 ? 0 : q.j + 1;
-    const int k1 = q.k, k2 = q.k + 1 == n2 ? 0 : q.k + 1;
-    const bool hy = q.fy >= 0.5, hz = q.fz >= 0.5;
-    const int jx = hy ? (j2 + 1 == n1 ? 0 : j2 + 1) : (j1 == 0 ? n1 - 1 : j1 - 1);
-    const int kx = hz ? (k2 + 1 == n2 ? 0 : k2 + 1) : (k1 == 0 ? n2 - 1 : k1 - 1);
-    const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2),
-                   rx = __umul24((unsigned)jx, (unsigned)n2);
-    const double c11 = ld_at(a, r1 + k1), c21 = ld_at(a, r2 + k1), c12 = ld_at(a, r1 + k2), c22 = ld_at(a, r2 + k2);
-    const double ey1 = ld_at(a, rx + k1), ey2 = ld_at(a, rx + k2);
-    const double ez1 = ld_at(a, r1 + kx), ez2 = ld_at(a, r2 + kx);
-    const double yf = q.fy, zf = q.fz;
+// Value and staggered derivatives from the eight nodes (cell corners c, the
+// y-stencil's extra row e_y, the z-stencil's extra column e_z).
+__device__ __forceinline__ ValGrad vg_math(double c11, double c21, double c12, double c22, double ey1, double ey2,
+                                           double ez1, double ez2, double yf, double zf, bool hy, bool hz,
+                                           double inv_d1, double inv_d2) {
     ValGrad o;
-    o.v = c11 * (1 - yf) * (1 - zf) + c21 * yf * (1 - zf) + c12 * (1 - yf) * zf + c22 * yf * zf;
+    o.v = bilin(c11, c21, c12, c22, yf, zf);
     // Derivatives: along each line of the other axis the half-node stencil
     // interpolates between the middle difference (at the cell's half node) and
     // the outer one (at the half node beyond the cell on the particle's side),
@@ -254,6 +250,18 @@ __device__ __forceinline__ ValGrad mx_gather_vg(const double *a, const Pos &q, i
         o.dz = (g1 * (1 - yf) + g2 * yf) * inv_d2;
     }
     return o;
+}
+__device__ __forceinline__ ValGrad mx_gather_vg(const double *a, const Pos &q, int n1, int n2, double inv_d1,
+                                                double inv_d2) {
+    const int j1 = q.j, j2 = q.j + 1 == n1 ? 0 : q.j + 1;
+    const int k1 = q.k, k2 = q.k + 1 == n2 ? 0 : q.k + 1;
+    const bool hy = q.fy >= 0.5, hz = q.fz >= 0.5;
+    const int jx = hy ? (j2 + 1 == n1 ? 0 : j2 + 1) : (j1 == 0 ? n1 - 1 : j1 - 1);
+    const int kx = hz ? (k2 + 1 == n2 ? 0 : k2 + 1) : (k1 == 0 ? n2 - 1 : k1 - 1);
+    const unsigned r1 = __umul24((unsigned)j1, (unsigned)n2), r2 = __umul24((unsigned)j2, (unsigned)n2),
+                   rx = __umul24((unsigned)jx, (unsigned)n2);
+    return vg_math(ld_at(a, r1 + k1), ld_at(a, r2 + k1), ld_at(a, r1 + k2), ld_at(a, r2 + k2), ld_at(a, rx + k1),
+                   ld_at(a, rx + k2), ld_at(a, r1 + kx), ld_at(a, r2 + kx), q.fy, q.fz, hy, hz, inv_d1, inv_d2);
 }
 
 // ------------------------------------------------------ LDS deposit window
@@ -534,6 +542,67 @@ __device__ __forceinline__ void range_add(int j1, int k1, int jref, int kref, in
     rmax = max(rmax, r + 1);
     cmin = min(cmin, c);
     cmax = max(cmax, c + 1);
+}
+
+// ------------------------------------------------------ LDS gather window
+// The gathers of a block's particles read the grid nodes around the block's
+// cells.  Issued per particle they are scattered 8-byte loads, and those keep
+// the CU's vector-memory address and data units (TA, TD) busy 80-90 % of a push
+// launch at 1024^2 (counters), which bounds the kernel.  Staged instead: the
+// block copies the window of nodes its stencils can touch into LDS with
+// row-contiguous loads, and every gather reads LDS.  Window rows and columns are
+// relative to its origin and cover each stencil, so no periodic wrap is needed
+// inside; the arithmetic is that of the global gathers, so both give the same
+// bits.  A block whose window exceeds the capacity gathers from global memory.
+// Nodes per staged grid (rows x columns of the window); 384 covers a 256-particle
+// block of a sorted 4-per-cell lattice (about 7 x 19 nodes) with room for motion.
+constexpr int kGwCap = 384;
+static_assert(kGwCap < 1448, "window slot rows come from a single-precision reciprocal");
+template <int NG>
+__device__ inline void gwin_load(double *gw, int gcap, const double *const (&src)[NG], const Win &w, int n1,
+                                 int n2) {
+    const int m = w.wj * w.wk;
+    const float inv_wk = 1.0f / (float)w.wk;  // exact rows for m < 1448 (as in win_flush)
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const int dj = (int)(((float)i + 0.5f) * inv_wk), dk = i - dj * w.wk;
+        int j = w.j0 + dj;
+        if (j >= n1) j -= n1;
+        int k = w.k0 + dk;
+        if (k >= n2) k -= n2;
+        const unsigned t = nidx(j, k, n2);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) gw[g * gcap + i] = ld_at(src[g], t);
+    }
+}
+// Window-relative row and column of cell (j, k).
+__device__ __forceinline__ void gwin_rc(const Win &w, int j, int k, int n1, int n2, int &r, int &c) {
+    r = j - w.j0;
+    if (r < 0) r += n1;
+    c = k - w.k0;
+    if (c < 0) c += n2;
+}
+// Node-origin gather at window cell (r, c).
+__device__ __forceinline__ double lg_node(const double *l, int wk, int r, int c, double yf, double zf) {
+    const double *l1 = l + r * wk + c;
+    return bilin(l1[0], l1[wk], l1[1], l1[wk + 1], yf, zf);
+}
+// B_y on (0, dz/2): node rows, half-node columns (ax_node x ax_half).
+__device__ __forceinline__ double lg_hz(const double *l, int wk, int r, int c, double fy, double fz) {
+    const bool h = fz >= 0.5;
+    return lg_node(l, wk, r, h ? c : c - 1, fy, h ? fz - 0.5 : fz + 0.5);
+}
+// B_z on (dy/2, 0): half-node rows, node columns (ax_half x ax_node).
+__device__ __forceinline__ double lg_hy(const double *l, int wk, int r, int c, double fy, double fz) {
+    const bool h = fy >= 0.5;
+    return lg_node(l, wk, h ? r : r - 1, c, h ? fy - 0.5 : fy + 0.5, fz);
+}
+__device__ __forceinline__ ValGrad lg_vg(const double *l, int wk, int r, int c, double fy, double fz, double inv_d1,
+                                         double inv_d2) {
+    const bool hy = fy >= 0.5, hz = fz >= 0.5;
+    const double *l1 = l + r * wk, *l2 = l1 + wk, *lx = l + (hy ? r + 2 : r - 1) * wk;
+    const int cx = hz ? c + 2 : c - 1;
+    return vg_math(l1[c], l2[c], l1[c + 1], l2[c + 1], lx[c], lx[c + 1], l1[cx], l2[cx], fy, fz, hy, hz, inv_d1,
+                   inv_d2);
 }
 
 }  // namespace qp

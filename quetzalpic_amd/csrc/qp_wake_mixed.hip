@@ -191,23 +191,57 @@ __device__ __forceinline__ void mx_B(MxPart P, MxGeom g, MxSliceIn si, const dou
     const int lp = bx * PT + threadIdx.x;
     const bool valid = lp < mx_np(P, wk);
     const unsigned p = (unsigned)(wk * P.pstride + lp);
-    Pos qm{0, 0, 0.0, 0.0};
+    Pos q{0, 0, 0.0, 0.0}, qm{0, 0, 0.0, 0.0};
     double w = 0.0;
+    double2 pp = make_double2(0.0, 0.0), pst = pp;
     if (valid) {
-        // The particle's loads and its four gathers sit in one basic block ahead
-        // of the arithmetic, so that they are in flight together (p* is loaded
-        // also in the first iteration: a load behind the iteration branch waited
-        // for every earlier load and held the B_y / B_z gathers back for another
-        // round trip).  Loading the particle ahead of the control block, as the A
-        // phase does, measured slower here at 512^2 (25.9 -> 27.1 us).
-        const Pos q = load_pos(P, p);
+        q = load_pos(P, p);
         w = P.w[p];
-        const double2 pp = P.p[p];
-        const double2 pst = P.ps[p];  // P6 of the previous iteration (:503-505)
-        const ValGrad ga = mx_gather_vg(asq, q, n1, n2, g.inv_d1, g.inv_d2);
-        const ValGrad gp = mx_gather_vg(psi, q, n1, n2, g.inv_d1, g.inv_d2);
-        const double byp = mx_gather_hz(by, q, n1, n2);
-        const double bzp = mx_gather_hy(bz, q, n1, n2);
+        pp = P.p[p];
+        pst = P.ps[p];  // P6 of the previous iteration (:503-505); unused in the first
+    }
+    // Gather window: rows and columns -1 .. +2 around the block's cells hold every
+    // node of the value + gradient stencils and of the staggered B stencils.
+    __shared__ double gw[4 * kGwCap];
+    Win gwn;
+    {
+        if (threadIdx.x == 0) {
+            sh[4 * (PT / 64)] = q.j;
+            sh[4 * (PT / 64) + 1] = q.k;
+        }
+        __syncthreads();
+        const int jref = sh[4 * (PT / 64)], kref = sh[4 * (PT / 64) + 1];
+        int rmin = INT_MAX, rmax = INT_MIN, cmin = INT_MAX, cmax = INT_MIN;
+        if (valid) {
+            const int r = rel_idx(q.j, jref, n1), c = rel_idx(q.k, kref, n2);
+            rmin = r - 1;
+            rmax = r + 2;
+            cmin = c - 1;
+            cmax = c + 2;
+        }
+        gwn = block_window<kGwCap>(rmin, rmax, cmin, cmax, jref, kref, n1, n2, sh);
+        if (gwn.lds) {
+            const double *src[4] = {psi, asq, by, bz};
+            gwin_load<4>(gw, kGwCap, src, gwn, n1, n2);
+        }
+        __syncthreads();
+    }
+    if (valid) {
+        ValGrad ga, gp;
+        double byp, bzp;
+        if (gwn.lds) {
+            int r, c;
+            gwin_rc(gwn, q.j, q.k, n1, n2, r, c);
+            gp = lg_vg(gw, gwn.wk, r, c, q.fy, q.fz, g.inv_d1, g.inv_d2);
+            ga = lg_vg(gw + kGwCap, gwn.wk, r, c, q.fy, q.fz, g.inv_d1, g.inv_d2);
+            byp = lg_hz(gw + 2 * kGwCap, gwn.wk, r, c, q.fy, q.fz);
+            bzp = lg_hy(gw + 3 * kGwCap, gwn.wk, r, c, q.fy, q.fz);
+        } else {
+            ga = mx_gather_vg(asq, q, n1, n2, g.inv_d1, g.inv_d2);
+            gp = mx_gather_vg(psi, q, n1, n2, g.inv_d1, g.inv_d2);
+            byp = mx_gather_hz(by, q, n1, n2);
+            bzp = mx_gather_hy(bz, q, n1, n2);
+        }
         const double py = pp.x, pz = pp.y;
         const double a = ga.v, ps = gp.v;
         const double inv1ps = 1.0 / (1 + ps);  // the FP64 1/(1+psi) factor

@@ -156,3 +156,42 @@ def test_mixed_three_two_grid_matches_oracle(qp_ctx, oracle):
     wk.close()
     assert st0 == st1 == 0
     check(got, want)
+
+
+@pytest.mark.skip(reason="the tuner-chosen gather mode (qp_wake_set_gather_window) was taken out: "
+                         "its first GPU run faulted the GPU and the cause was not found")
+@pytest.mark.parametrize("a0", [2.0, 5.0])
+def test_gather_modes_agree(qp_ctx, oracle, a0):
+    """The predictor's gathers from the block's LDS-staged grid window (mode 1)
+    and from global memory (mode 0) give the same pass: both against the oracle,
+    and each other to the rounding of the deposits' atomic order.  a0 = 5 moves
+    particles far enough that some blocks' windows exceed the staging capacity
+    and fall back to global gathers inside mode 1."""
+    from quetzalpic_amd import capi
+    g = oracle.grid_from_box(5.0, 20.0, 20.0, 0.05, 20 / 64, 20 / 64, K=5)
+    y, z, w = oracle.init_lattice(g, 2, 2)
+    a = oracle.gaussian_a_sqr(g, a0, 4.243, 8.0, 8.0, 5.0 - 3.0, 10.0, 10.0)
+    fields = ("psi", "ex", "ey", "by", "jz")
+    want, st0, _ = oracle.wake_run(g, y, z, w, a, fields=fields)
+    got = {}
+    for mode in (1, 0):
+        wk = capi.Wake(qp_ctx, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ly, g.lz, K=g.K,
+                       precision=capi.QP_PREC_MIXED)
+        wk.set_particles(y, z, w)
+        wk.set_a_sqr(a)
+        wk.set_gather_window(mode)
+        got[mode], st, _ = wk.run(fields=fields)
+        assert st == st0
+        assert json.loads(wk.describe())["gather_window"] == mode
+        wk.close()
+        for f in ("psi", "ex"):
+            mx, rms = rel_err(got[mode][f], want[f])
+            assert mx < TOL_PSI_EX and rms < TOL_PSI_EX, (mode, f, mx, rms)
+    for f in fields:
+        assert rel_err(got[1][f], got[0][f])[0] < 1e-9, f
+    with pytest.raises(capi.QpError, match="mode"):
+        wk = capi.Wake(qp_ctx, g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.ly, g.lz, K=g.K, precision=capi.QP_PREC_MIXED)
+        try:
+            wk.set_gather_window(2)
+        finally:
+            wk.close()
