@@ -473,7 +473,7 @@ class C4Job:
     so the groups' dependency chains overlap.  chains = 1: all 64 wakes in one
     launch per phase; chains = 5: one batch per stream."""
 
-    def __init__(self, capi, workloads, use_graph=1, only_class=0, chains=1):
+    def __init__(self, capi, workloads, use_graph=1, only_class=0, chains=1, slice_limit=0):
         members = workloads.c4_members()
         if only_class:  # one grid-size class only (profiling runs)
             members = [m for m in members if m.ny == only_class]
@@ -496,7 +496,7 @@ class C4Job:
                 ms = self.classes[n]
                 m0 = ms[0]
                 b = capi.WakeBatch(ctx, len(ms), m0.nx, m0.ny, m0.nz, m0.dx, m0.dy, m0.dz, m0.L, m0.L, K=m0.K,
-                                   use_graph=use_graph)
+                                   use_graph=use_graph, slice_limit=slice_limit)
                 for i, m in enumerate(ms):
                     y, z, w, py, pz = m.particles()
                     b.set_particles(i, y, z, w, py, pz)
@@ -504,10 +504,13 @@ class C4Job:
                 self.batches[n] = (ctx, b, sum(m.n_particles for m in ms))
                 log(f"[bench] C4 class {n}x{n}: {len(ms)} members, {self.batches[n][2]} particles")
             if len(chain) > 1:
-                self.groups.append(capi.WakeGroup(ctx, [self.batches[n][1] for n in chain]))
+                g = capi.WakeGroup(ctx, [self.batches[n][1] for n in chain])
+                g.set_fork(chains == 1)   # a second stream per group only pays when the group runs alone
+                self.groups.append(g)
             else:
                 self.groups.append(self.batches[chain[0]][1])
-        self.pushes = sum(m.n_particles * (m.nx - 1) for m in members)
+        slices = slice_limit if 0 < slice_limit < members[0].nx else members[0].nx
+        self.pushes = sum(m.n_particles * (slices - 1) for m in members)
 
     def describe(self):
         return {"members": {f"{n}x{n}": len(self.classes[n]) for n in self.sizes},
@@ -569,7 +572,8 @@ def c4_pass(capi, workloads, steps, warmup=1, chains=C4_CHAINS):
 
 def bench_c4(args, capi, workloads):
     """value = all members' particle-slice pushes of a pass / wall time."""
-    job = C4Job(capi, workloads, use_graph=0 if args.no_graph else 1, only_class=args.c4_class, chains=args.c4_chains)
+    job = C4Job(capi, workloads, use_graph=0 if args.no_graph else 1, only_class=args.c4_class, chains=args.c4_chains,
+                slice_limit=args.slice_limit)
     for _ in range(args.warmup):
         job.run()
     job.synchronize()

@@ -223,6 +223,12 @@ int qp_group_last_time_ms(qp_group g, double *ms);
 /* Particle tile of every launch of the group (as qp_wake_set_tile; by default
  * the group's first run tunes one for all its batches together). */
 int qp_group_set_tile(qp_group g, int threads_per_block, int window_nodes);
+/* Solver passes of a group holding both transform families (sides 2^p and
+ * 3 * 2^p): nonzero (the default) runs the two families' launches on two
+ * streams at once (a fork and join inside the slice step); 0 runs them in
+ * sequence on the context's stream, which is faster when several groups run
+ * concurrently on one device.  Results are the same either way. */
+int qp_group_set_fork(qp_group g, int fork_families);
 
 /* ---- Laser envelope (no reference counterpart) ------------------------------
  * The reference keeps <a^2> fixed for the whole run (src/System_3d.cpp:805-816

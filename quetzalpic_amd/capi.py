@@ -96,6 +96,7 @@ def load():
         "qp_group_run": ([vp, GROUP_CB, vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "qp_group_last_time_ms": ([vp, _dp], ctypes.c_int),
         "qp_group_set_tile": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "qp_group_set_fork": ([vp, ctypes.c_int], ctypes.c_int),
         "qp_wake_set_tile": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "qp_envelope_create": ([vp] + [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.POINTER(vp)],
                                ctypes.c_int),
@@ -513,12 +514,12 @@ class WakeBatch:
     member has its own particles, Gaussian laser, fields and stop rule."""
 
     def __init__(self, ctx, nwakes, nx, ny, nz, dx, dy, dz, ly, lz, K=5, D=10.0, psi_threshold=1e-4,
-                 b_threshold=100.0, use_graph=1, longitudinal_b=0, async_output=0):
+                 b_threshold=100.0, use_graph=1, longitudinal_b=0, async_output=0, slice_limit=0):
         self.ctx = ctx
         self.nwakes = nwakes
         self.shape = (nx, ny, nz)
         self.params = WakeParams(nx, ny, nz, dx, dy, dz, ly, lz, K, D, psi_threshold, b_threshold,
-                                 QP_PREC_MIXED, use_graph, 0, longitudinal_b, async_output, 0)
+                                 QP_PREC_MIXED, use_graph, 0, longitudinal_b, async_output, slice_limit)
         self.h = ctypes.c_void_p()
         _check(load().qp_wake_create_batch(ctx.h, ctypes.byref(self.params), nwakes, ctypes.byref(self.h)),
                "qp_wake_create_batch")
@@ -674,6 +675,11 @@ class WakeGroup:
     def set_tile(self, threads_per_block, window_nodes):
         """Force the particle tile of every group launch ((0, 0): tune again)."""
         _check(load().qp_group_set_tile(self.h, threads_per_block, window_nodes), "qp_group_set_tile")
+
+    def set_fork(self, fork_families):
+        """Run the two transform families' solver launches on two streams (True, the
+        default) or in sequence on the context's stream (False)."""
+        _check(load().qp_group_set_fork(self.h, 1 if fork_families else 0), "qp_group_set_fork")
 
     def close(self):
         if self.h:

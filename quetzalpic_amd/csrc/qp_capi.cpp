@@ -530,6 +530,12 @@ int qp_group_set_tile(qp_group h, int threads_per_block, int window_nodes) {
     return QP_OK;
 }
 
+int qp_group_set_fork(qp_group h, int fork_families) {
+    ARG_CHECK(h, "null group");
+    h->g.fork_families = fork_families != 0;
+    return QP_OK;
+}
+
 int qp_wake_describe(qp_wake h, char *buf, size_t len) {
     ARG_CHECK(h && buf && len > 0, "null argument");
     const Wake *w = &h->w;

@@ -145,6 +145,7 @@ struct Group {
     void *dev = nullptr;   // launch tables and graphs (owned by qp_wake_mixed.hip)
     int tile_pt = 256, tile_cap = 640;   // particle tile of every batch in the group's launches
     bool tile_tuned = false;
+    bool fork_families = true;   // the two transform families' solver launches on two streams
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
 };

@@ -2089,6 +2089,11 @@ struct GroupMaps {
 };
 
 struct GroupState {
+    // the second transform family's launches run on this stream, forked from and
+    // joined back into the group's (a parallel branch of the captured step)
+    hipStream_t side = nullptr;
+    std::vector<hipEvent_t> ev;   // fork / join pairs of one slice step
+    size_t ev_next = 0;
     GroupTabs *d_tabs = nullptr;
     GroupTabs h_tabs;
     GroupMaps maps{}, graph_maps{};
@@ -2170,16 +2175,53 @@ void group_tables(Group *G, std::vector<MxRun> &R, GroupTabs &T, GroupMaps &M) {
     } while (0)
 // One solver pass of every batch: one launch per transform family present
 // (GSOLVE_T for the kernels templated on their source / sink type).
-#define GSOLVE(kern, maps, nspec, tab)                                                              \
-    do {                                                                                            \
-        GLAUNCH((kern<0>), maps[0], kGrpTB, sizeof(double2) * 2 * grp_lds(0), tab, nspec, nspec);   \
-        GLAUNCH((kern<1>), maps[1], kGrpTB, sizeof(double2) * 2 * grp_lds(1), tab, nspec, nspec);   \
-    } while (0)
-#define GSOLVE_T(kern, T, maps, nspec, tab)                                                           \
-    do {                                                                                              \
-        GLAUNCH((kern<T, 0>), maps[0], kGrpTB, sizeof(double2) * 2 * grp_lds(0), tab, nspec, nspec);  \
-        GLAUNCH((kern<T, 1>), maps[1], kGrpTB, sizeof(double2) * 2 * grp_lds(1), tab, nspec, nspec);  \
-    } while (0)
+// The two transform families of one solver pass are independent launches with
+// few blocks each (latency-bound): with both present and Group::fork_families
+// set, the 3 * 2^p family's runs on the side stream beside the 2^p family's.
+// The 64-wake group of five grid sizes: 2.50 -> 2.40 ms per slice.  With three
+// such groups on their own streams the extra stream was slower (1.99 -> 2.32 ms
+// per slice of all three), so callers running several groups at once switch it off.
+template <class L0, class L1>
+int fam_pair(GroupState *st, hipStream_t s, bool both, L0 &&l0, L1 &&l1) {
+    if (!both) {
+        QP_TRY(l0(s));
+        return l1(s);
+    }
+    if (st->ev_next + 2 > st->ev.size()) {
+        set_error("group: fork events exhausted");
+        return QP_ERR_HIP;
+    }
+    hipEvent_t ef = st->ev[st->ev_next++], ej = st->ev[st->ev_next++];
+    QP_HIP_CHECK(hipEventRecord(ef, s));
+    QP_HIP_CHECK(hipStreamWaitEvent(st->side, ef, 0));
+    QP_TRY(l1(st->side));
+    QP_HIP_CHECK(hipEventRecord(ej, st->side));
+    QP_TRY(l0(s));
+    QP_HIP_CHECK(hipStreamWaitEvent(s, ej, 0));
+    return QP_OK;
+}
+#define GSOLVE(kern, maps, nspec, tab)                                                                     \
+    QP_TRY(fam_pair(                                                                                       \
+        st, s, G->fork_families && (maps)[0].nb > 0 && (maps)[1].nb > 0,                                   \
+        [&](hipStream_t s) {                                                                               \
+            GLAUNCH((kern<0>), maps[0], kGrpTB, sizeof(double2) * 2 * grp_lds(0), tab, nspec, nspec);      \
+            return QP_OK;                                                                                  \
+        },                                                                                                 \
+        [&](hipStream_t s) {                                                                               \
+            GLAUNCH((kern<1>), maps[1], kGrpTB, sizeof(double2) * 2 * grp_lds(1), tab, nspec, nspec);      \
+            return QP_OK;                                                                                  \
+        }))
+#define GSOLVE_T(kern, T, maps, nspec, tab)                                                                \
+    QP_TRY(fam_pair(                                                                                       \
+        st, s, G->fork_families && (maps)[0].nb > 0 && (maps)[1].nb > 0,                                   \
+        [&](hipStream_t s) {                                                                               \
+            GLAUNCH((kern<T, 0>), maps[0], kGrpTB, sizeof(double2) * 2 * grp_lds(0), tab, nspec, nspec);   \
+            return QP_OK;                                                                                  \
+        },                                                                                                 \
+        [&](hipStream_t s) {                                                                               \
+            GLAUNCH((kern<T, 1>), maps[1], kGrpTB, sizeof(double2) * 2 * grp_lds(1), tab, nspec, nspec);   \
+            return QP_OK;                                                                                  \
+        }))
 
 // One slice step of every batch: enqueue_slice with one launch per phase.
 template <int PT, int CAP>
@@ -2187,6 +2229,7 @@ int enqueue_group_t(Group *G, GroupState *st, hipStream_t s, int i) {
     const qp_wake_params &p = G->b[0]->p;
     const GroupMaps &M = st->maps;
     GroupTabs *T = st->d_tabs;
+    st->ev_next = 0;
     const int slot = i & 1;
     const bool first = (i == p.nx - 1);
     const bool s_phase = (i == 0) || (p.K == 0);
@@ -2283,6 +2326,9 @@ void group_destroy_mixed(Group *G) {
     for (auto &g : st->gexec)
         if (g) (void)hipGraphExecDestroy(g);
     if (st->d_tabs) (void)hipFree(st->d_tabs);
+    for (hipEvent_t e : st->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (st->side) (void)hipStreamDestroy(st->side);
     delete st;
     G->dev = nullptr;
 }
@@ -2328,6 +2374,13 @@ int group_run_mixed(Group *G, group_cb cb, void *user, int *stop_slices) {
         st = new GroupState();
         G->dev = st;
         QP_HIP_CHECK(hipMalloc(&st->d_tabs, sizeof(GroupTabs)));
+        QP_HIP_CHECK(hipStreamCreateWithFlags(&st->side, hipStreamNonBlocking));
+    }
+    // fork / join events: two per solver pass of one slice step (3 + 6 K passes)
+    while (st->ev.size() < (size_t)(2 * (3 + 6 * G->b[0]->p.K))) {
+        hipEvent_t e = nullptr;
+        QP_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        st->ev.push_back(e);
     }
     std::vector<MxRun> R(nb);
     for (int b = 0; b < nb; ++b) QP_TRY(pass_setup(G->b[b], R[b]));
@@ -2337,6 +2390,7 @@ int group_run_mixed(Group *G, group_cb cb, void *user, int *stop_slices) {
     std::vector<unsigned> epochs(nb);
     for (int b = 0; b < nb; ++b) epochs[b] = G->b[b]->inputs_epoch;
     epochs.push_back((unsigned)G->tile_cap);   // the maps carry the block counts of the tile, not its capacity
+    epochs.push_back(G->fork_families ? 1u : 0u);
     if (std::memcmp(&st->maps, &st->graph_maps, sizeof(GroupMaps)) != 0 || epochs != st->graph_epochs) {
         for (auto &g : st->gexec)
             if (g) (void)hipGraphExecDestroy(g);

@@ -130,6 +130,39 @@ def test_group_graph_step_matches(qp_ctx, oracle):
         b.close()
 
 
+def test_group_fork_switch(qp_ctx, oracle):
+    """qp_group_set_fork(0) runs the two transform families in sequence on one
+    stream: the same fields and particles as the forked default (with callback),
+    and after switching back the recaptured graph step equals the batches alone."""
+    from quetzalpic_amd import capi
+    grids, inps, batches = build(qp_ctx, oracle, LAYOUT)
+    grp = capi.WakeGroup(qp_ctx, batches)
+    forked, st0, _ = grp.run(fields=FIELDS)
+    grp.set_fork(False)
+    seq, st1, _ = grp.run(fields=FIELDS)
+    assert st0 == 0 and st1 == 0
+    for bi, b in enumerate(batches):
+        for m in range(b.nwakes):
+            for f in FIELDS:
+                e = rel_err(seq[bi][m][f], forked[bi][m][f])
+                assert e[0] < 1e-9, (LAYOUT[bi], m, f, e)
+    _, st2, _ = grp.run()   # graph step without the fork
+    grp.set_fork(True)
+    _, st3, _ = grp.run()   # recaptured with it
+    assert st2 == 0 and st3 == 0
+    got = [[b.get_particles(m) for m in range(b.nwakes)] for b in batches]
+    grp.close()
+    for bi, b in enumerate(batches):
+        _, st4, _ = b.run()
+        assert st4 == 0
+        for m in range(b.nwakes):
+            want = b.get_particles(m)
+            for k in range(4):
+                d = np.abs(got[bi][m][k] - want[k]).max()
+                assert d < 1e-9 * max(1.0, np.abs(want[k]).max()), (bi, m, k, d)
+        b.close()
+
+
 def test_group_argument_errors(qp_ctx, oracle):
     from quetzalpic_amd import capi
     g = grid(oracle, 40)   # generic FFT size: not a group side
