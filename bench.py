@@ -627,7 +627,8 @@ def bench_c4(args, capi, workloads):
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "C4: 64 wakes over [-12,12]^2, grids from {64,96,128,192,256} (one batch per size, "
-                               "batches on concurrent streams), 4 ppc, 800 slices dxi=0.025, a0~U[0.5,3], "
+                               "the batches dealt into the groups of `chains`, each group on its own stream), "
+                               "4 ppc, 800 slices dxi=0.025, a0~U[0.5,3], "
                                "w0~U[2,5], sigma_xi=2, transverse density up-ramp slope~U[0,0.1]", **desc},
         "classes_alone": alone, "kernel_classes_largest": kc, "roofline": roofline,
         "err_vs_oracle_member": err, "cpu_baseline": cpu,

@@ -929,8 +929,13 @@ struct GArgsRowsI {
     size_t sstride;
 };
 
+// The group kernels read their arguments from the table (not the kernel
+// argument segment): left alone the compiler holds more of them in vector
+// registers than the single-batch kernels do (B 88 VGPRs and 5 waves per SIMD
+// against 70 and 7, A 68 against 50).  The bounds restore 7 and 8 waves at
+// 256 threads without scratch (B 66, A 64 VGPRs).
 template <int PT, int CAP>
-__global__ __launch_bounds__(PT) void kg_mx_A(GrpMap m, const GArgsA *__restrict__ t, int init_rho, int advance) {
+__global__ __launch_bounds__(PT, 8) void kg_mx_A(GrpMap m, const GArgsA *__restrict__ t, int init_rho, int advance) {
     int b, bx, wk;
     grp_locate(m, b, bx, wk);
     const GArgsA &a = t[b];
@@ -938,7 +943,7 @@ __global__ __launch_bounds__(PT) void kg_mx_A(GrpMap m, const GArgsA *__restrict
          m.gx[b]);
 }
 template <int PT, int CAP>
-__global__ __launch_bounds__(PT) void kg_mx_B(GrpMap m, const GArgsB *__restrict__ t, int first_iter, int init_rho) {
+__global__ __launch_bounds__(PT, 7) void kg_mx_B(GrpMap m, const GArgsB *__restrict__ t, int first_iter, int init_rho) {
     int b, bx, wk;
     grp_locate(m, b, bx, wk);
     const GArgsB &a = t[b];
