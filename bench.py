@@ -85,6 +85,23 @@ def solve_bytes(n1, n2):
     return (real + spec) + 2 * spec + (spec + real)
 
 
+def solve_bytes_slice(n1, n2, K, bunch):
+    """Algorithmic bytes of one slice's solves with the sources and sinks fused
+    into their first and last launches: 1 + K psi solves (source: the deposit
+    accumulator and the ion background, plus the bunch's rho and j_x planes when
+    a bunch drives the wake; src/System_3d.cpp:342-365, 437-469) and K B_y / B_z
+    pair solves (sources from B_y, B_z, j_x, j_y, j_z and the previous j_y, j_z;
+    :529-564; results B_y, B_z).  Every plane read once, each half spectrum
+    written, read and written by the column pass and read once, each result
+    written once."""
+    nh = n2 // 2 + 1
+    real = 8.0 * n1 * n2
+    spec = 16.0 * n1 * nh
+    psi = (4 if bunch else 2) * real + spec + 2 * spec + spec + real
+    bpair = 7 * real + 2 * spec + 4 * spec + 2 * spec + 2 * real
+    return (1 + K) * psi + K * bpair
+
+
 def read_traffic(kernel_cls, workload):
     """Per-launch HBM bytes of a kernel from the committed rocprofv3 --pmc summary
     (profiles/traffic_<class>_<workload>.txt: lines 'FETCH_KIB <v>', 'WRITE_KIB <v>',
@@ -215,15 +232,23 @@ def roofline_of(classes, K, n_particles, workload):
             "frac_own_layout": round(own / t / 1e9 / HBM_PEAK_GBS, 4), "mean_us": classes[dom]["mean_us"]}
 
 
-def solve_roofline(classes, n1, n2):
-    """Bytes per transform launch (a third of a three-launch solve) / mean duration."""
+def solve_roofline(classes, n1, n2, K, bunch=False):
+    """Bytes per transform launch (a third of a three-launch solve) / mean
+    duration; beside it the whole solver class of a slice (3 + 6K launches) with
+    its fused sources and sinks (solve_bytes_slice) / the class's time per slice."""
     if "solve" not in classes:
         return None
+    t = classes["solve"]["mean_us"] * 1e-6
     sb = solve_bytes(n1, n2) / 3.0
-    ach = sb / (classes["solve"]["mean_us"] * 1e-6) / 1e9
+    ach = sb / t / 1e9
+    slice_b = solve_bytes_slice(n1, n2, K, bunch)
+    slice_t = (3 + 6 * K) * t
+    ach_s = slice_b / slice_t / 1e9
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": sb,
-            "mean_us": classes["solve"]["mean_us"], "launches": classes["solve"]["launches"]}
+            "mean_us": classes["solve"]["mean_us"], "launches": classes["solve"]["launches"],
+            "with_sources": {"bytes_per_slice": slice_b, "us_per_slice": round(slice_t * 1e6, 1),
+                             "achieved": round(ach_s, 1), "frac": round(ach_s / HBM_PEAK_GBS, 4)}}
 
 
 def err_pair(got, ref):
@@ -332,7 +357,7 @@ def bench_c5(args, capi, workloads):
         db.set_particles(b0)
         classes = class_profile(wk, ("push_A", "push_B", "push_C", "solve"))
         roofline = roofline_of(classes, wl.K, wl.n_particles, "C5")
-        sroof = solve_roofline(classes, wl.ny, wl.nz)
+        sroof = solve_roofline(classes, wl.ny, wl.nz, wl.K, bunch=True)
         log(f"[bench] roofline {roofline}")
     wk.close()
     err64 = fp64_rate = None
@@ -667,7 +692,7 @@ def bench_single(args, capi, workloads):
     if not args.no_profile:
         classes = class_profile(wk, ("push_A", "push_B", "push_C", "solve"))
         roofline = roofline_of(classes, wl.K, wl.n_particles, args.workload)
-        sroof = solve_roofline(classes, wl.ny, wl.nz)
+        sroof = solve_roofline(classes, wl.ny, wl.nz, wl.K)
     wk.close()
     err64 = fp64_rate = None
     if not args.no_fp64:
