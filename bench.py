@@ -378,7 +378,15 @@ def bench_c5(args, capi, workloads):
         cpu, oref = cpu_baseline(wl, S, bunch=b0)
         if mixed_lead is not None:
             k = (S + 2) // 3   # the kept slices among the first S (every 3rd)
-            err_cpu = {f: err_pair(mixed_lead[f][:k], oref[f][::3][:k]) for f in ("psi", "ex")}
+            ref = {f: oref[f][::3][:k] for f in ("psi", "ex")}
+            if all(np.abs(ref[f]).max() == 0 and np.abs(mixed_lead[f][:k]).max() == 0 for f in ref):
+                # the sample's slices lie ahead of the bunch's head (and of any
+                # laser): the fields are zero on both sides and a relative error
+                # says nothing; the oracle comparison through the bunch is the
+                # C5 parity test (tests/test_gpu_configs.py)
+                err_cpu = {"fields_zero_on_both_sides": True}
+            else:
+                err_cpu = {f: err_pair(mixed_lead[f][:k], ref[f]) for f in ref}
         log(f"[bench] cpu baseline: {cpu['value']:.3e} pushes/s on {cpu['cores']} threads ({cpu['cpu_model']}); "
             f"err vs oracle {err_cpu}")
     windows = None if args.no_profile else window_report(capi, ctx, wl, bunch=db, bunch_particles=b0)
