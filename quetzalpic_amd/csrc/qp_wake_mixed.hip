@@ -846,9 +846,9 @@ constexpr int kGroupMax = 16;
 constexpr int kGrpTB = 64;
 // LDS of the two transform families per 64-thread block: T * PADN = 64 E / N *
 // 17 N / 16 = 68 E complex values per exchange buffer (E = 4 for sides 2^p,
-// 12 for 3 * 2^p).  The families launch separately: one kernel holding both
+// kWfE3 for 3 * 2^p).  The families launch separately: one kernel holding both
 // would give every block the registers and LDS of the 3 * 2^p transforms.
-constexpr int grp_lds(int fam) { return 68 * (fam ? 12 : 4); }
+constexpr int grp_lds(int fam) { return 68 * (fam ? kWfE3 : 4); }
 
 struct GrpMap {
     int nb;                      // entries
@@ -986,7 +986,7 @@ inline bool grp_side_ok(int n) {
 #define QP_GRP_SIDES_P2(F) F(16) F(32) F(64) F(128) F(256)
 #define QP_GRP_SIDES_P3(F) F(48) F(96) F(192)
 // Elements per lane of a side's transform (WfCfg<N>::E at run time).
-inline int grp_elems(int n) { return n % 3 == 0 ? 12 : 4; }
+inline int grp_elems(int n) { return n % 3 == 0 ? kWfE3 : 4; }
 
 template <class Src, int FAM>
 __global__ __launch_bounds__(kGrpTB) void kg_rows_fwd(GrpMap m, const GArgsRowsF<Src> *__restrict__ t, int skip_mode,
@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(kGrpTB) void kg_cols(GrpMap m, const GArgsCols *__r
     int b, bx, by;
     grp_locate(m, b, bx, by);
     const GArgsCols &a = t[b];
-    const int tpb = kGrpTB * (FAM ? 12 : 4) / a.n1;   // transforms per block
+    const int tpb = kGrpTB * (FAM ? kWfE3 : 4) / a.n1;   // transforms per block
     const int gx = (a.nh + tpb - 1) / tpb;
     double2 *sb0 = gl2, *sb1 = gl2 + grp_lds(FAM);
 #define QP_CASE(N)                                                                                                  \

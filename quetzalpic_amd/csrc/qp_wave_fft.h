@@ -1,7 +1,7 @@
 // qp_wave_fft.h -- register-resident complex FFTs of length N = 2^p in
 // [16, 1024] or N = 3 * 2^p in [48, 384] (device code).
 //
-// E elements per lane (E in {4, 8, 16} for 2^p, 12 for 3 * 2^p), L = N/E lanes
+// E elements per lane (E in {4, 8, 16} for 2^p, 6 or 12 for 3 * 2^p), L = N/E lanes
 // per transform, a block of TB threads carries T = TB/L transforms.  Lane j of a transform
 // holds element j + L*m in register m, on entry and on exit (natural order
 // both ways), so a forward transform, a pointwise operation and an inverse
@@ -10,12 +10,12 @@
 // and radix R = min(E, N/Ns) has butterflies jj = j + L b (b < E/R) reading
 // in[jj + r N/R], twiddled by w^{k r N/(Ns R)} (k = jj mod Ns), writing
 // out[(jj/Ns) Ns R + k + q Ns]; the last pass lands in registers.  Passes
-// exchange through two LDS buffers (one barrier per pass).  For E = 12 the
-// first pass has radix 12 (3 x 4) and the later radices are 4 or 2, each
-// dividing E (wf_radix).  Twiddles come from the axis table e^{-2 pi i t/N},
-// t < N, in device memory: with E = 4 each lane loads the ones it applies into
-// registers at kernel start, and so with E = 12 up to N = 192; at N = 384
-// the block stages the table in LDS (WfTw).
+// exchange through two LDS buffers (one barrier per pass).  For 3 * 2^p the
+// first pass has radix E, 6 (3 x 2) or 12 (3 x 4), and the later radices are 4
+// or 2, each dividing E (wf_radix).  Twiddles come from the axis table
+// e^{-2 pi i t/N}, t < N, in device memory: with E = 4 each lane loads the ones
+// it applies into registers at kernel start, and so for 3 * 2^p up to N = 192;
+// at N = 384 the block stages the table in LDS (WfTw).
 #pragma once
 
 #include <type_traits>
@@ -33,8 +33,8 @@ constexpr bool wf_length(int n) {
 template <int N, int E, int TB>
 struct WfShape {
     static_assert(wf_length(N), "wave FFT length must be 2^p in [16, 1024] or 3 * 2^p in [48, 384]");
-    static_assert(N % 3 == 0 ? E == 12 : (E == 4 || E == 8 || E == 16),
-                  "elements per lane must be 4, 8 or 16 (12 for lengths 3 * 2^p)");
+    static_assert(N % 3 == 0 ? (E == 6 || E == 12) : (E == 4 || E == 8 || E == 16),
+                  "elements per lane must be 4, 8 or 16 (6 or 12 for lengths 3 * 2^p)");
     static constexpr int L = N / E;                      // lanes per transform
     static_assert(TB % 64 == 0 && TB >= L && TB % L == 0, "block must hold whole transforms");
     static constexpr int T = TB / L;                     // transforms per block
@@ -160,14 +160,27 @@ __device__ __forceinline__ void dft4(double2 &v0, double2 &v1, double2 &v2, doub
     v3 = csub(t1, t3);
 }
 
-// In-register DFT of length R in {1, 2, 3, 4, 8, 12, 16}: u[q] <- sum_r u[r] e^{S 2 pi i r q / R}.
+// In-register DFT of length R in {1, 2, 3, 4, 6, 8, 12, 16}: u[q] <- sum_r u[r] e^{S 2 pi i r q / R}.
 // Radix 8 and 16 split as r = r1 + (R/4 or 4) r2 with internal twiddles; radix
 // 12 as r = r1 + 4 r2, q = q2 + 3 q1: length-3 DFTs over r2, twiddles
-// e^{S 2 pi i r1 q2 / 12}, length-4 DFTs over r1.
+// e^{S 2 pi i r1 q2 / 12}, length-4 DFTs over r1; radix 6 the same way with
+// r = r1 + 2 r2 and length-2 DFTs over r1.
 template <int R, int S>
 __device__ __forceinline__ void dft(double2 (&u)[R]) {
     if constexpr (R == 3) {
         dft3<S>(u[0], u[1], u[2]);
+    } else if constexpr (R == 6) {
+        double2 a0 = u[0], a1 = u[2], a2 = u[4], b0 = u[1], b1 = u[3], b2 = u[5];
+        dft3<S>(a0, a1, a2);
+        dft3<S>(b0, b1, b2);
+        b1 = w12<2, S>(b1);  // e^{S 2 pi i q2 / 6} for r1 = 1
+        b2 = w12<4, S>(b2);
+        u[0] = cadd(a0, b0);
+        u[3] = csub(a0, b0);
+        u[1] = cadd(a1, b1);
+        u[4] = csub(a1, b1);
+        u[2] = cadd(a2, b2);
+        u[5] = csub(a2, b2);
     } else if constexpr (R == 12) {
         double2 a[4][3];
 #pragma unroll
@@ -264,13 +277,13 @@ __device__ __forceinline__ void wf_sync() {
 }
 
 // Radix of the pass after sub-transforms of size NS: for 2^p lengths the
-// remaining length capped at E; for E = 12 the remaining length when it divides
-// E (the last pass), else 4 or 2 (every radix must divide E).
+// remaining length capped at E; for 3 * 2^p (E = 6 or 12) the remaining length
+// when it divides E (the last pass), else 4 or 2 (every radix must divide E).
 constexpr int wf_radix(int N, int E, int NS) {
     const int rem = N / NS;
     if (E % 3 != 0) return rem < E ? rem : E;
     if (rem <= E && E % rem == 0) return rem;
-    return rem % 4 == 0 ? 4 : 2;
+    return (rem % 4 == 0 && E % 4 == 0) ? 4 : 2;
 }
 
 // Twiddles one lane applies in the passes after the first (sub-transforms of
@@ -300,8 +313,8 @@ __device__ __forceinline__ void wf_tw_load(double2 (&t)[NT], const double2 *__re
 // Twiddle source of wave_fft.  Every lane loads the twiddles it applies into
 // registers at kernel start, next to its data loads: no LDS table and no
 // barrier to publish it.  That holds 3 per pass with 4 elements per lane, and
-// up to 18 in all for lengths 3 * 2^p <= 192 (12 per lane); for 384 (24) the
-// block stages the N-entry table in LDS instead.
+// up to 15 in all for lengths 3 * 2^p <= 192 with 6 per lane (18 with 12); for
+// 384 the block stages the N-entry table in LDS instead.
 template <int N, int E, int TB>
 struct WfTw {
     static constexpr bool kRegs = E == 4 || N <= 192;

@@ -24,9 +24,14 @@ namespace qp {
 // contiguous instead of 32 B (a 1024^2 x 4000-slice beam-driven pass 5.86 ->
 // 5.74 s); the same at 512 (four transforms) was slower (a 512^2 x 2000-slice
 // pass 665 -> 733 ms).
+// Elements per lane of the 3 * 2^p lengths: 6 (a radix-6 first pass, then
+// radix 2) gives twice the wavefronts of 12 (radix 12, then 4) at half the work
+// per lane; these launches have few blocks, so more, shorter waves win (the
+// 64-wake C4 pass in three chains 1555 -> 1480 ms, as one group 1808 -> 1779 ms).
+constexpr int kWfE3 = 6;
 template <int N>
 struct WfCfg {
-    static constexpr int E = N % 3 == 0 ? 12 : 4;  // 3 * 2^p lengths: radix-12 first pass
+    static constexpr int E = N % 3 == 0 ? kWfE3 : 4;
     static constexpr int TB = N == 1024 ? 512 : (N == 512 ? 256 : (N == 256 ? 128 : 64));
     // Column pass of the wake solves: one transform per block (columns are
     // contiguous, so the row passes' reason for several does not apply; more

@@ -59,7 +59,7 @@ def test_solve2d_single_mode_closed_form(qp_ctx):
 @pytest.mark.parametrize("n1,n2", [(16, 16), (32, 64), (256, 256), (128, 1024), (1024, 1024), (512, 32),
                                    (48, 48), (96, 192), (192, 192), (384, 96), (48, 384), (96, 256)])
 def test_wave_and_generic_kernels_agree(qp_ctx, n1, n2, monkeypatch):
-    """Grids with sides 2^p or 3 * 2^p (radix-12 first pass) use the register-
+    """Grids with sides 2^p or 3 * 2^p (radix-6 first pass) use the register-
     resident kernels; QP_FFT=generic forces the generic LDS kernels on the same
     grid.  Both must give the same solve."""
     from quetzalpic_amd import capi

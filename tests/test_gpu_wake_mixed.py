@@ -148,7 +148,7 @@ def test_mixed_stopped_pass_matches_oracle(qp_ctx, oracle):
 
 def test_mixed_three_two_grid_matches_oracle(qp_ctx, oracle):
     """A 48 x 48 grid (3 * 2^4) runs the register-resident FFT kernels with a
-    radix-12 first pass."""
+    radix-6 first pass."""
     g, y, z, w, a = case(oracle, n=48, K=5)
     want, st0, _ = oracle.wake_run(g, y, z, w, a)
     wk, got, st1, _ = run_mixed(qp_ctx, g, y, z, w, a)
