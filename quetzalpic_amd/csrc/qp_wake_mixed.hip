@@ -767,42 +767,60 @@ __global__ __launch_bounds__(kSolverThreads) void k_cols_psi(double2 *spec, int 
 }
 
 // ------------------------------------------------------------ re-binning
-// Counting sort of the particles by 16x16-cell tile (stable order is not
-// needed: the deposits are order-independent up to rounding).
+// Counting sort of the particles by cell, the cells taken 16x16 tile by tile
+// and row by row inside a tile: the order of the initial state
+// (build_initial_state), so a re-binned block again covers a few rows of one
+// tile and the lanes of a quad again share cells.  Particles of one cell come
+// out in any order (the deposits are order-independent up to rounding).
 constexpr int kTile = 16;
+constexpr int kTileBins = kTile * kTile;
+__device__ __forceinline__ int sort_bin(uint32_t c, int ntk) {
+    const int j = (int)(c >> 16), k = (int)(c & 0xffffu);
+    return ((j / kTile) * ntk + k / kTile) * kTileBins + (j % kTile) * kTile + (k % kTile);
+}
 __global__ void k_sort_hist(MxPart P, unsigned *count, int ntk) {
     const int p = blockIdx.x * kPT + threadIdx.x;
     if (p >= P.np) return;
-    const uint32_t c = P.cell[p];
-    const int t = (int)((c >> 16) / kTile) * ntk + (int)((c & 0xffffu) / kTile);
-    atomicAdd(&count[t], 1u);
+    atomicAdd(&count[sort_bin(P.cell[p], ntk)], 1u);
 }
 
-__global__ void k_sort_scan(unsigned *count, unsigned *cursor, int ntiles) {
-    // single block exclusive scan; count becomes the offsets
-    __shared__ unsigned part[1024];
-    const int per = (ntiles + blockDim.x - 1) / blockDim.x;
-    const int b = threadIdx.x * per;
-    unsigned s = 0;
-    for (int i = b; i < b + per && i < ntiles; ++i) s += count[i];
-    part[threadIdx.x] = s;
+// Exclusive scan of the bin counts: every block scans a chunk of kScanChunk
+// bins in place (kScanPer consecutive bins per thread), clears their cursors
+// (when given) and writes the chunk's total; the same kernel on one block
+// scans the totals; k_scan_add then adds each chunk's offset.
+constexpr int kScanTB = 1024, kScanPer = 4, kScanChunk = kScanTB * kScanPer;
+__global__ __launch_bounds__(kScanTB) void k_scan_chunks(unsigned *a, unsigned *cursor, unsigned *totals, int n) {
+    __shared__ unsigned wsum[kScanTB / 64];
+    const int base = blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+    unsigned v[kScanPer], s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanPer; ++i) {
+        v[i] = base + i < n ? a[base + i] : 0u;
+        s += v[i];
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned x = s;  // inclusive scan over the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned run = 0;
-        for (int i = 0; i < (int)blockDim.x; ++i) {
-            const unsigned v = part[i];
-            part[i] = run;
-            run += v;
+    unsigned run = x - s;
+    for (int i = 0; i < wv; ++i) run += wsum[i];
+    if (threadIdx.x == kScanTB - 1) totals[blockIdx.x] = run + s;
+#pragma unroll
+    for (int i = 0; i < kScanPer; ++i) {
+        if (base + i < n) {
+            a[base + i] = run;
+            if (cursor) cursor[base + i] = 0;
         }
+        run += v[i];
     }
-    __syncthreads();
-    unsigned run = part[threadIdx.x];
-    for (int i = b; i < b + per && i < ntiles; ++i) {
-        const unsigned v = count[i];
-        count[i] = run;
-        cursor[i] = 0;
-        run += v;
-    }
+}
+__global__ void k_scan_add(unsigned *count, const unsigned *offs, int n) {
+    const int b = blockIdx.x * kPT + threadIdx.x;
+    if (b < n) count[b] += offs[b / kScanChunk];
 }
 
 __global__ void k_sort_scatter(MxPart P, MxPart Q, const unsigned *offset, unsigned *cursor, const int *perm,
@@ -810,7 +828,7 @@ __global__ void k_sort_scatter(MxPart P, MxPart Q, const unsigned *offset, unsig
     const int p = blockIdx.x * kPT + threadIdx.x;
     if (p >= P.np) return;
     const uint32_t c = P.cell[p];
-    const int t = (int)((c >> 16) / kTile) * ntk + (int)((c & 0xffffu) / kTile);
+    const int t = sort_bin(c, ntk);
     const unsigned d = offset[t] + atomicAdd(&cursor[t], 1u);
     if (d >= (unsigned)P.np) return;
     Q.cell[d] = c;
@@ -1062,7 +1080,7 @@ struct MixedState {
     int *npw = nullptr;     // particles per member (device)
     int pstride = 0;
     bool init_valid = false;
-    unsigned *count = nullptr, *cursor = nullptr;
+    unsigned *count = nullptr, *cursor = nullptr, *totals = nullptr;  // re-binning: per bin, per scan chunk
     double *bpart = nullptr;
     size_t bstride = 0;     // partial sums per member
     double *jbuf[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [parity][y/z], nw planes each
@@ -1119,7 +1137,7 @@ void wake_destroy_mixed(Wake *w) {
     free_part(m->Q);
     free_part(m->I);
     void *arrs[] = {m->perm,       m->perm2,      m->perm0,      m->npw,        m->count,     m->cursor,
-                    m->bpart,      m->jbuf[0][0], m->jbuf[0][1], m->jbuf[1][0], m->jbuf[1][1]};
+                    m->totals,     m->bpart,      m->jbuf[0][0], m->jbuf[0][1], m->jbuf[1][0], m->jbuf[1][1]};
     for (void *a : arrs)
         if (a) (void)hipFree(a);
     for (auto &g : m->gexec)
@@ -1161,8 +1179,10 @@ static int ensure_state(Wake *w) {
         for (int a = 0; a < 2; ++a)
             for (int b = 0; b < 2; ++b) QP_HIP_CHECK(hipMalloc(&m->jbuf[a][b], sizeof(double) * nn * nw));
         const int ntk = (w->p.nz + kTile - 1) / kTile, ntj = (w->p.ny + kTile - 1) / kTile;
-        QP_HIP_CHECK(hipMalloc(&m->count, sizeof(unsigned) * ntk * ntj));
-        QP_HIP_CHECK(hipMalloc(&m->cursor, sizeof(unsigned) * ntk * ntj));
+        const size_t nbins = (size_t)ntk * ntj * kTileBins;
+        QP_HIP_CHECK(hipMalloc(&m->count, sizeof(unsigned) * nbins));
+        QP_HIP_CHECK(hipMalloc(&m->cursor, sizeof(unsigned) * nbins));
+        QP_HIP_CHECK(hipMalloc(&m->totals, sizeof(unsigned) * ((nbins + kScanChunk - 1) / kScanChunk + 1)));
         QP_HIP_CHECK(hipMalloc(&m->npw, sizeof(int) * nw));
     }
     const int pstride = member_stride(w);
@@ -1527,11 +1547,17 @@ int resort(MxRun &R, hipStream_t s) {
     Wake *w = R.w;
     MixedState *m = R.m;
     const int ntk = (w->p.nz + kTile - 1) / kTile, ntj = (w->p.ny + kTile - 1) / kTile;
-    const int ntiles = ntk * ntj;
+    const int nbins = ntk * ntj * kTileBins, nchunks = (nbins + kScanChunk - 1) / kScanChunk;
+    if (nchunks > kScanChunk) {   // the totals are scanned by one block
+        set_error("re-binning: grid too large for the bin scan");
+        return QP_ERR_ARG;
+    }
     const dim3 pg(m->pstride / kPT, 1);
-    QP_HIP_CHECK(hipMemsetAsync(m->count, 0, sizeof(unsigned) * ntiles, s));
+    QP_HIP_CHECK(hipMemsetAsync(m->count, 0, sizeof(unsigned) * nbins, s));
     MLAUNCH(k_sort_hist, pg, kPT, 0, m->P, m->count, ntk);
-    MLAUNCH(k_sort_scan, dim3(1), 1024, 0, m->count, m->cursor, ntiles);
+    MLAUNCH(k_scan_chunks, dim3(nchunks), kScanTB, 0, m->count, m->cursor, m->totals, nbins);
+    MLAUNCH(k_scan_chunks, dim3(1), kScanTB, 0, m->totals, (unsigned *)nullptr, m->totals + nchunks, nchunks);
+    MLAUNCH(k_scan_add, dim3((nbins + kPT - 1) / kPT), kPT, 0, m->count, (const unsigned *)m->totals, nbins);
     MLAUNCH(k_sort_scatter, pg, kPT, 0, m->P, m->Q, (const unsigned *)m->count, m->cursor, (const int *)m->perm,
             m->perm2, ntk);
     MLAUNCH(k_sort_copyback, pg, kPT, 0, m->P, m->Q, m->perm, (const int *)m->perm2);
