@@ -2209,9 +2209,10 @@ void group_tables(Group *G, std::vector<MxRun> &R, GroupTabs &T, GroupMaps &M) {
 // The two transform families of one solver pass are independent launches with
 // few blocks each (latency-bound): with both present and Group::fork_families
 // set, the 3 * 2^p family's runs on the side stream beside the 2^p family's.
-// The 64-wake group of five grid sizes: 2.50 -> 2.40 ms per slice.  With three
-// such groups on their own streams the extra stream was slower (1.99 -> 2.32 ms
-// per slice of all three), so callers running several groups at once switch it off.
+// The 64-wake group of five grid sizes: 2.50 -> 2.40 ms per slice.  With the
+// same wakes in three chains on their own streams (one chain a group of three
+// batches) the extra stream was slower (1.99 -> 2.32 ms per slice), so callers
+// running several groups or batches at once switch it off.
 template <class L0, class L1>
 int fam_pair(GroupState *st, hipStream_t s, bool both, L0 &&l0, L1 &&l1) {
     if (!both) {
